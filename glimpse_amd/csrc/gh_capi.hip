@@ -1,0 +1,222 @@
+// gh_capi.hip -- the C-ABI (include/glimpse_hip.h): argument checks, host<->device
+// staging for the batch entry points, error capture.  No entry point exits.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "gh_device.hpp"
+#include "gh_internal.hpp"
+
+namespace gh {
+void panel_create(const gh_panel_desc *d, gh_panel **out);
+void panel_destroy(gh_panel *p);
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+
+template <typename F>
+static int guard(F &&f) {
+    try {
+        f();
+        return GH_OK;
+    } catch (const Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return GH_ERR_ARG;
+    }
+}
+
+// Device staging of conditioning sets for a batch of explicit state lists.
+struct CondBatch {
+    DBuf items, states, poly, pos, t, hvar, mark;
+    std::vector<CondItem> h;
+    int maxW = 0;
+    int64_t total_words = 0;
+
+    void build(gh_panel *P, int n, const int32_t *state_ptr, const int32_t *states_h, int ne, hipStream_t s) {
+        h.resize(n);
+        const int L = P->L;
+        maxW = 0;
+        int64_t words = 0;
+        for (int i = 0; i < n; i++) {
+            const int K = state_ptr[i + 1] - state_ptr[i];
+            GH_CHECK(K > 0, GH_ERR_STATE, "conditioning set " + std::to_string(i) + " has zero states");
+            GH_CHECK(K <= P->N, GH_ERR_ARG, "more states than panel haplotypes");
+            for (int k = state_ptr[i]; k < state_ptr[i + 1]; k++) {
+                GH_CHECK(states_h[k] >= 0 && states_h[k] < P->N, GH_ERR_ARG, "state id out of range");
+                GH_CHECK(k == state_ptr[i] || states_h[k] > states_h[k - 1], GH_ERR_ARG, "states must be sorted and unique");
+            }
+            CondItem &c = h[i];
+            c.K = K;
+            c.W = (K + 63) / 64;
+            c.Lp = 0;
+            c.pad = 0;
+            c.state_off = state_ptr[i];
+            c.site_off = (int64_t)i * L;
+            c.hvar_off = words;
+            words += (int64_t)L * c.W;
+            maxW = std::max(maxW, c.W);
+        }
+        total_words = words;
+        dalloc_copy(items, h.data(), n, s);
+        dalloc_copy(states, states_h, state_ptr[n], s);
+        poly.reserve(sizeof(int32_t) * (size_t)n * L);
+        pos.reserve(sizeof(int32_t) * (size_t)n * L);
+        t.reserve(sizeof(float) * (size_t)n * L);
+        mark.reserve((size_t)n * L);
+        hvar.reserve(sizeof(uint64_t) * std::max<int64_t>(words, 1));
+        condset_build(P, n, items.as<CondItem>(), h.data(), states.as<int32_t>(), poly.as<int32_t>(),
+                      pos.as<int32_t>(), t.as<float>(), hvar.as<uint64_t>(), mark.as<uint8_t>(), ne, maxW, s);
+        GH_HIP(hipMemcpyAsync(h.data(), items.p, sizeof(CondItem) * n, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+    }
+};
+
+}  // namespace gh
+
+using namespace gh;
+
+extern "C" {
+
+const char *gh_last_error(void) { return g_err.c_str(); }
+
+void gh_default_params(gh_params *p) {
+    // phase/src/caller/caller_parameters.cpp:34-98
+    p->burnin = 5;
+    p->main_iter = 15;
+    p->ne = 100000;
+    p->min_gl = 1e-10f;
+    p->err_imp = 1e-12f;
+    p->err_phase = 0.0001f;
+    p->pbwt_depth = 12;
+    p->pbwt_modulo_cm = 0.1f;
+    p->kinit = 1000;
+    p->kpbwt = 2000;
+    p->seed = 15052011u;
+}
+
+int gh_panel_create(const gh_panel_desc *d, gh_panel **out) {
+    return guard([&] { panel_create(d, out); });
+}
+
+int gh_panel_destroy(gh_panel *p) {
+    return guard([&] {
+        if (!p) return;
+        panel_destroy(p);
+    });
+}
+
+int gh_panel_info(const gh_panel *p, int32_t *n_common, int32_t *n_common_hq, int64_t *device_bytes) {
+    return guard([&] {
+        GH_CHECK(p, GH_ERR_ARG, "null panel");
+        if (n_common) *n_common = p->Lc;
+        if (n_common_hq) *n_common_hq = p->Lhq;
+        if (device_bytes) *device_bytes = p->device_bytes;
+    });
+}
+
+int gh_condset_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int32_t *states, int32_t ne,
+                     int32_t *poly_ptr, int32_t *poly, int64_t poly_cap, int64_t *hvar_ptr, uint64_t *hvar,
+                     int64_t hvar_cap, float *t) {
+    return guard([&] {
+        GH_CHECK(P && n > 0 && state_ptr && states, GH_ERR_ARG, "gh_condset_batch: bad arguments");
+        hipStream_t s = P->stream;
+        CondBatch B;
+        B.build(P, n, state_ptr, states, ne, s);
+        int64_t np = 0, nw = 0;
+        for (int i = 0; i < n; i++) { np += B.h[i].Lp; nw += (int64_t)B.h[i].Lp * B.h[i].W; }
+        GH_CHECK(np <= poly_cap && nw <= hvar_cap, GH_ERR_CAPACITY, "gh_condset_batch: output capacity");
+        int64_t po = 0, wo = 0;
+        for (int i = 0; i < n; i++) {
+            const CondItem &c = B.h[i];
+            poly_ptr[i] = (int32_t)po;
+            hvar_ptr[i] = wo;
+            GH_HIP(hipMemcpyAsync(poly + po, B.poly.as<int32_t>() + c.site_off, sizeof(int32_t) * c.Lp,
+                                  hipMemcpyDeviceToHost, s));
+            if (c.Lp > 1)
+                GH_HIP(hipMemcpyAsync(t + po - i, B.t.as<float>() + c.site_off, sizeof(float) * (c.Lp - 1),
+                                      hipMemcpyDeviceToHost, s));
+            GH_HIP(hipMemcpyAsync(hvar + wo, B.hvar.as<uint64_t>() + c.hvar_off, sizeof(uint64_t) * c.Lp * c.W,
+                                  hipMemcpyDeviceToHost, s));
+            po += c.Lp;
+            wo += (int64_t)c.Lp * c.W;
+        }
+        poly_ptr[n] = (int32_t)po;
+        hvar_ptr[n] = wo;
+        GH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int gh_impute_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int32_t *states, const float *HL,
+                    const uint8_t *flat, float err_imp, int32_t ne, float *HP) {
+    return guard([&] {
+        GH_CHECK(P && n > 0 && state_ptr && states && HL && flat && HP, GH_ERR_ARG, "gh_impute_batch: bad arguments");
+        hipStream_t s = P->stream;
+        const int L = P->L;
+        CondBatch B;
+        B.build(P, n, state_ptr, states, ne, s);
+        std::vector<FbItem> fb(n);
+        int64_t aoff = 0;
+        for (int i = 0; i < n; i++) {
+            fb[i].K = B.h[i].K;
+            fb[i].W = B.h[i].W;
+            fb[i].Lp = B.h[i].Lp;
+            fb[i].cond = i;
+            fb[i].alpha_off = aoff;
+            fb[i].hl_off = (int64_t)i * 2 * L;
+            fb[i].flat_off = (int64_t)i * L;
+            aoff += (int64_t)B.h[i].Lp * 64 * B.h[i].W;
+        }
+        DBuf d_fb, d_HL, d_flat, d_alpha, d_HP;
+        dalloc_copy(d_fb, fb.data(), n, s);
+        dalloc_copy(d_HL, HL, (size_t)n * 2 * L, s);
+        dalloc_copy(d_flat, flat, (size_t)n * L, s);
+        d_alpha.reserve(sizeof(float) * std::max<int64_t>(aoff, 1));
+        d_HP.reserve(sizeof(float) * (size_t)n * 2 * L);
+        const float ei = std::min(std::max(err_imp, 1e-12f), 1e-3f);   // caller_initialise.cpp:152
+        const float ed = ei, ee = (float)(1.0 - (double)ei);
+        fb_launch(P, n, d_fb.as<FbItem>(), B.items.as<CondItem>(), B.maxW, B.poly.as<int32_t>(), B.t.as<float>(),
+                  B.hvar.as<uint64_t>(), d_HL.as<float>(), d_flat.as<uint8_t>(), d_alpha.as<float>(),
+                  d_HP.as<float>(), ee, ed, s);
+        fb_mono_launch(P, n, d_fb.as<FbItem>(), B.items.as<CondItem>(), B.pos.as<int32_t>(), d_HL.as<float>(),
+                       d_flat.as<uint8_t>(), d_HP.as<float>(), ee, ed, s);
+        GH_HIP(hipMemcpyAsync(HP, d_HP.p, sizeof(float) * (size_t)n * 2 * L, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int gh_build_sparse_pbwt(const gh_panel_desc *, uint8_t *, int64_t *, int32_t *, int32_t *, int64_t *, int32_t *) {
+    g_err = "gh_build_sparse_pbwt: not built in this library version";
+    return GH_ERR_ARG;
+}
+
+int gh_rephase_batch(gh_panel *, int32_t, const int32_t *, const int32_t *, const uint8_t *, uint8_t *, uint8_t *,
+                     float, int32_t, uint32_t, int32_t, const int32_t *) {
+    g_err = "gh_rephase_batch: not built in this library version";
+    return GH_ERR_ARG;
+}
+
+int gh_pbwt_select(gh_panel *, int32_t, const uint8_t *, const uint8_t *, const int32_t *, int32_t, float, int32_t *,
+                   int32_t *, int32_t *, int64_t) {
+    g_err = "gh_pbwt_select: not built in this library version";
+    return GH_ERR_ARG;
+}
+
+int gh_phase_run(gh_panel *, int32_t, const uint8_t *, const uint8_t *, const gh_params *, const int32_t *, float *,
+                 float *, uint8_t *, uint8_t *, uint8_t *, int32_t *, gh_run_stats *) {
+    g_err = "gh_phase_run: not built in this library version";
+    return GH_ERR_ARG;
+}
+
+}  // extern "C"
+
+namespace gh {
+void panel_destroy(gh_panel *p) {
+    hipStream_t s = p->stream;
+    delete p;
+    if (s) (void)hipStreamDestroy(s);
+}
+}  // namespace gh

@@ -1,0 +1,84 @@
+// gh_device.hpp -- device-side helpers shared by the HIP kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GH_WAVE 64
+
+namespace gh {
+
+// ---------------------------------------------------------------- DPP / wave
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// Sum of v over the 64 lanes, returned wave-uniform.  Four DPP adds reduce each
+// 16-lane row (xor1, xor2, half-mirror, mirror), then the four row sums are
+// read with v_readlane.
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dppf<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dppf<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dppf<0x141>(v);  // row_half_mirror
+    v += dppf<0x140>(v);  // row_mirror
+    float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (a + b) + (c + d);
+}
+// Three independent sums at once (interleaved for ILP).
+__device__ __forceinline__ void wave_sum3(float &x, float &y, float &z) {
+    x += dppf<0xB1>(x); y += dppf<0xB1>(y); z += dppf<0xB1>(z);
+    x += dppf<0x4E>(x); y += dppf<0x4E>(y); z += dppf<0x4E>(z);
+    x += dppf<0x141>(x); y += dppf<0x141>(y); z += dppf<0x141>(z);
+    x += dppf<0x140>(x); y += dppf<0x140>(y); z += dppf<0x140>(z);
+#define GH_RL(v, l) __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l))
+    x = (GH_RL(x, 0) + GH_RL(x, 16)) + (GH_RL(x, 32) + GH_RL(x, 48));
+    y = (GH_RL(y, 0) + GH_RL(y, 16)) + (GH_RL(y, 32) + GH_RL(y, 48));
+    z = (GH_RL(z, 0) + GH_RL(z, 16)) + (GH_RL(z, 32) + GH_RL(z, 48));
+#undef GH_RL
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T uniform(T v) {
+    return (T)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// ---------------------------------------------------------------- counter RNG
+// Philox4x32-10 (Salmon et al., SC'11).  Every draw of the Gibbs sampler is
+// philox({it<<8 | purpose, a, b, 0}, {seed, 0})[0].
+__host__ __device__ __forceinline__ uint32_t philox_draw(uint32_t seed, uint32_t it, uint32_t purpose,
+                                                         uint32_t a, uint32_t b) {
+    uint32_t c0 = (it << 8) | purpose, c1 = a, c2 = b, c3 = 0u;
+    uint32_t k0 = seed, k1 = 0u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c0;
+}
+__host__ __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+__host__ __device__ __forceinline__ uint32_t urange(uint32_t x, uint32_t lo, uint32_t hi) {
+    return lo + (uint32_t)(((uint64_t)x * (uint64_t)(hi - lo + 1u)) >> 32);
+}
+
+enum Purpose {
+    P_H0 = 0, P_H1 = 1, P_DIP_ROOT = 2, P_DIP_SEG = 3, P_FLAT_HET = 4,
+    P_MONO_HET = 5, P_PBWT_STORED = 6, P_PBWT_START = 7, P_INIT_SEL = 8
+};
+
+// site flag bits in the resident panel
+enum { SF_COMMON = 1, SF_MAJOR = 2, SF_LQ = 4 };
+
+}  // namespace gh

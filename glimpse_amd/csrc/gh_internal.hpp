@@ -1,0 +1,125 @@
+// gh_internal.hpp -- host-side internals shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "glimpse_hip.h"
+
+namespace gh {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string &m);
+
+#define GH_HIP(x)                                                                               \
+    do {                                                                                        \
+        hipError_t _e = (x);                                                                    \
+        if (_e != hipSuccess)                                                                   \
+            throw ::gh::Error(GH_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(_e));      \
+    } while (0)
+
+#define GH_CHECK(cond, code, msg)                                                               \
+    do {                                                                                        \
+        if (!(cond)) throw ::gh::Error((code), (msg));                                          \
+    } while (0)
+
+// Device buffer with RAII; grows, never shrinks.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void *reserve(size_t b) {
+        if (b > bytes) {
+            release();
+            GH_HIP(hipMalloc(&p, b ? b : 1));
+            bytes = b;
+        }
+        return p;
+    }
+    template <typename T>
+    T *as() const { return (T *)p; }
+};
+
+template <typename T>
+inline T *dalloc_copy(DBuf &buf, const T *h, size_t n, hipStream_t s) {
+    buf.reserve(sizeof(T) * (n ? n : 1));
+    if (n) GH_HIP(hipMemcpyAsync(buf.p, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return buf.as<T>();
+}
+
+}  // namespace gh
+
+// Resident panel (one chunk).
+struct gh_panel {
+    int L = 0, N = 0, Lc = 0, Lhq = 0;
+    int W64 = 0;   // ceil(N/64): words per common site row
+    int Wc = 0;    // ceil(Lc/64): words per haplotype row of the transposed panel
+    std::vector<double> cm;
+    std::vector<float> cm_pos;
+    std::vector<uint8_t> flags;       // SF_* bits
+    std::vector<int32_t> common2tot;
+    std::vector<uint32_t> cref, calt;
+    int64_t device_bytes = 0;
+    hipStream_t stream = nullptr;
+
+    gh::DBuf d_cm, d_cmpos, d_flags, d_crow, d_c2t, d_sm, d_hm, d_shap_ptr, d_shap_idx, d_rare_ptr,
+        d_rare_idx;
+    // sparse PBWT (per-site structures for the selection kernel)
+    std::vector<uint8_t> ypacked;
+    std::vector<int32_t> asmall_ptr, asmall_idx;
+    struct PbwtDev;
+    PbwtDev *pbwt = nullptr;
+};
+
+namespace gh {
+
+// Per-item description of a conditioning set living on the device.
+struct CondItem {
+    int32_t K;          // states
+    int32_t W;          // ceil(K/64)
+    int32_t Lp;         // polymorphic sites (filled by the index kernel)
+    int32_t pad;
+    int64_t state_off;  // into states
+    int64_t site_off;   // into per-site scratch (L per item): poly list, poly_of_site
+    int64_t hvar_off;   // words, rows of W words, L rows reserved
+};
+
+// Builds conditioning sets for n items whose sorted states are on the device.
+// items/poly/hvar/t are device arrays sized by the caller:
+//   poly, pos: L per item; t: L per item; hvar: L*W per item (words).
+// Returns with Lp of each item written to items[i].Lp (device).
+void condset_build(gh_panel *P, int n, CondItem *d_items, const CondItem *h_items, const int32_t *d_states,
+                   int32_t *d_poly, int32_t *d_pos, float *d_t, uint64_t *d_hvar, uint8_t *d_mark,
+                   int ne, int maxW, hipStream_t s);
+
+// FB over n haploid items.  HL [item][2L], flat [item][L] (device); alpha
+// scratch per item: Lp*64W floats at alpha_off[i].  HP out [item][2L].
+struct FbItem {
+    int32_t K, W, Lp, cond;   // cond = index of the CondItem
+    int64_t alpha_off;        // floats
+    int64_t hl_off;           // into HL / HP ([2L] per item)
+    int64_t flat_off;         // into flat ([L] per item)
+};
+void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
+               const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_alpha,
+               float *d_HP, float ee, float ed, hipStream_t s);
+// HP at monomorphic sites (closed form) for the same items.
+void fb_mono_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_pos,
+                    const float *d_HL, const uint8_t *d_flat, float *d_HP, float ee, float ed, hipStream_t s);
+
+}  // namespace gh

@@ -1,0 +1,230 @@
+"""ctypes front-end of libglimpse_hip.so (the C-ABI in include/glimpse_hip.h).
+
+This is the Python host mirror of the phase module's compute interfaces:
+Panel ~ the loaded split_reference panel, condset_batch ~ conditioning_set::select,
+impute_batch ~ imputation_hmm::computePosteriors, rephase_batch ~
+phasing_hmm::rephaseHaplotypes, pbwt_select ~
+haplotype_set::matchHapsFromCompressedPBWTSmall, phase_run ~ caller::phase_loop.
+There is no CPU fallback: every call goes to the HIP library and raises if it
+is missing or reports an error."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libglimpse_hip.so")
+_lib = None
+vp = ctypes.c_void_p
+
+
+class GhError(RuntimeError):
+    pass
+
+
+class PanelDesc(ctypes.Structure):
+    _fields_ = [("n_sites", ctypes.c_int32), ("n_haps", ctypes.c_int32), ("cm", vp), ("cref", vp),
+                ("calt", vp), ("lq", vp), ("common_words", vp), ("rare_ptr", vp), ("rare_idx", vp),
+                ("ypacked", vp), ("ypacked_len", ctypes.c_int64), ("asmall_ptr", vp), ("asmall_idx", vp),
+                ("sparse_maf", ctypes.c_float)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("burnin", ctypes.c_int32), ("main_iter", ctypes.c_int32), ("ne", ctypes.c_int32),
+                ("min_gl", ctypes.c_float), ("err_imp", ctypes.c_float), ("err_phase", ctypes.c_float),
+                ("pbwt_depth", ctypes.c_int32), ("pbwt_modulo_cm", ctypes.c_float),
+                ("kinit", ctypes.c_int32), ("kpbwt", ctypes.c_int32), ("seed", ctypes.c_uint32)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "fb_state_sites", "fb_poly_sites", "ph_state_sites", "ph_segments", "pbwt_hap_sites",
+        "ms_select", "ms_condset", "ms_fb", "ms_phase", "ms_other", "ms_total")]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GhError("libglimpse_hip.so is not built; run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        i32, i64, f32, u32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_uint32
+        L.gh_last_error.restype = ctypes.c_char_p
+        L.gh_default_params.argtypes = [ctypes.POINTER(Params)]
+        L.gh_build_sparse_pbwt.argtypes = [ctypes.POINTER(PanelDesc), vp, vp, vp, vp, vp, vp]
+        L.gh_panel_create.argtypes = [ctypes.POINTER(PanelDesc), ctypes.POINTER(vp)]
+        L.gh_panel_destroy.argtypes = [vp]
+        L.gh_panel_info.argtypes = [vp, vp, vp, vp]
+        L.gh_condset_batch.argtypes = [vp, i32, vp, vp, i32, vp, vp, i64, vp, vp, i64, vp]
+        L.gh_impute_batch.argtypes = [vp, i32, vp, vp, vp, vp, f32, i32, vp]
+        L.gh_rephase_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp, f32, i32, u32, i32, vp]
+        L.gh_pbwt_select.argtypes = [vp, i32, vp, vp, vp, i32, f32, vp, vp, vp, i64]
+        L.gh_phase_run.argtypes = [vp, i32, vp, vp, ctypes.POINTER(Params), vp, vp, vp, vp, vp, vp, vp,
+                                   ctypes.POINTER(RunStats)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+def _check(rc):
+    if rc != 0:
+        raise GhError("glimpse_hip error %d: %s" % (rc, lib().gh_last_error().decode()))
+
+
+def default_params(**kw):
+    p = Params()
+    lib().gh_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _csr(lists):
+    lists = [np.asarray(x, np.int32) for x in lists]
+    ptr = np.zeros(len(lists) + 1, np.int32)
+    ptr[1:] = np.cumsum([x.size for x in lists])
+    ids = np.concatenate(lists + [np.zeros(0, np.int32)]).astype(np.int32)
+    if ids.size == 0:
+        ids = np.zeros(1, np.int32)
+    return ptr, ids
+
+
+def build_sparse_pbwt(P):
+    """split_reference's PBWT build (product implementation)."""
+    keep = _desc_arrays(P)
+    d = _desc(P, keep)
+    yl, al, nhq = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+    _check(lib().gh_build_sparse_pbwt(ctypes.byref(d), None, ctypes.byref(yl), None, None, ctypes.byref(al),
+                                      ctypes.byref(nhq)))
+    y = np.zeros(max(yl.value, 1), np.uint8)
+    ap = np.zeros(nhq.value + 2, np.int32)
+    ai = np.zeros(max(al.value, 1), np.int32)
+    _check(lib().gh_build_sparse_pbwt(ctypes.byref(d), _p(y), ctypes.byref(yl), _p(ap), _p(ai), ctypes.byref(al),
+                                      ctypes.byref(nhq)))
+    return y[:yl.value], ap, ai[:al.value]
+
+
+def _desc_arrays(P):
+    return dict(cm=np.ascontiguousarray(P["cm"], np.float64), cref=np.ascontiguousarray(P["cref"], np.uint32),
+                calt=np.ascontiguousarray(P["calt"], np.uint32),
+                lq=np.ascontiguousarray(P.get("lq", np.zeros(P["L"], np.uint8)), np.uint8),
+                cw=np.ascontiguousarray(P["common_words"], np.uint64),
+                rp=np.ascontiguousarray(P["rare_ptr"], np.int32), ri=np.ascontiguousarray(P["rare_idx"], np.int32))
+
+
+def _desc(P, k, ypacked=None, asmall=None):
+    d = PanelDesc()
+    d.n_sites, d.n_haps = P["L"], P["N"]
+    d.cm, d.cref, d.calt, d.lq = _p(k["cm"]).value, _p(k["cref"]).value, _p(k["calt"]).value, _p(k["lq"]).value
+    d.common_words, d.rare_ptr, d.rare_idx = _p(k["cw"]).value, _p(k["rp"]).value, _p(k["ri"]).value
+    if ypacked is not None:
+        d.ypacked, d.ypacked_len = _p(ypacked).value, ypacked.size
+        d.asmall_ptr, d.asmall_idx = _p(asmall[0]).value, _p(asmall[1]).value
+    d.sparse_maf = 0.001
+    return d
+
+
+class Panel:
+    """A chunk panel resident on the GPU."""
+
+    def __init__(self, P, with_pbwt=True):
+        self.L, self.N = P["L"], P["N"]
+        self._k = _desc_arrays(P)
+        y = a = None
+        if with_pbwt:
+            y, ap, ai = build_sparse_pbwt(P)
+            self._k.update(y=y, ap=ap, ai=ai)
+            a = (ap, ai)
+        d = _desc(P, self._k, y, a)
+        h = vp()
+        _check(lib().gh_panel_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().gh_panel_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        nc, nh, nb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().gh_panel_info(self.h, ctypes.byref(nc), ctypes.byref(nh), ctypes.byref(nb)))
+        return dict(n_common=nc.value, n_common_hq=nh.value, device_bytes=nb.value)
+
+    def condset_batch(self, state_lists, ne=100000):
+        ptr, ids = _csr(state_lists)
+        n = len(state_lists)
+        cap_p = n * self.L
+        poly_ptr = np.zeros(n + 1, np.int32)
+        poly = np.zeros(cap_p, np.int32)
+        hv_ptr = np.zeros(n + 1, np.int64)
+        cap_w = sum(self.L * ((len(s) + 63) // 64) for s in state_lists)
+        hv = np.zeros(cap_w, np.uint64)
+        t = np.zeros(cap_p, np.float32)
+        _check(lib().gh_condset_batch(self.h, n, _p(ptr), _p(ids), ne, _p(poly_ptr), _p(poly), cap_p, _p(hv_ptr),
+                                      _p(hv), cap_w, _p(t)))
+        out = []
+        for i in range(n):
+            W = (len(state_lists[i]) + 63) // 64
+            p0, p1 = poly_ptr[i], poly_ptr[i + 1]
+            Lp = p1 - p0
+            out.append(dict(poly=poly[p0:p1].copy(), hvar=hv[hv_ptr[i]:hv_ptr[i + 1]].reshape(Lp, W).copy(),
+                            t=t[p0 - i:p1 - i - 1].copy()))
+        return out
+
+    def impute_batch(self, state_lists, HL, flat, err_imp=1e-12, ne=100000):
+        ptr, ids = _csr(state_lists)
+        n = len(state_lists)
+        HL = np.ascontiguousarray(HL, np.float32).reshape(n, 2 * self.L)
+        flat = np.ascontiguousarray(flat, np.uint8).reshape(n, self.L)
+        HP = np.zeros((n, 2 * self.L), np.float32)
+        _check(lib().gh_impute_batch(self.h, n, _p(ptr), _p(ids), _p(HL), _p(flat), ctypes.c_float(err_imp), ne,
+                                     _p(HP)))
+        return HP
+
+    def rephase_batch(self, state_lists, flat, h0, h1, seed, it, tgt, err_phase=1e-4, ne=100000):
+        ptr, ids = _csr(state_lists)
+        n = len(state_lists)
+        a = np.ascontiguousarray(h0, np.uint8).reshape(n, self.L).copy()
+        b = np.ascontiguousarray(h1, np.uint8).reshape(n, self.L).copy()
+        tg = np.ascontiguousarray(tgt, np.int32)
+        _check(lib().gh_rephase_batch(self.h, n, _p(ptr), _p(ids), _p(np.ascontiguousarray(flat, np.uint8)), _p(a),
+                                      _p(b), ctypes.c_float(err_phase), ne, seed, it, _p(tg)))
+        return a, b
+
+    def pbwt_select(self, hap_alleles, stored, start_idx, depth=12, modulo=0.1, cap=None):
+        haps = np.ascontiguousarray(hap_alleles, np.uint8)
+        T = haps.shape[0] // 2
+        cap = cap or max(1 << 16, 64 * depth * T * 64)
+        off = np.zeros(T * depth, np.int32)
+        cnt = np.zeros(T * depth, np.int32)
+        ids = np.zeros(cap, np.int32)
+        _check(lib().gh_pbwt_select(self.h, T, _p(haps), _p(np.ascontiguousarray(stored, np.uint8)),
+                                    _p(np.ascontiguousarray(start_idx, np.int32)), depth, ctypes.c_float(modulo),
+                                    _p(off), _p(cnt), _p(ids), cap))
+        return [[ids[off[t * depth + d]: off[t * depth + d] + cnt[t * depth + d]].copy() for d in range(depth)]
+                for t in range(T)]
+
+    def phase_run(self, gl, flat, params, kpbwt_per_target=None):
+        gl = np.ascontiguousarray(gl, np.uint8)
+        flat = np.ascontiguousarray(flat, np.uint8)
+        T, L = flat.shape
+        n_iter = 1 + params.burnin + params.main_iter
+        out = dict(gp0=np.zeros((T, L), np.float32), gp1=np.zeros((T, L), np.float32),
+                   hds=np.zeros((T, L), np.uint8), h0=np.zeros((T, L), np.uint8), h1=np.zeros((T, L), np.uint8),
+                   nstates=np.zeros((T, n_iter), np.int32))
+        kpt = None if kpbwt_per_target is None else np.ascontiguousarray(kpbwt_per_target, np.int32)
+        st = RunStats()
+        _check(lib().gh_phase_run(self.h, T, _p(gl), _p(flat), ctypes.byref(params), _p(kpt), _p(out["gp0"]),
+                                  _p(out["gp1"]), _p(out["hds"]), _p(out["h0"]), _p(out["h1"]),
+                                  _p(out["nstates"]), ctypes.byref(st)))
+        out["stats"] = {n: getattr(st, n) for n, _ in RunStats._fields_}
+        return out

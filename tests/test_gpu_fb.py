@@ -1,0 +1,82 @@
+"""GPU parity: conditioning sets (bit-exact) and the haploid forward-backward
+(HP within 1e-4 absolute of the oracle for identical states and likelihoods)."""
+import numpy as np
+import pytest
+
+import orc
+from conftest import hvar_bytes_to_words, mask_words
+
+pytestmark = pytest.mark.gpu
+
+FB_TOL = 1e-4   # absolute, on haplotype posteriors
+
+
+def _state_sets(N, rng):
+    sets = [np.arange(N)]
+    for K in (1, 7, 37, 64, 65, 300, 1000, 2000):
+        K = min(K, N)
+        sets.append(np.sort(rng.choice(N, K, replace=False)))
+    return sets
+
+
+@pytest.fixture(scope="module")
+def panels(small_case):
+    from glimpse_amd import hip
+    P, gl, flat = small_case
+    return P, gl, flat, orc.Panel(P), hip.Panel(P, with_pbwt=False)
+
+
+def test_condset_bit_exact(panels):
+    P, gl, flat, OP, GP = panels
+    rng = np.random.default_rng(1)
+    sets = _state_sets(P["N"], rng)
+    got = GP.condset_batch(sets)
+    for s, g in zip(sets, got):
+        ref = OP.condset_from_states(s)
+        K = len(s)
+        assert np.array_equal(g["poly"], ref["poly"]), "polymorphic site list differs (K=%d)" % K
+        assert np.array_equal(mask_words(g["hvar"], K), hvar_bytes_to_words(ref["hvar"], K)), "Hvar differs (K=%d)" % K
+        # transitions: double expm1 on the device vs glibc; compare as float bits, allow 1 ulp
+        d = np.abs(g["t"].view(np.int32).astype(np.int64) - ref["t"].view(np.int32).astype(np.int64))
+        assert d.max(initial=0) <= 1, "transition t differs by more than 1 ulp (K=%d)" % K
+
+
+def test_fb_posteriors(panels):
+    P, gl, flat, OP, GP = panels
+    rng = np.random.default_rng(2)
+    sets = _state_sets(P["N"], rng)
+    T = flat.shape[0]
+    HLs, flats, refs = [], [], []
+    for i, s in enumerate(sets):
+        t = i % T
+        other = P["tgt_haps"][2 * t + 1] if i % 2 else None
+        HL = orc.hap_likelihoods(gl[t], flat[t], other)
+        HLs.append(HL)
+        flats.append(flat[t])
+        refs.append(OP.impute_hap(s, HL, flat[t]))
+    HP = GP.impute_batch(sets, np.stack(HLs), np.stack(flats))
+    for i, s in enumerate(sets):
+        err = np.abs(HP[i] - refs[i]).max()
+        assert err <= FB_TOL, "K=%d: max |HP - oracle| = %g" % (len(s), err)
+        assert np.allclose(HP[i][0::2] + HP[i][1::2], 1.0, atol=1e-5)
+
+
+def test_fb_all_flat_and_lq(panels):
+    """Edge cases: every site flat (no emissions) and low-quality sites."""
+    P, gl, flat, OP, GP = panels
+    from glimpse_amd import hip
+    rng = np.random.default_rng(3)
+    s = np.sort(rng.choice(P["N"], 500, replace=False))
+    fl = np.ones(P["L"], np.uint8)
+    HL = orc.hap_likelihoods(gl[0], fl)
+    HP = GP.impute_batch([s], HL[None], fl[None])[0]
+    ref = OP.impute_hap(s, HL, fl)
+    assert np.abs(HP - ref).max() <= FB_TOL
+    # low-quality flags: a panel copy with every 7th site LQ
+    P2 = dict(P)
+    P2["lq"] = (np.arange(P["L"]) % 7 == 3).astype(np.uint8)
+    OP2, GP2 = orc.Panel(P2), hip.Panel(P2, with_pbwt=False)
+    HL = orc.hap_likelihoods(gl[1], flat[1])
+    HP = GP2.impute_batch([s], HL[None], flat[1][None])[0]
+    ref = OP2.impute_hap(s, HL, flat[1])
+    assert np.abs(HP - ref).max() <= FB_TOL
