@@ -7,6 +7,7 @@
 
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
+#include "gh_phase.hpp"
 
 namespace gh {
 void panel_create(const gh_panel_desc *d, gh_panel **out);
@@ -193,10 +194,65 @@ int gh_build_sparse_pbwt(const gh_panel_desc *, uint8_t *, int64_t *, int32_t *,
     return GH_ERR_ARG;
 }
 
-int gh_rephase_batch(gh_panel *, int32_t, const int32_t *, const int32_t *, const uint8_t *, uint8_t *, uint8_t *,
-                     float, int32_t, uint32_t, int32_t, const int32_t *) {
-    g_err = "gh_rephase_batch: not built in this library version";
-    return GH_ERR_ARG;
+int gh_rephase_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int32_t *states, const uint8_t *flat,
+                     uint8_t *h0, uint8_t *h1, float err_phase, int32_t ne, uint32_t seed, int32_t it,
+                     const int32_t *tgt) {
+    return guard([&] {
+        GH_CHECK(P && n > 0 && state_ptr && states && flat && h0 && h1 && tgt, GH_ERR_ARG,
+                 "gh_rephase_batch: bad arguments");
+        hipStream_t s = P->stream;
+        const int L = P->L;
+        CondBatch B;
+        B.build(P, n, state_ptr, states, ne, s);
+        int maxK = 0;
+        std::vector<PhItem> ph(n);
+        for (int i = 0; i < n; i++) {
+            std::memset(&ph[i], 0, sizeof(PhItem));
+            ph[i].cond = i;
+            ph[i].tgt = tgt[i];
+            ph[i].hap_off = (int64_t)i * L;
+            ph[i].locus_off = (int64_t)i * L;
+            maxK = std::max(maxK, B.h[i].K);
+        }
+        DBuf d_ph, d_H0, d_H1, d_flat, d_loci, d_seg, d_segsum, d_imp, d_impsum, d_of1, d_dip, d_status;
+        dalloc_copy(d_ph, ph.data(), n, s);
+        dalloc_copy(d_H0, h0, (size_t)n * L, s);
+        dalloc_copy(d_H1, h1, (size_t)n * L, s);
+        dalloc_copy(d_flat, flat, (size_t)n * L, s);
+        d_loci.reserve(sizeof(Locus) * (size_t)n * L);
+        phase_prep_launch(n, d_ph.as<PhItem>(), B.items.as<CondItem>(), B.poly.as<int32_t>(), d_H0.as<uint8_t>(),
+                          d_H1.as<uint8_t>(), d_flat.as<uint8_t>(), P->d_flags.as<uint8_t>(), d_loci.as<Locus>(), s);
+        GH_HIP(hipMemcpyAsync(ph.data(), d_ph.p, sizeof(PhItem) * n, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+        const int64_t KH = (int64_t)phase_kpt_states(maxK) * 8;
+        int64_t so = 0, ss = 0, io = 0, is = 0, dd = 0;
+        for (int i = 0; i < n; i++) {
+            ph[i].seg_off = so; so += ph[i].n_segs * KH;
+            ph[i].segsum_off = ss; ss += ph[i].n_segs * 9;
+            ph[i].imp_off = io; io += ph[i].n_miss * KH;
+            ph[i].impsum_off = is; is += ph[i].n_miss * 8;
+            ph[i].dip_off = dd; dd += ph[i].n_segs;
+        }
+        dalloc_copy(d_ph, ph.data(), n, s);
+        PhaseBufs PB;
+        PB.seg = (float *)d_seg.reserve(sizeof(float) * std::max<int64_t>(so, 1));
+        PB.segsum = (float *)d_segsum.reserve(sizeof(float) * std::max<int64_t>(ss, 1));
+        PB.imp = (float *)d_imp.reserve(sizeof(float) * std::max<int64_t>(io, 1));
+        PB.impsum = (float *)d_impsum.reserve(sizeof(float) * std::max<int64_t>(is, 1));
+        PB.of1 = (float *)d_of1.reserve(sizeof(float) * std::max<int64_t>(is, 1));
+        PB.dip = (int32_t *)d_dip.reserve(sizeof(int32_t) * std::max<int64_t>(dd, 1));
+        int32_t *st = (int32_t *)d_status.reserve(sizeof(int32_t));
+        GH_HIP(hipMemsetAsync(st, 0, sizeof(int32_t), s));
+        phase_launch(P, n, maxK, d_ph.as<PhItem>(), B.items.as<CondItem>(), B.hvar.as<uint64_t>(), d_loci.as<Locus>(),
+                     B.pos.as<int32_t>(), ne, PB, d_H0.as<uint8_t>(), d_H1.as<uint8_t>(), err_phase, seed,
+                     (uint32_t)it, st, s);
+        int32_t status = 0;
+        GH_HIP(hipMemcpyAsync(&status, st, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GH_HIP(hipMemcpyAsync(h0, d_H0.p, (size_t)n * L, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipMemcpyAsync(h1, d_H1.p, (size_t)n * L, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+        GH_CHECK(status == 0, GH_ERR_NUMERIC, "phasing HMM underflow (TRANS_HAP / SAMPLE_DIP)");
+    });
 }
 
 int gh_pbwt_select(gh_panel *, int32_t, const uint8_t *, const uint8_t *, const int32_t *, int32_t, float, int32_t *,

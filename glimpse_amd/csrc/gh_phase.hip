@@ -1,0 +1,484 @@
+// gh_phase.hip -- diploid phasing HMM and its Gibbs draws
+// (phasing_hmm::reallocate / backward / forward / rephaseHaplotypes,
+//  phase/src/models/phasing_hmm.h:115-423, phasing_hmm.cpp:65-302).
+//
+// phase_prep_kernel (one workgroup per target): classifies the polymorphic
+// loci (peak het j = het ordinal mod 3, peak hom, flat het), assigns segments
+// of three hets (a segment starts at het ordinals 0, 3, 6, ...), the flat-het
+// ordinal, and counts segments / flat hets.
+//
+// phase_kernel<KPT> (one 256-thread workgroup per target): thread t owns
+// states t + 256*i (i < KPT) and keeps their 8 hypothesis values in
+// registers.  Per locus the 8 per-hypothesis sums are reduced with DPP inside
+// each wave and across the four waves through LDS.  The backward pass stores
+// segment-start and flat-het messages (K x 8 floats, [state][hyp] with the
+// state index interleaved as in registers) in HBM; the forward pass samples
+// the segment diplotypes (TRANS_HAP + SAMPLE_DIP) and the flat hets, then the
+// write-back updates H0/H1 (including the reference's FLAT_HET write-back
+// that assigns H0 twice, phasing_hmm.cpp:281-284).
+#include "gh_device.hpp"
+#include "gh_internal.hpp"
+#include "gh_phase.hpp"
+
+namespace gh {
+
+constexpr int PH_NT = 256;
+constexpr int PREP_NT = 1024;
+
+template <int NT>
+__device__ __forceinline__ void block_scan3(int a, int b, int c, int &ea, int &eb, int &ec, int &ta, int &tb,
+                                            int &tc) {
+    __shared__ int ws[3][NT / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = a, y = b, z = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int xa = __shfl_up(x, o), ya = __shfl_up(y, o), za = __shfl_up(z, o);
+        if (lane >= o) { x += xa; y += ya; z += za; }
+    }
+    if (lane == 63) { ws[0][wid] = x; ws[1][wid] = y; ws[2][wid] = z; }
+    __syncthreads();
+    if (wid == 0) {
+        int u = lane < NT / 64 ? ws[0][lane] : 0, v = lane < NT / 64 ? ws[1][lane] : 0,
+            w = lane < NT / 64 ? ws[2][lane] : 0;
+#pragma unroll
+        for (int o = 1; o < NT / 64; o <<= 1) {
+            int ua = __shfl_up(u, o), va = __shfl_up(v, o), wa = __shfl_up(w, o);
+            if (lane >= o) { u += ua; v += va; w += wa; }
+        }
+        if (lane < NT / 64) { ws[0][lane] = u; ws[1][lane] = v; ws[2][lane] = w; }
+    }
+    __syncthreads();
+    ta = ws[0][NT / 64 - 1]; tb = ws[1][NT / 64 - 1]; tc = ws[2][NT / 64 - 1];
+    ea = (wid ? ws[0][wid - 1] : 0) + x - a;
+    eb = (wid ? ws[1][wid - 1] : 0) + y - b;
+    ec = (wid ? ws[2][wid - 1] : 0) + z - c;
+    __syncthreads();
+}
+
+// Locus classes per polymorphic site: 0 = not a phasing locus, 1 = peak het,
+// 2 = peak hom, 3 = flat het  (phasing_hmm.cpp:71-96).
+__device__ __forceinline__ int locus_class(const uint8_t *H0, const uint8_t *H1, const uint8_t *flat,
+                                           const uint8_t *flags, int site) {
+    const bool a0 = H0[site], a1 = H1[site];
+    const bool data = !flat[site] && !(flags[site] & SF_LQ);
+    if (data) return a0 != a1 ? 1 : 2;
+    return a0 != a1 ? 3 : 0;
+}
+
+__global__ __launch_bounds__(PREP_NT) void phase_prep_kernel(PhItem *__restrict__ items,
+                                                            const CondItem *__restrict__ cond,
+                                                            const int32_t *__restrict__ poly_all,
+                                                            const uint8_t *__restrict__ H0_all,
+                                                            const uint8_t *__restrict__ H1_all,
+                                                            const uint8_t *__restrict__ flat_all,
+                                                            const uint8_t *__restrict__ flags,
+                                                            Locus *__restrict__ loci_all) {
+    PhItem &it = items[blockIdx.x];
+    const CondItem c = cond[it.cond];
+    const int Lp = c.Lp, tid = threadIdx.x;
+    const int32_t *poly = poly_all + c.site_off;
+    const uint8_t *H0 = H0_all + it.hap_off, *H1 = H1_all + it.hap_off, *flat = flat_all + it.hap_off;
+    Locus *loci = loci_all + it.locus_off;
+    const int C = (Lp + PREP_NT - 1) / PREP_NT;
+    const int q0 = min(Lp, tid * C), q1 = min(Lp, q0 + C);
+    int nv = 0, nh = 0, nm = 0;
+    for (int q = q0; q < q1; q++) {
+        const int cl = locus_class(H0, H1, flat, flags, poly[q]);
+        nv += cl != 0; nh += cl == 1; nm += cl == 3;
+    }
+    int ev, eh, em, tv, th, tm;
+    block_scan3<PREP_NT>(nv, nh, nm, ev, eh, em, tv, th, tm);
+    for (int q = q0; q < q1; q++) {
+        const int site = poly[q];
+        const int cl = locus_class(H0, H1, flat, flags, site);
+        if (!cl) continue;
+        Locus L;
+        L.rel = q;
+        L.abs = site;
+        L.alt = H0[site];
+        L.miss = cl == 3 ? em : -1;
+        if (cl == 1) {
+            L.type = (int8_t)(eh % 3);
+            L.seg = eh / 3;
+        } else {
+            L.type = cl == 2 ? (int8_t)-1 : (int8_t)-2;
+            L.seg = eh == 0 ? 0 : (eh - 1) / 3;
+        }
+        loci[ev] = L;
+        ev++;
+        eh += cl == 1;
+        em += cl == 3;
+    }
+    if (tid == 0) {
+        it.n_loci = tv;
+        it.n_segs = th == 0 ? 1 : (th - 1) / 3 + 1;
+        it.n_miss = tm;
+    }
+}
+
+// EMIT tables (phasing_hmm.cpp:38-56): hypothesis h carries allele 0 at het j
+// when bit (2-j) of h is set.
+__device__ __forceinline__ float emit_het(int j, int h, bool allele, float ee, float ed) {
+    const bool hyp0 = (h >> (2 - j)) & 1;
+    return (hyp0 != allele) ? ee : ed;
+}
+
+// ((v0+v4)+(v1+v5)) + ((v2+v6)+(v3+v7)) -- the reference's horizontal_add order
+__device__ __forceinline__ float hadd8(const float *v) {
+    return ((v[0] + v[4]) + (v[1] + v[5])) + ((v[2] + v[6]) + (v[3] + v[7]));
+}
+
+// Sum of 8 per-thread values over the workgroup; result uniform in every thread.
+__device__ __forceinline__ void block_sum8(float v[8], float *lds) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = 0; h < 8; h++) v[h] = wave_sum(v[h]);
+    if (lane == 0) {
+#pragma unroll
+        for (int h = 0; h < 8; h++) lds[wid * 8 + h] = v[h];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 8; h++) v[h] = (lds[h] + lds[8 + h]) + (lds[16 + h] + lds[24 + h]);
+    __syncthreads();
+}
+
+__device__ __forceinline__ float transition(const double *cm, int prev, int next, double nrho) {
+    double x = -expm1(nrho * (cm[next] - cm[prev]));
+    const double one_l = 1.0 - 1e-7;
+    x = x < 1e-7 ? 1e-7 : (x > one_l ? one_l : x);
+    return (float)x;
+}
+
+template <int KPT>
+__global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__ items, const CondItem *__restrict__ cond,
+                                                     const uint64_t *__restrict__ hvar_all,
+                                                     const Locus *__restrict__ loci_all,
+                                                     const int32_t *__restrict__ pos_all,
+                                                     const double *__restrict__ cm, double nrho,
+                                                     float *__restrict__ seg_all, float *__restrict__ segsum_all,
+                                                     float *__restrict__ imp_all, float *__restrict__ impsum_all,
+                                                     float *__restrict__ of1_all, int32_t *__restrict__ dip_all,
+                                                     uint8_t *__restrict__ H0_all, uint8_t *__restrict__ H1_all,
+                                                     int L, float ee, float ed, uint32_t seed, uint32_t iter,
+                                                     int32_t *__restrict__ status) {
+    __shared__ float lds[32];
+    __shared__ float red64[4][64];
+    __shared__ float hprobs[64];
+    __shared__ int dip_sh;
+    const PhItem it = items[blockIdx.x];
+    const CondItem c = cond[it.cond];
+    const int K = c.K, W = c.W, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint64_t *hv = hvar_all + c.hvar_off;
+    const Locus *loci = loci_all + it.locus_off;
+    const int nloc = it.n_loci;
+    float *segp = seg_all + it.seg_off;
+    float *segs = segsum_all + (size_t)it.segsum_off;
+    float *impp = imp_all + it.imp_off;
+    float *imps = impsum_all + (size_t)it.impsum_off;
+    float *of1 = of1_all + (size_t)it.impsum_off;
+    int32_t *dip = dip_all + it.dip_off;
+    uint8_t *H0 = H0_all + it.hap_off, *H1 = H1_all + it.hap_off;
+    const float Kf = (float)K;
+    const float mism = ed / ee;
+    const size_t KH = (size_t)PH_NT * KPT * 8;  // stored message stride (padded states)
+
+    bool valid[KPT];
+    int kk[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; i++) { kk[i] = tid + PH_NT * i; valid[i] = kk[i] < K; }
+
+    float p[KPT][8];
+    float sumK[KPT];
+    float sumH[8];
+    float sumT = 1.0f;
+#pragma unroll
+    for (int i = 0; i < KPT; i++) sumK[i] = 0.0f;
+
+    auto bit_of = [&](int rel, int i) -> bool {
+        const int k = kk[i];
+        if (!valid[i]) return false;
+        return (hv[(size_t)rel * W + (k >> 6)] >> (k & 63)) & 1ull;
+    };
+
+    // One locus update (INIT / RUN / COLLAPSE of the three kinds).
+    auto step = [&](const Locus &Lc, int mode, float yt) {
+        const float nt = 1.0f - yt;
+        float tF[8];
+        float NT = 0.0f;
+        if (mode != 0) {
+            const float sc = yt / (Kf * sumT);
+#pragma unroll
+            for (int h = 0; h < 8; h++) tF[h] = sumH[h] * sc;
+            NT = nt / sumT;
+        }
+        float acc[8];
+#pragma unroll
+        for (int h = 0; h < 8; h++) acc[h] = 0.0f;
+        if (Lc.type == -2 && mode == 0) {
+            const float v = 1.0f / (8 * (unsigned)K);
+#pragma unroll
+            for (int i = 0; i < KPT; i++)
+#pragma unroll
+                for (int h = 0; h < 8; h++) p[i][h] = valid[i] ? v : 0.0f;
+#pragma unroll
+            for (int h = 0; h < 8; h++) sumH[h] = 1.0f / 8;
+            sumT = 1.0f;
+            return;
+        }
+#pragma unroll
+        for (int i = 0; i < KPT; i++) {
+            if (!valid[i]) {
+#pragma unroll
+                for (int h = 0; h < 8; h++) p[i][h] = 0.0f;
+                continue;
+            }
+            const bool ah = bit_of(Lc.rel, i);
+#pragma unroll
+            for (int h = 0; h < 8; h++) {
+                float x;
+                if (mode == 0) x = 1.0f;
+                else if (mode == 1) x = fmaf(p[i][h], NT, tF[h]);
+                else x = fmaf(sumK[i], NT, tF[h]);
+                if (Lc.type >= 0) {
+                    const float e = emit_het(Lc.type, h, ah, ee, ed);
+                    x = (mode == 0) ? e : x * e;
+                } else if (Lc.type == -1) {
+                    if (ah != (bool)Lc.alt) x = (mode == 0) ? mism : x * mism;
+                }
+                p[i][h] = x;
+                acc[h] += x;
+            }
+        }
+        block_sum8(acc, lds);
+#pragma unroll
+        for (int h = 0; h < 8; h++) sumH[h] = acc[h];
+        sumT = hadd8(sumH);
+    };
+    auto sumk = [&]() {
+#pragma unroll
+        for (int i = 0; i < KPT; i++) sumK[i] = hadd8(p[i]);
+    };
+    auto store_msg = [&](float *dst, float *dsum, float *dsumT) {
+#pragma unroll
+        for (int i = 0; i < KPT; i++) {
+            float4 *d4 = (float4 *)(dst + ((size_t)i * PH_NT + tid) * 8);
+            d4[0] = make_float4(p[i][0], p[i][1], p[i][2], p[i][3]);
+            d4[1] = make_float4(p[i][4], p[i][5], p[i][6], p[i][7]);
+        }
+        if (tid < 8) dsum[tid] = sumH[tid];
+        if (tid == 0 && dsumT) *dsumT = sumT;
+    };
+
+    // ---------------- backward (phasing_hmm.cpp:193-245)
+    {
+        int seg = it.n_segs - 1;
+        for (int q = nloc - 1; q >= 0; q--) {
+            const Locus Lc = loci[q];
+            const bool last = q == nloc - 1;
+            const float yt = last ? 0.0f : transition(cm, Lc.abs, loci[q + 1].abs, nrho);
+            const bool seg_last = last || loci[q + 1].seg != Lc.seg;   // last locus of its segment
+            const bool seg_first = q == 0 || loci[q - 1].seg != Lc.seg;
+            step(Lc, last ? 0 : (seg_last ? 2 : 1), yt);
+            if (seg_first) {
+                sumk();
+                store_msg(segp + (size_t)Lc.seg * KH, segs + Lc.seg * 9, segs + Lc.seg * 9 + 8);
+            }
+            if (Lc.type == -2) store_msg(impp + (size_t)Lc.miss * KH, imps + Lc.miss * 8, nullptr);
+            (void)seg;
+        }
+    }
+    __syncthreads();
+    // ---------------- root diplotype (phasing_hmm.cpp:250-256)
+    if (tid == 0) {
+        float D[8], S = 0.0f;
+        if (nloc > 0) {
+            const float s0 = segs[8];
+            for (int d = 0; d < 8; d++) { D[d] = (segs[d] / s0) * (segs[7 - d] / s0); S += D[d]; }
+        } else {
+            for (int d = 0; d < 8; d++) D[d] = 0.0f;
+            S = 0.0f;
+        }
+        const float u = u01(philox_draw(seed, iter, P_DIP_ROOT, it.tgt, 0u)) * S;
+        float cs = D[0];
+        int r = 7;
+        for (int i = 0; nloc > 0 && i < 7; i++) {
+            if (u <= cs) { r = i; break; }
+            cs += D[i + 1];
+        }
+        dip[0] = r;
+        dip_sh = r;
+    }
+    __syncthreads();
+    int cur_dip = dip_sh;
+    // ---------------- forward (phasing_hmm.cpp:142-191)
+    for (int q = 0; q < nloc; q++) {
+        const Locus Lc = loci[q];
+        const float yt = q == 0 ? 0.0f : transition(cm, loci[q - 1].abs, Lc.abs, nrho);
+        const bool seg_first = q == 0 || loci[q - 1].seg != Lc.seg;
+        const bool seg_last = q == nloc - 1 || loci[q + 1].seg != Lc.seg;
+        step(Lc, q == 0 ? 0 : (seg_first ? 2 : 1), yt);
+        if (seg_last) sumk();
+        if (seg_last && q != nloc - 1) {
+            // TRANS_HAP
+            const float yt2 = transition(cm, Lc.abs, loci[q + 1].abs, nrho);
+            const float nt2 = 1.0f - yt2;
+            const float F2 = nt2 / sumT;
+            const float *nxt = segp + (size_t)(Lc.seg + 1) * KH;
+            float part[64];
+#pragma unroll
+            for (int e = 0; e < 64; e++) part[e] = 0.0f;
+#pragma unroll
+            for (int i = 0; i < KPT; i++) {
+                if (!valid[i]) continue;
+                const float4 *n4 = (const float4 *)(nxt + ((size_t)i * PH_NT + tid) * 8);
+                const float4 a = n4[0], b = n4[1];
+                const float nv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int h1 = 0; h1 < 8; h1++) {
+                    const float F1 = (sumH[h1] / sumT) * yt2 / Kf;
+                    const float pv = fmaf(p[i][h1], F2, F1);
+#pragma unroll
+                    for (int h2 = 0; h2 < 8; h2++) part[h1 * 8 + h2] += pv * nv[h2];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 64; e++) {
+                const float s = wave_sum(part[e]);
+                if (lane == 0) red64[wid][e] = s;
+            }
+            __syncthreads();
+            if (tid < 64) hprobs[tid] = (red64[0][tid] + red64[1][tid]) + (red64[2][tid] + red64[3][tid]);
+            __syncthreads();
+            if (tid == 0) {
+                float sumHP = 0.0f;
+                for (int h1 = 0; h1 < 8; h1++) sumHP += hadd8(&hprobs[h1 * 8]);
+                float D[8], S = 0.0f;
+                const int p0 = cur_dip, p1 = 7 - cur_dip;
+                for (int d = 0; d < 8; d++) {
+                    D[d] = (hprobs[p0 * 8 + d] / sumHP) * (hprobs[p1 * 8 + (7 - d)] / sumHP);
+                    S += D[d];
+                }
+                int r = 7;
+                if (!(S >= 1.17549435e-38f) || !(sumHP >= 1.17549435e-38f) || isinf(S) || isinf(sumHP)) {
+                    atomicExch(status, 1);
+                } else {
+                    const float u = u01(philox_draw(seed, iter, P_DIP_SEG, it.tgt, (uint32_t)(Lc.seg + 1))) * S;
+                    float cs = D[0];
+                    for (int i = 0; i < 7; i++) {
+                        if (u <= cs) { r = i; break; }
+                        cs += D[i + 1];
+                    }
+                }
+                dip[Lc.seg + 1] = r;
+                dip_sh = r;
+            }
+            __syncthreads();
+            cur_dip = dip_sh;
+        }
+        if (Lc.type == -2) {
+            // IMPUTE_FLAT_HET (phasing_hmm.h:389-423)
+            const float *ip = impp + (size_t)Lc.miss * KH;
+            float sR[8], sL[8], s0[8], s1[8];
+#pragma unroll
+            for (int h = 0; h < 8; h++) {
+                sR[h] = 1.0f / imps[Lc.miss * 8 + h];
+                sL[h] = 1.0f / sumH[h];
+                s0[h] = 0.0f;
+                s1[h] = 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < KPT; i++) {
+                if (!valid[i]) continue;
+                const bool ah = bit_of(Lc.rel, i);
+                const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
+                const float4 a = i4[0], b = i4[1];
+                const float iv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int h = 0; h < 8; h++) {
+                    const float v = (iv[h] * sR[h]) * (p[i][h] * sL[h]);
+                    if (ah) s1[h] += v; else s0[h] += v;
+                }
+            }
+            block_sum8(s0, lds);
+            block_sum8(s1, lds);
+            if (tid < 8) {
+                const float n = s1[tid] / (s0[tid] + s1[tid]);
+                of1[Lc.miss * 8 + tid] = fmaxf(0.0f, fminf(1.0f, n));
+            }
+        }
+    }
+    __syncthreads();
+    // ---------------- write-back (phasing_hmm.cpp:257-301)
+    for (int q = tid; q < nloc; q += PH_NT) {
+        const Locus Lc = loci[q];
+        const int d0 = dip[Lc.seg], d1 = 7 - d0;
+        if (Lc.type >= 0) {
+            H0[Lc.abs] = ((7 - d0) >> (2 - Lc.type)) & 1;
+            H1[Lc.abs] = ((7 - d1) >> (2 - Lc.type)) & 1;
+        } else if (Lc.type == -2) {
+            const float h0a1 = of1[Lc.miss * 8 + d0], h0a0 = 1.0f - h0a1;
+            const float h1a1 = of1[Lc.miss * 8 + d1], h1a0 = 1.0f - h1a1;
+            float p01 = (h0a0 * ee + h0a1 * ed) * (h1a1 * ee + h1a0 * ed);
+            float p10 = (h0a1 * ee + h0a0 * ed) * (h1a0 * ee + h1a1 * ed);
+            float sum = p01 + p10;
+            p01 = fminf(fmaxf(p01 / sum, 0.0f), 1.0f);
+            p10 = fminf(fmaxf(p10 / sum, 0.0f), 1.0f);
+            sum = p01 + p10;
+            const float u = u01(philox_draw(seed, iter, P_FLAT_HET, it.tgt, (uint32_t)Lc.miss));
+            const bool rf = ((double)u * (double)sum) < (double)p01;
+            H0[Lc.abs] = !rf;
+        }
+    }
+    // heterozygous monomorphic sites get a random phase (phasing_hmm.cpp:294-301)
+    const int32_t *pos = pos_all + c.site_off;
+    for (int s = tid; s < L; s += PH_NT) {
+        if (pos[s] >= 0) continue;
+        if (H0[s] != H1[s]) {
+            const bool rf = u01(philox_draw(seed, iter, P_MONO_HET, it.tgt, (uint32_t)s)) < 0.5f;
+            H0[s] = rf;
+            H1[s] = !rf;
+        }
+    }
+}
+
+void phase_prep_launch(int n, PhItem *d_items, const CondItem *d_cond, const int32_t *d_poly, const uint8_t *d_H0,
+                       const uint8_t *d_H1, const uint8_t *d_flat, const uint8_t *d_flags, Locus *d_loci,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(phase_prep_kernel, dim3(n), dim3(PREP_NT), 0, s, d_items, d_cond, d_poly, d_H0, d_H1, d_flat,
+                       d_flags, d_loci);
+    GH_HIP(hipGetLastError());
+}
+
+template <int KPT>
+static void phase_go(gh_panel *P, int n, const PhItem *d_items, const CondItem *d_cond, const uint64_t *d_hvar,
+                     const Locus *d_loci, const int32_t *d_pos, double nrho, PhaseBufs &B, uint8_t *d_H0,
+                     uint8_t *d_H1, float ee, float ed, uint32_t seed, uint32_t iter, int32_t *d_status,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(phase_kernel<KPT>, dim3(n), dim3(PH_NT), 0, s, d_items, d_cond, d_hvar, d_loci, d_pos,
+                       P->d_cm.as<double>(), nrho, B.seg, B.segsum, B.imp, B.impsum, B.of1, B.dip, d_H0, d_H1,
+                       P->L, ee, ed, seed, iter, d_status);
+}
+
+void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
+                  const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
+                  uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
+                  hipStream_t s) {
+    if (n <= 0) return;
+    const double nrho = (-0.04 * ne) / P->N;
+    const float ed = err_phase, ee = (float)(1.0 - (double)err_phase);
+    if (maxK <= 256) phase_go<1>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
+    else if (maxK <= 512) phase_go<2>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
+    else if (maxK <= 1024) phase_go<4>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
+    else if (maxK <= 2048) phase_go<8>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
+    else throw Error(GH_ERR_ARG, "phasing: more than 2048 conditioning states is not supported");
+    GH_HIP(hipGetLastError());
+}
+
+int phase_kpt_states(int maxK) {
+    return maxK <= 256 ? 256 : maxK <= 512 ? 512 : maxK <= 1024 ? 1024 : 2048;
+}
+
+}  // namespace gh

@@ -1,0 +1,50 @@
+// gh_phase.hpp -- host/device structures of the phasing HMM.
+#pragma once
+#include <stdint.h>
+
+#include "gh_internal.hpp"
+
+namespace gh {
+
+// One phasing locus (phasing_hmm.cpp:71-96): rel = row in the conditioning
+// set, abs = site, type = het ordinal mod 3 (0..2), -1 peak hom, -2 flat het.
+struct Locus {
+    int32_t rel;
+    int32_t abs;
+    int32_t seg;
+    int32_t miss;
+    int8_t type;
+    uint8_t alt;
+    uint8_t pad[2];
+};
+
+// Per-target phasing work item.
+struct PhItem {
+    int32_t cond;        // CondItem index
+    int32_t tgt;         // target index (keys the counter draws)
+    int32_t n_loci, n_segs, n_miss, pad;
+    int64_t hap_off;     // into H0/H1/flat ([L] per target)
+    int64_t locus_off;   // into loci (Lp reserved)
+    int64_t seg_off;     // floats: n_segs * KH
+    int64_t segsum_off;  // floats: n_segs * 9
+    int64_t imp_off;     // floats: n_miss * KH
+    int64_t impsum_off;  // floats: n_miss * 8 (also imputeProbOf1s)
+    int64_t dip_off;     // ints: n_segs
+};
+
+struct PhaseBufs {
+    float *seg, *segsum, *imp, *impsum, *of1;
+    int32_t *dip;
+};
+
+void phase_prep_launch(int n, PhItem *d_items, const CondItem *d_cond, const int32_t *d_poly, const uint8_t *d_H0,
+                       const uint8_t *d_H1, const uint8_t *d_flat, const uint8_t *d_flags, Locus *d_loci,
+                       hipStream_t s);
+void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
+                  const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
+                  uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
+                  hipStream_t s);
+// Padded state count the phasing kernel uses for stored messages (K x 8 floats each).
+int phase_kpt_states(int maxK);
+
+}  // namespace gh
