@@ -8,6 +8,7 @@
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
 #include "gh_phase.hpp"
+#include "gh_pbwt.hpp"
 
 namespace gh {
 void panel_create(const gh_panel_desc *d, gh_panel **out);
@@ -189,9 +190,26 @@ int gh_impute_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int3
     });
 }
 
-int gh_build_sparse_pbwt(const gh_panel_desc *, uint8_t *, int64_t *, int32_t *, int32_t *, int64_t *, int32_t *) {
-    g_err = "gh_build_sparse_pbwt: not built in this library version";
-    return GH_ERR_ARG;
+int gh_build_sparse_pbwt(const gh_panel_desc *d, uint8_t *ypacked, int64_t *ypacked_len, int32_t *asmall_ptr,
+                         int32_t *asmall_idx, int64_t *asmall_len, int32_t *n_common_hq) {
+    return guard([&] {
+        GH_CHECK(d && ypacked_len && asmall_len && n_common_hq, GH_ERR_ARG, "gh_build_sparse_pbwt: bad arguments");
+        std::vector<uint8_t> Y;
+        std::vector<int32_t> ap, ai;
+        int nhq = 0;
+        build_sparse_pbwt(d, Y, ap, ai, nhq);
+        const bool fill = ypacked && asmall_ptr && asmall_idx;
+        if (fill) {
+            GH_CHECK(*ypacked_len >= (int64_t)Y.size() && *asmall_len >= (int64_t)ai.size(), GH_ERR_CAPACITY,
+                     "gh_build_sparse_pbwt: output capacity");
+            std::memcpy(ypacked, Y.data(), Y.size());
+            std::memcpy(asmall_ptr, ap.data(), sizeof(int32_t) * ap.size());
+            std::memcpy(asmall_idx, ai.data(), sizeof(int32_t) * ai.size());
+        }
+        *ypacked_len = (int64_t)Y.size();
+        *asmall_len = (int64_t)ai.size();
+        *n_common_hq = nhq;
+    });
 }
 
 int gh_rephase_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int32_t *states, const uint8_t *flat,
@@ -255,10 +273,90 @@ int gh_rephase_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int
     });
 }
 
-int gh_pbwt_select(gh_panel *, int32_t, const uint8_t *, const uint8_t *, const int32_t *, int32_t, float, int32_t *,
-                   int32_t *, int32_t *, int64_t) {
-    g_err = "gh_pbwt_select: not built in this library version";
-    return GH_ERR_ARG;
+int gh_pbwt_select(gh_panel *P, int32_t T, const uint8_t *hap_alleles, const uint8_t *stored,
+                   const int32_t *start_idx, int32_t depth, float modulo_cm, int32_t *out_off, int32_t *out_cnt,
+                   int32_t *out_ids, int64_t cap, int32_t kpbwt, int32_t *out_states, int32_t *out_K) {
+    return guard([&] {
+        GH_CHECK(P && T > 0 && hap_alleles && stored && start_idx && out_off && out_cnt && out_ids, GH_ERR_ARG,
+                 "gh_pbwt_select: bad arguments");
+        GH_CHECK(depth > 0 && depth <= 64, GH_ERR_ARG, "gh_pbwt_select: depth must be in 1..64");
+        GH_CHECK(P->pbwt, GH_ERR_ARG, "gh_pbwt_select: panel has no sparse-PBWT stream");
+        hipStream_t s = P->stream;
+        const int L = P->L, nh = 2 * T, rec = 2 + 2 * depth;
+        int ngrp = 0;
+        for (int i = 0; i < P->Lhq; i++) ngrp += stored[i] != 0;
+        for (int h = 0; h < nh; h++)
+            GH_CHECK(start_idx[h] >= 0 && start_idx[h] < P->N, GH_ERR_ARG, "gh_pbwt_select: start index out of range");
+        const int evcap = pbwt_event_cap(P, modulo_cm, ngrp);
+        DBuf d_h, d_st, d_stored, d_cls, d_mod, d_ev, d_cnt, d_status, d_kp, d_sh, d_out, d_K;
+        PbwtRun R;
+        std::memset(&R, 0, sizeof(R));
+        R.n_haps = nh; R.n_samples = T; R.depth = depth; R.cap = evcap;
+        R.haps = dalloc_copy(d_h, hap_alleles, (size_t)nh * L, s);
+        R.start = dalloc_copy(d_st, start_idx, nh, s);
+        R.stored = dalloc_copy(d_stored, stored, P->Lhq, s);
+        std::vector<int32_t> cls(nh, 0);
+        R.hap_class = dalloc_copy(d_cls, cls.data(), nh, s);
+        R.modulo = dalloc_copy(d_mod, &modulo_cm, 1, s);
+        R.ev = (int32_t *)d_ev.reserve(sizeof(int32_t) * (size_t)nh * evcap * rec);
+        R.ev_cnt = (int32_t *)d_cnt.reserve(sizeof(int32_t) * nh);
+        R.status = (int32_t *)d_status.reserve(sizeof(int32_t));
+        GH_HIP(hipMemsetAsync(R.status, 0, sizeof(int32_t), s));
+        std::vector<int32_t> kp(T, kpbwt), sh(T);
+        for (int t = 0; t < T; t++) sh[t] = 2 * t;
+        if (out_states && out_K) {
+            GH_CHECK(kpbwt > 0, GH_ERR_ARG, "gh_pbwt_select: kpbwt must be positive");
+            R.kpbwt = dalloc_copy(d_kp, kp.data(), T, s);
+            R.sample_hap = dalloc_copy(d_sh, sh.data(), T, s);
+            R.out_stride = std::min(kpbwt, P->N);
+            R.out_states = (int32_t *)d_out.reserve(sizeof(int32_t) * (size_t)T * R.out_stride);
+            R.out_K = (int32_t *)d_K.reserve(sizeof(int32_t) * T);
+        }
+        pbwt_select_launch(P, R, s);
+        int32_t status = 0;
+        std::vector<int32_t> cnt(nh), ev((size_t)nh * evcap * rec);
+        GH_HIP(hipMemcpyAsync(&status, R.status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GH_HIP(hipMemcpyAsync(cnt.data(), R.ev_cnt, sizeof(int32_t) * nh, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipMemcpyAsync(ev.data(), R.ev, sizeof(int32_t) * ev.size(), hipMemcpyDeviceToHost, s));
+        if (out_states && out_K) {
+            std::vector<int32_t> ob((size_t)T * R.out_stride);
+            GH_HIP(hipMemcpyAsync(ob.data(), R.out_states, sizeof(int32_t) * ob.size(), hipMemcpyDeviceToHost, s));
+            GH_HIP(hipMemcpyAsync(out_K, R.out_K, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
+            GH_HIP(hipStreamSynchronize(s));
+            for (int t = 0; t < T; t++)
+                std::memcpy(out_states + (size_t)t * kpbwt, ob.data() + (size_t)t * R.out_stride,
+                            sizeof(int32_t) * out_K[t]);
+        }
+        GH_HIP(hipStreamSynchronize(s));
+        GH_CHECK(status == 0, GH_ERR_CAPACITY, "gh_pbwt_select: selection record capacity exceeded");
+        // rebuild pbwt_states[t][o] in the reference's push order: records ascending by
+        // (site, phase), first haplotype before second; per rank, down entry then up entry
+        int64_t o_ids = 0;
+        for (int t = 0; t < T; t++) {
+            std::vector<std::vector<int32_t>> lists(depth);
+            const int32_t *A = ev.data() + (size_t)(2 * t) * evcap * rec;
+            const int32_t *B = ev.data() + (size_t)(2 * t + 1) * evcap * rec;
+            int ia = 0, ib = 0;
+            const int na = cnt[2 * t], nb = cnt[2 * t + 1];
+            while (ia < na || ib < nb) {
+                const int32_t *r;
+                if (ia < na && (ib >= nb || A[(size_t)ia * rec] <= B[(size_t)ib * rec])) r = A + (size_t)ia++ * rec;
+                else r = B + (size_t)ib++ * rec;
+                const int nd = r[1] & 0xffff, nu = r[1] >> 16;
+                for (int o = 0; o < depth; o++) {
+                    if (o < nd) lists[o].push_back(r[2 + o]);
+                    if (o < nu) lists[o].push_back(r[2 + depth + o]);
+                }
+            }
+            for (int o = 0; o < depth; o++) {
+                out_off[t * depth + o] = (int32_t)o_ids;
+                out_cnt[t * depth + o] = (int32_t)lists[o].size();
+                GH_CHECK(o_ids + (int64_t)lists[o].size() <= cap, GH_ERR_CAPACITY, "gh_pbwt_select: ids capacity");
+                std::memcpy(out_ids + o_ids, lists[o].data(), sizeof(int32_t) * lists[o].size());
+                o_ids += lists[o].size();
+            }
+        }
+    });
 }
 
 int gh_phase_run(gh_panel *, int32_t, const uint8_t *, const uint8_t *, const gh_params *, const int32_t *, float *,
@@ -272,6 +370,8 @@ int gh_phase_run(gh_panel *, int32_t, const uint8_t *, const uint8_t *, const gh
 namespace gh {
 void panel_destroy(gh_panel *p) {
     hipStream_t s = p->stream;
+    delete p->pbwt;
+    p->pbwt = nullptr;
     delete p;
     if (s) (void)hipStreamDestroy(s);
 }

@@ -12,6 +12,7 @@
 
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
+#include "gh_pbwt.hpp"
 
 namespace gh {
 
@@ -95,9 +96,9 @@ void panel_create(const gh_panel_desc *d, gh_panel **out) {
         dalloc_copy(P->d_c2t, P->common2tot.data(), P->Lc, s);
         dalloc_copy(P->d_sm, d->common_words, (size_t)P->Lc * P->W64, s);
         dalloc_copy(P->d_shap_ptr, shap_ptr.data(), N + 1, s);
-        dalloc_copy(P->d_shap_idx, shap_idx.data(), std::max<int64_t>(nr, 1), s);
+        dalloc_copy(P->d_shap_idx, shap_idx.data(), (size_t)nr, s);
         dalloc_copy(P->d_rare_ptr, d->rare_ptr, L + 1, s);
-        dalloc_copy(P->d_rare_idx, d->rare_idx, std::max<int64_t>(nr, 1), s);
+        dalloc_copy(P->d_rare_idx, d->rare_idx, (size_t)nr, s);
         P->d_hm.reserve(sizeof(uint64_t) * std::max<size_t>((size_t)N * P->Wc, 1));
         if (P->Lc > 0) {
             dim3 grid(P->Wc, P->W64);
@@ -111,12 +112,14 @@ void panel_create(const gh_panel_desc *d, gh_panel **out) {
             P->asmall_idx.assign(d->asmall_idx, d->asmall_idx + P->asmall_ptr.back());
         }
         GH_HIP(hipStreamSynchronize(s));
-        P->device_bytes = (int64_t)(P->d_cm.bytes + P->d_cmpos.bytes + P->d_flags.bytes + P->d_crow.bytes +
+        if (!P->ypacked.empty()) pbwt_decode(P);
+        P->device_bytes += (int64_t)(P->d_cm.bytes + P->d_cmpos.bytes + P->d_flags.bytes + P->d_crow.bytes +
                                     P->d_c2t.bytes + P->d_sm.bytes + P->d_hm.bytes + P->d_shap_ptr.bytes +
                                     P->d_shap_idx.bytes + P->d_rare_ptr.bytes + P->d_rare_idx.bytes);
         *out = P;
     } catch (...) {
         if (P->stream) (void)hipStreamDestroy(P->stream);
+        delete P->pbwt;
         delete P;
         throw;
     }

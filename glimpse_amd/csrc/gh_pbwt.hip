@@ -1,0 +1,597 @@
+// gh_pbwt.hip -- sparse-PBWT conditioning-state selection on the GPU.
+//
+// split_reference side (host, once per panel):
+//   build_sparse_pbwt  -- ref_haplotype_set::build_sparsePBWT
+//                         (common/src/containers/ref_haplotype_set.cpp:47-193): pack3
+//                         run-length stream of every non-LQ site in prefix order,
+//                         with a "small" PBWT over the carriers of each rare run.
+//   pbwt_decode        -- replays the stream once and keeps, per non-LQ site, the
+//                         column bits in prefix order with per-word popcount prefixes
+//                         (so the reference's rank array V[x] is one popcount), and
+//                         the prefix arrays A that selectK reads.  All of it stays
+//                         resident in HBM; the per-iteration sweep then needs no
+//                         decode at all.
+// Per Gibbs iteration (device):
+//   pbwt_walk_kernel   -- one thread per target haplotype replays
+//                         haplotype_set::matchHapsFromCompressedPBWTSmall
+//                         (phase/src/containers/haplotype_set.cpp:317-728) for that
+//                         haplotype alone: LF-mapping of the insertion index and
+//                         match interval, resets, rare-run re-indexing, and selectK
+//                         events recorded as (site, phase) keyed records.
+//   pbwt_compact_kernel -- one wave per target merges its two haplotypes' records
+//                         newest first, rank by rank, into the first Kpbwt distinct
+//                         panel haplotypes (conditioning_set.cpp:96-104), then
+//                         writes them sorted.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+#include "gh_device.hpp"
+#include "gh_internal.hpp"
+#include "gh_pbwt.hpp"
+
+namespace gh {
+
+/* ------------------------------------------------------------------ host: pack3 */
+// pack3 code (ref_haplotype_set.h:45-85): bit 7 = symbol; 0x40 clear -> count in
+// low 6 bits; 0x40 set and 0x20 clear -> (low 5 bits) << 6; both set -> << 11.
+static int p3dec(uint8_t z) {
+    const int c = z & 0x7f;
+    return c < 64 ? c : (c < 96 ? (c - 64) << 6 : (c - 96) << 11);
+}
+static void p3add(uint8_t sym, uint32_t n, std::vector<uint8_t> &out) {
+    const uint8_t y = (uint8_t)(sym << 7);
+    const uint32_t M1 = 64, M2 = (95 - 63) << 6, M3 = (127 - 96) << 11;
+    while (n >= M3) { out.push_back(y | 0x7f); n -= M3; }
+    if (n >= M2) { out.push_back((uint8_t)(y | 0x60 | (n >> 11))); n &= 0x7ff; }
+    if (n >= M1) { out.push_back((uint8_t)(y | 0x40 | (n >> 6))); n &= 0x3f; }
+    if (n) out.push_back((uint8_t)(y | n));
+}
+
+struct HostSites {
+    int L, N, Lc, Lhq;
+    std::vector<uint8_t> common, major, lq;
+    std::vector<uint32_t> cref, calt;
+    std::vector<std::vector<int32_t>> carriers;  // per site, ascending
+    const uint64_t *words;                        // [Lc][W64]
+    int W64;
+    std::vector<int32_t> crow;
+};
+
+static HostSites host_sites(const gh_panel_desc *d) {
+    HostSites H;
+    H.L = d->n_sites; H.N = d->n_haps; H.W64 = (H.N + 63) / 64;
+    H.words = d->common_words;
+    H.common.resize(H.L); H.major.resize(H.L); H.lq.resize(H.L); H.crow.assign(H.L, -1);
+    H.cref.assign(d->cref, d->cref + H.L); H.calt.assign(d->calt, d->calt + H.L);
+    H.carriers.resize(H.L);
+    H.Lc = H.Lhq = 0;
+    const float maf = d->sparse_maf > 0 ? d->sparse_maf : 0.001f;
+    for (int l = 0; l < H.L; l++) {
+        const uint32_t cr = d->cref[l], ca = d->calt[l];
+        const double MAF = std::min(cr * 1.0f / (cr + ca), ca * 1.0f / (cr + ca));
+        H.common[l] = MAF >= maf;
+        H.major[l] = ca > cr;
+        H.lq[l] = d->lq ? (d->lq[l] != 0) : 0;
+        if (H.common[l]) { H.crow[l] = H.Lc++; H.Lhq += !H.lq[l]; }
+        H.carriers[l].assign(d->rare_idx + d->rare_ptr[l], d->rare_idx + d->rare_ptr[l + 1]);
+        std::sort(H.carriers[l].begin(), H.carriers[l].end());
+    }
+    return H;
+}
+static inline int hbit(const HostSites &H, int row, int h) {
+    return (int)((H.words[(size_t)row * H.W64 + (h >> 6)] >> (h & 63)) & 1ull);
+}
+
+void build_sparse_pbwt(const gh_panel_desc *d, std::vector<uint8_t> &Y, std::vector<int32_t> &aptr,
+                       std::vector<int32_t> &aidx, int &n_hq) {
+    const HostSites H = host_sites(d);
+    const int N = H.N;
+    n_hq = H.Lhq;
+    std::vector<std::vector<int32_t>> asmall(H.Lhq + 1);
+    std::vector<int32_t> A(N), B(N), ref_idx(N), map_big_small(N, -1), sA, sB;
+    std::iota(A.begin(), A.end(), 0);
+    std::iota(ref_idx.begin(), ref_idx.end(), 0);
+    std::vector<uint8_t> small_hap;
+    Y.clear();
+    int l_hq = 0, last_k = -1;
+    for (int k = 0; k < H.L; k++) {
+        if (H.lq[k]) continue;
+        if (H.common[k]) {
+            if (!sA.empty() && !(last_k >= 0 && H.common[last_k])) {
+                // rare run -> common: non-carriers keep their order, carriers follow in small order
+                std::vector<uint8_t> in_small(N, 0);
+                for (int p : asmall[l_hq]) in_small[p] = 1;
+                int nz = 0;
+                for (int p = 0; p < N; p++) if (!in_small[p]) B[nz++] = A[p];
+                for (size_t j = 0; j < sA.size(); j++) B[nz++] = A[asmall[l_hq][sA[j]]];
+                A.swap(B);
+            }
+            const int row = H.crow[k];
+            int occ[2] = {0, (int)H.cref[k]};
+            int m = 0;
+            while (m < N) {
+                const int z = hbit(H, row, A[m]);
+                const int m0 = m;
+                while (++m < N && hbit(H, row, A[m]) == z) {}
+                for (int j = m0; j < m; j++) B[occ[z]++] = A[j];
+                p3add((uint8_t)z, (uint32_t)(m - m0), Y);
+            }
+            A.swap(B);
+            for (int p = 0; p < N; p++) ref_idx[A[p]] = p;
+            l_hq++;
+        } else {
+            if (last_k < 0 || H.common[last_k]) {
+                // new rare run: carriers of any non-LQ rare site until the next non-LQ common site
+                if (l_hq > 0) for (int p : asmall[l_hq - 1]) map_big_small[p] = -1;
+                std::vector<int32_t> haps;
+                for (int kk = k; kk < H.L; kk++) {
+                    if (H.lq[kk]) continue;
+                    if (H.common[kk]) break;
+                    haps.insert(haps.end(), H.carriers[kk].begin(), H.carriers[kk].end());
+                }
+                std::sort(haps.begin(), haps.end());
+                haps.erase(std::unique(haps.begin(), haps.end()), haps.end());
+                std::map<int, int> by_pos;
+                for (int h : haps) by_pos.emplace(ref_idx[h], h);
+                asmall[l_hq].clear();
+                int j = 0;
+                for (auto &e : by_pos) { map_big_small[e.second] = j++; asmall[l_hq].push_back(e.first); }
+                sA.resize(asmall[l_hq].size());
+                std::iota(sA.begin(), sA.end(), 0);
+                sB.resize(sA.size());
+                small_hap.resize(sA.size());
+            }
+            std::fill(small_hap.begin(), small_hap.end(), H.major[k]);
+            for (int h : H.carriers[k]) small_hap[map_big_small[h]] = !H.major[k];
+            const int ns = (int)sA.size();
+            const int ref = H.major[k] ? N - (int)H.calt[k] : ns - (int)H.calt[k];
+            int occ[2] = {0, ref};
+            int m = 0;
+            while (m < ns) {
+                const int z = small_hap[sA[m]];
+                const int m0 = m;
+                while (++m < ns && small_hap[sA[m]] == z) {}
+                for (int j = m0; j < m; j++) sB[occ[z]++] = sA[j];
+                p3add((uint8_t)z, (uint32_t)(m - m0), Y);
+            }
+            sA.swap(sB);
+        }
+        last_k = k;
+    }
+    aptr.assign(H.Lhq + 2, 0);
+    for (int l = 0; l <= H.Lhq; l++) aptr[l + 1] = aptr[l] + (int32_t)asmall[l].size();
+    aidx.clear();
+    for (auto &v : asmall) aidx.insert(aidx.end(), v.begin(), v.end());
+}
+
+/* ------------------------------------------------------------------ host: decode */
+void pbwt_decode(gh_panel *P) {
+    GH_CHECK(!P->ypacked.empty(), GH_ERR_ARG, "panel has no sparse-PBWT stream");
+    const int N = P->N, L = P->L;
+    auto *D = new gh_panel::PbwtDev();
+    P->pbwt = D;
+    std::vector<SiteOp> ops;
+    std::vector<uint64_t> bits;
+    std::vector<uint32_t> cnt;
+    std::vector<int32_t> Astore;   // [arrays][N]
+    const int Lhq = P->Lhq;
+    Astore.reserve((size_t)(Lhq + 2) * N);
+    Astore.resize((size_t)Lhq * N);
+    std::vector<int32_t> A(N), B(N), sA, sB;
+    std::iota(A.begin(), A.end(), 0);
+    const uint8_t *y = P->ypacked.data(), *yend = y + P->ypacked.size();
+    int l_hq = 0, last_k = -1, prev_ref = 0, n_extra = 0, n_runs = 0;
+    auto push_array = [&](const std::vector<int32_t> &a) {
+        Astore.insert(Astore.end(), a.begin(), a.end());
+        return Lhq + n_extra++;
+    };
+    int iota_arr = -1;
+    auto decode_col = [&](int n, int ref, std::vector<int32_t> &a, std::vector<int32_t> &b) {
+        // appends column bits (prefix order) + popcount prefixes, permutes a -> b -> a
+        const int W = (n + 63) / 64;
+        const size_t base = bits.size();
+        bits.resize(base + W, 0ull);
+        cnt.resize(base + W + 1, 0u);
+        int occ[2] = {0, ref};
+        int m = 0;
+        while (m < n) {
+            GH_CHECK(y < yend, GH_ERR_ARG, "sparse-PBWT stream truncated");
+            const uint8_t z = *y++;
+            const int c = p3dec(z), sym = z >> 7;
+            GH_CHECK(c > 0 && m + c <= n, GH_ERR_ARG, "sparse-PBWT stream inconsistent with the panel");
+            for (int j = m; j < m + c; j++) {
+                if (sym) bits[base + (j >> 6)] |= 1ull << (j & 63);
+                b[occ[sym]++] = a[j];
+            }
+            m += c;
+        }
+        GH_CHECK(occ[0] == ref && occ[1] == n, GH_ERR_ARG, "sparse-PBWT symbol counts inconsistent with AC/AN");
+        uint32_t acc = 0;
+        for (int w = 0; w < W; w++) { cnt[base + w] = acc; acc += (uint32_t)__builtin_popcountll(bits[base + w]); }
+        cnt[base + W] = acc;
+        // pad so the count array stays aligned with the word array
+        bits.push_back(0ull);
+        a.swap(b);
+        return (int64_t)base;
+    };
+    for (int k = 0; k < L; k++) {
+        const uint8_t fl = P->flags[k];
+        if (fl & SF_LQ) continue;
+        SiteOp op;
+        std::memset(&op, 0, sizeof(op));
+        op.k = k;
+        if (fl & SF_COMMON) {
+            op.kind = 0;
+            op.l = l_hq;
+            const int ns = P->asmall_ptr[l_hq + 1] - P->asmall_ptr[l_hq];
+            const int prev_out = l_hq > 0 ? l_hq - 1 : -1;
+            if (prev_out < 0 && iota_arr < 0) {
+                std::vector<int32_t> io(N);
+                std::iota(io.begin(), io.end(), 0);
+                iota_arr = push_array(io);
+            }
+            op.a_prev = prev_out >= 0 ? prev_out : iota_arr;
+            if (ns > 0 && last_k >= 0 && !(P->flags[last_k] & SF_COMMON)) {
+                op.init = 1;
+                const int32_t *si = P->asmall_idx.data() + P->asmall_ptr[l_hq];
+                std::vector<uint8_t> in_small(N, 0);
+                for (int j = 0; j < ns; j++) in_small[si[j]] = 1;
+                int nz = 0;
+                for (int p = 0; p < N; p++) if (!in_small[p]) B[nz++] = A[p];
+                for (size_t j = 0; j < sA.size(); j++) B[nz++] = sA[j];
+                A.swap(B);
+                op.a_in = push_array(A);
+            } else {
+                op.a_in = op.a_prev;
+            }
+            op.ref = (int)P->cref[k];
+            op.prev_ref = prev_ref;
+            op.nbits = N;
+            op.roff = decode_col(N, op.ref, A, B);
+            std::memcpy(Astore.data() + (size_t)l_hq * N, A.data(), sizeof(int32_t) * N);
+            op.a_out = l_hq;
+            prev_ref = op.ref;
+            l_hq++;
+        } else {
+            op.kind = 1;
+            op.l = l_hq;
+            op.major = (fl & SF_MAJOR) ? 1 : 0;
+            const int ns = P->asmall_ptr[l_hq + 1] - P->asmall_ptr[l_hq];
+            if (last_k < 0 || (P->flags[last_k] & SF_COMMON)) {
+                op.init = 1;
+                n_runs++;
+                const int32_t *si = P->asmall_idx.data() + P->asmall_ptr[l_hq];
+                sA.resize(ns);
+                for (int j = 0; j < ns; j++) sA[j] = A[si[j]];
+                sB.resize(ns);
+            }
+            op.ref = op.major ? (int)P->cref[k] : ns - (int)P->calt[k];
+            op.nbits = ns;
+            op.roff = decode_col(ns, op.ref, sA, sB);
+        }
+        ops.push_back(op);
+        last_k = k;
+    }
+    GH_CHECK(y == yend, GH_ERR_ARG, "sparse-PBWT stream longer than the panel");
+    D->n_ops = (int)ops.size();
+    D->n_runs = n_runs;
+    D->n_arrays = Lhq + n_extra;
+    hipStream_t s = P->stream;
+    dalloc_copy(D->ops, ops.data(), ops.size(), s);
+    dalloc_copy(D->bits, bits.data(), bits.size(), s);
+    dalloc_copy(D->cnt, cnt.data(), cnt.size(), s);
+    dalloc_copy(D->A, Astore.data(), Astore.size(), s);
+    dalloc_copy(D->aptr, P->asmall_ptr.data(), P->asmall_ptr.size(), s);
+    dalloc_copy(D->aidx, P->asmall_idx.data(), P->asmall_idx.size(), s);
+    GH_HIP(hipStreamSynchronize(s));
+    P->device_bytes += (int64_t)(D->ops.bytes + D->bits.bytes + D->cnt.bytes + D->A.bytes + D->aptr.bytes +
+                                 D->aidx.bytes);
+}
+
+/* ------------------------------------------------------------------ device: walk */
+struct WalkArgs {
+    const SiteOp *ops;
+    int n_ops;
+    const uint64_t *bits;
+    const uint32_t *cnt;
+    const int32_t *A;
+    int N;
+    const int32_t *aptr, *aidx;
+    const float *cm_pos;
+    const int32_t *c2t;
+    const uint8_t *haps;   // [n_haps][L]
+    int L;
+    int n_haps;
+    const int32_t *start;  // [n_haps]
+    const uint8_t *stored; // [n_classes][Lhq]
+    const int32_t *hap_class;  // [n_haps]
+    const float *modulo;   // [n_classes]
+    int Lhq;
+    int depth;
+    int32_t *ev;           // [n_haps][cap][rec]
+    int cap;
+    int32_t *ev_cnt;       // [n_haps]
+    int32_t *status;
+};
+
+__device__ __forceinline__ int rank1(const uint64_t *bits, const uint32_t *cnt, int64_t off, int x) {
+    const int w = x >> 6, b = x & 63;
+    int r = (int)cnt[off + w];
+    if (b) r += __popcll(bits[off + w] & ((1ull << b) - 1ull));
+    return r;
+}
+__device__ __forceinline__ int lower_count(const int32_t *a, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void pbwt_walk_kernel(WalkArgs w) {
+    const int h = blockIdx.x * 64 + threadIdx.x;
+    if (h >= w.n_haps) return;
+    const int N = w.N, depth = w.depth, rec = 2 + 2 * depth;
+    const uint8_t *hap = w.haps + (size_t)h * w.L;
+    const uint8_t *stored = w.stored + (size_t)w.hap_class[h] * w.Lhq;
+    const float modulo = w.modulo[w.hap_class[h]];
+    int32_t *ev = w.ev + (size_t)h * w.cap * rec;
+    int nev = 0;
+    int idx = w.start[h], f = 0, g = N, last_reset = 0, last_rare = 0, tar = 0;
+    int sidx = 0, sf = 0, sg = 0;
+    bool in_rare = false;
+    bool overflow = false;
+
+    // haplotype_set::selectK (haplotype_set.cpp:682-728) as one record
+    auto selectK = [&](int key, int ref, const int32_t *arr, int a) {
+        const int d_up = idx - f, d_down = g - idx, k1 = depth;
+        int nh_up = 0, nh_down = 0;
+        if (d_up < k1 && d_down < k1) { nh_up = d_up; nh_down = d_down; }
+        else if (d_up >= k1 && d_down >= k1) { nh_up = k1; nh_down = k1; }
+        else if (d_up < k1) nh_down = k1 - d_up;
+        else if (d_down < k1) nh_up = k1 - d_down;
+        if (nev >= w.cap) { overflow = true; return; }
+        int32_t *r = ev + (size_t)nev * rec;
+        int nd = 0, nu = 0;
+        for (int j = 0; j < nh_down; j++) {
+            const int i2 = idx + j;
+            if (i2 < N && ((i2 >= ref) == (a != 0))) r[2 + nd++] = arr[i2]; else break;
+        }
+        for (int j = 0; j < nh_up; j++) {
+            const int i2 = idx - (j + 1);
+            if (i2 >= 0 && ((i2 >= ref) == (a != 0))) r[2 + depth + nu++] = arr[i2]; else break;
+        }
+        if (nd == 0 && nu == 0) return;   // nothing pushed: no record needed
+        r[0] = key;
+        r[1] = nd | (nu << 16);
+        nev++;
+    };
+
+    for (int o = 0; o < w.n_ops; o++) {
+        const SiteOp op = w.ops[o];
+        const int k = op.k;
+        if (op.kind == 0) {
+            if (op.init) {
+                // haplotype_set::init_common (haplotype_set.cpp:562-630)
+                const int ns = w.aptr[op.l + 1] - w.aptr[op.l];
+                const int32_t *si = w.aidx + w.aptr[op.l];
+                const int32_t *Aprev = w.A + (size_t)op.a_prev * N;
+                int id, fd, gd;
+                if (in_rare) {
+                    id = N - ns + sidx;
+                    fd = last_reset >= last_rare ? 0 : N - ns + sf;
+                    gd = N - ns + sg;
+                    if (gd <= fd) {
+                        selectK(2 * k, op.prev_ref, Aprev, tar);
+                        fd = 0; gd = N;
+                        last_reset = k - 1;
+                    }
+                } else {
+                    const int c0 = lower_count(si, ns, f), c1 = lower_count(si, ns, g), c2 = lower_count(si, ns, idx);
+                    id = idx - c2; fd = f - c0; gd = g - c1;
+                    if (gd <= fd) {
+                        const int ps = op.l >= 1 ? w.c2t[op.l - 1] : 0;
+                        if (w.cm_pos[ps] - w.cm_pos[last_reset] > modulo / 5.0f) selectK(2 * k, op.prev_ref, Aprev, tar);
+                        fd = 0; gd = N - ns;
+                        last_reset = ps;
+                    }
+                }
+                idx = id; f = fd; g = gd;
+                in_rare = false;
+            }
+            // haplotype_set::select_common_pd_fg (haplotype_set.cpp:445-490)
+            const int a = hap[k];
+            const int prev = tar;
+            tar = a;
+            const int ref = op.ref;
+            const int vi = rank1(w.bits, w.cnt, op.roff, idx);
+            const int vf = rank1(w.bits, w.cnt, op.roff, f);
+            const int vg = rank1(w.bits, w.cnt, op.roff, g);
+            const int id = a ? ref + vi : idx - vi;
+            int fd = a ? ref + vf : f - vf;
+            int gd = a ? ref + vg : g - vg;
+            bool reset = false;
+            const bool st = stored[op.l] != 0;
+            if (gd <= fd) {
+                if (st || w.cm_pos[k > 0 ? k - 1 : 0] - w.cm_pos[last_reset] > modulo / 2.0f)
+                    selectK(2 * k + 1, op.prev_ref, w.A + (size_t)op.a_in * N, prev);
+                fd = a * ref;
+                gd = a * N + (1 - a) * ref;
+                last_reset = k;
+                reset = true;
+            }
+            idx = id; f = fd; g = gd;
+            if (!reset && st) {
+                selectK(2 * k + 1, ref, w.A + (size_t)op.a_out * N, a);
+                f = a * ref;
+                g = a * N + (1 - a) * ref;
+                last_reset = k;
+            }
+        } else {
+            const int ns = op.nbits;
+            if (op.init) {
+                // haplotype_set::init_rare (haplotype_set.cpp:632-680)
+                in_rare = false;
+                if (ns > 0) {
+                    for (int j = o; j < w.n_ops; j++) {
+                        const SiteOp oj = w.ops[j];
+                        if (oj.kind == 0) break;
+                        if (hap[oj.k] != oj.major) { in_rare = true; break; }
+                    }
+                }
+                if (in_rare) {
+                    const int32_t *si = w.aidx + w.aptr[op.l];
+                    int fd = lower_count(si, ns, f), gd = lower_count(si, ns, g);
+                    const int c2 = lower_count(si, ns, idx);
+                    if (gd <= fd) { fd = 0; gd = ns; last_reset = k; }
+                    sidx = c2; sf = fd; sg = gd;
+                }
+            }
+            if (in_rare) {
+                // haplotype_set::select_rare_pd_fg (haplotype_set.cpp:492-560)
+                const int ref = op.ref;
+                const int vi = rank1(w.bits, w.cnt, op.roff, sidx);
+                const int vf = rank1(w.bits, w.cnt, op.roff, sf);
+                const int vg = rank1(w.bits, w.cnt, op.roff, sg);
+                int si, fd, gd;
+                if (hap[k] != op.major) {
+                    last_rare = k;
+                    si = ref + vi; fd = ref + vf; gd = ref + vg;
+                    if (gd <= fd) { fd = ref; gd = ns; last_reset = k; }
+                } else {
+                    si = sidx - vi; fd = sf - vf; gd = sg - vg;
+                    if (gd <= fd) { fd = 0; gd = ref; last_reset = k; }
+                }
+                sidx = si; sf = fd; sg = gd;
+            }
+        }
+    }
+    w.ev_cnt[h] = nev;
+    if (overflow) atomicExch(w.status, 2);
+}
+
+/* ------------------------------------------------------------------ device: compact */
+struct CompactArgs {
+    const int32_t *ev;
+    const int32_t *ev_cnt;
+    int cap, depth, N;
+    const int32_t *kpbwt;       // [n_samples]
+    const int32_t *sample_hap;  // [n_samples]: first haplotype index (second is +1)
+    int32_t *out_states;        // [n_samples][out_stride]
+    int out_stride;
+    int32_t *out_K;             // [n_samples]
+};
+
+__global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
+    extern __shared__ uint64_t bm[];
+    const int t = blockIdx.x, lane = threadIdx.x;
+    const int Wn = (c.N + 63) / 64;
+    for (int i = lane; i < Wn; i += 64) bm[i] = 0ull;
+    __syncthreads();
+    const int rec = 2 + 2 * c.depth;
+    const int h0 = c.sample_hap[t], h1 = h0 + 1;
+    const int32_t *eA = c.ev + (size_t)h0 * c.cap * rec, *eB = c.ev + (size_t)h1 * c.cap * rec;
+    const int nA = c.ev_cnt[h0], nB = c.ev_cnt[h1];
+    const int kp = c.kpbwt[t];
+    if (lane == 0) {
+        int count = 0;
+        for (int o = 0; o < c.depth && count < kp; o++) {
+            int ia = nA - 1, ib = nB - 1;
+            while ((ia >= 0 || ib >= 0) && count < kp) {
+                const int32_t *r;
+                // newest first; at equal key the second haplotype pushed last
+                if (ib >= 0 && (ia < 0 || eB[(size_t)ib * rec] >= eA[(size_t)ia * rec])) r = eB + (size_t)ib-- * rec;
+                else r = eA + (size_t)ia-- * rec;
+                const int nd = r[1] & 0xffff, nu = r[1] >> 16;
+                if (o < nu) {
+                    const int x = r[2 + c.depth + o];
+                    const uint64_t m = 1ull << (x & 63);
+                    if (!(bm[x >> 6] & m)) { bm[x >> 6] |= m; count++; }
+                }
+                if (o < nd && count < kp) {
+                    const int x = r[2 + o];
+                    const uint64_t m = 1ull << (x & 63);
+                    if (!(bm[x >> 6] & m)) { bm[x >> 6] |= m; count++; }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // sorted emission: wave prefix over per-word popcounts
+    int base = 0;
+    int32_t *out = c.out_states + (size_t)t * c.out_stride;
+    for (int w0 = 0; w0 < Wn; w0 += 64) {
+        const int wi = w0 + lane;
+        const uint64_t x = wi < Wn ? bm[wi] : 0ull;
+        const int pc = __popcll(x);
+        int incl = pc;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(incl, d);
+            if (lane >= d) incl += y;
+        }
+        int p = base + incl - pc;
+        uint64_t v = x;
+        while (v) {
+            const int b = __ffsll((long long)v) - 1;
+            out[p++] = wi * 64 + b;
+            v &= v - 1;
+        }
+        base += __shfl(incl, 63);
+    }
+    if (lane == 0) c.out_K[t] = base;
+}
+
+void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
+    auto *D = P->pbwt;
+    GH_CHECK(D, GH_ERR_ARG, "panel was created without a sparse-PBWT stream");
+    WalkArgs w;
+    w.ops = D->ops.as<SiteOp>(); w.n_ops = D->n_ops;
+    w.bits = D->bits.as<uint64_t>(); w.cnt = D->cnt.as<uint32_t>();
+    w.A = D->A.as<int32_t>(); w.N = P->N;
+    w.aptr = D->aptr.as<int32_t>(); w.aidx = D->aidx.as<int32_t>();
+    w.cm_pos = P->d_cmpos.as<float>(); w.c2t = P->d_c2t.as<int32_t>();
+    w.haps = R.haps; w.L = P->L; w.n_haps = R.n_haps; w.start = R.start;
+    w.stored = R.stored; w.hap_class = R.hap_class; w.modulo = R.modulo; w.Lhq = P->Lhq;
+    w.depth = R.depth; w.ev = R.ev; w.cap = R.cap; w.ev_cnt = R.ev_cnt; w.status = R.status;
+    hipLaunchKernelGGL(pbwt_walk_kernel, dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
+    GH_HIP(hipGetLastError());
+    if (R.out_states) {
+        CompactArgs c;
+        c.ev = R.ev; c.ev_cnt = R.ev_cnt; c.cap = R.cap; c.depth = R.depth; c.N = P->N;
+        c.kpbwt = R.kpbwt; c.sample_hap = R.sample_hap; c.out_states = R.out_states; c.out_stride = R.out_stride;
+        c.out_K = R.out_K;
+        const size_t shm = sizeof(uint64_t) * ((P->N + 63) / 64);
+        hipLaunchKernelGGL(pbwt_compact_kernel, dim3(R.n_samples), dim3(64), shm, s, c);
+        GH_HIP(hipGetLastError());
+    }
+}
+
+int pbwt_event_cap(gh_panel *P, float modulo, int n_groups) {
+    const float len = P->cm_pos.back() - P->cm_pos.front();
+    const double c = (double)n_groups + (P->pbwt ? P->pbwt->n_runs : 0) + 8.0 * (len / modulo) + 64.0;
+    return (int)std::min(2.0 * c, 1.0e7);
+}
+
+// allocatePBWT group boundaries and modulo auto-adjust (haplotype_set.cpp:229-303)
+float pbwt_groups(gh_panel *P, int depth, float modulo_in, int kpbwt, std::vector<int> &grp) {
+    float modulo = modulo_in;
+    const float length = P->cm_pos.back() - P->cm_pos[0];
+    if ((length / modulo) * 2 * depth < kpbwt)
+        while ((length / modulo) * 2 * depth < kpbwt && modulo > 0.02) modulo /= 2;
+    grp.clear();
+    for (int l_hq = 0, l_all = 0, src = 0; l_all < P->Lc; l_all++) {
+        const int k = P->common2tot[l_all];
+        if (P->flags[k] & SF_LQ) continue;
+        const int tmp = (int)std::round(P->cm_pos[k] / modulo);
+        if (src != tmp) { src = tmp; if (l_hq > 0) grp.push_back(l_hq); }
+        ++l_hq;
+    }
+    if (grp.empty() || grp.back() < P->Lhq) grp.push_back(P->Lhq);
+    return modulo;
+}
+
+}  // namespace gh

@@ -58,7 +58,7 @@ def lib():
         L.gh_condset_batch.argtypes = [vp, i32, vp, vp, i32, vp, vp, i64, vp, vp, i64, vp]
         L.gh_impute_batch.argtypes = [vp, i32, vp, vp, vp, vp, f32, i32, vp]
         L.gh_rephase_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp, f32, i32, u32, i32, vp]
-        L.gh_pbwt_select.argtypes = [vp, i32, vp, vp, vp, i32, f32, vp, vp, vp, i64]
+        L.gh_pbwt_select.argtypes = [vp, i32, vp, vp, vp, i32, f32, vp, vp, vp, i64, i32, vp, vp]
         L.gh_phase_run.argtypes = [vp, i32, vp, vp, ctypes.POINTER(Params), vp, vp, vp, vp, vp, vp, vp,
                                    ctypes.POINTER(RunStats)]
         _lib = L
@@ -200,18 +200,25 @@ class Panel:
                                       _p(b), ctypes.c_float(err_phase), ne, seed, it, _p(tg)))
         return a, b
 
-    def pbwt_select(self, hap_alleles, stored, start_idx, depth=12, modulo=0.1, cap=None):
+    def pbwt_select(self, hap_alleles, stored, start_idx, depth=12, modulo=0.1, cap=None, kpbwt=0):
+        """Returns the pbwt_states lists per target and, when kpbwt > 0, the
+        compacted sorted conditioning set per target."""
         haps = np.ascontiguousarray(hap_alleles, np.uint8)
         T = haps.shape[0] // 2
         cap = cap or max(1 << 16, 64 * depth * T * 64)
         off = np.zeros(T * depth, np.int32)
         cnt = np.zeros(T * depth, np.int32)
         ids = np.zeros(cap, np.int32)
+        st = np.zeros((T, max(kpbwt, 1)), np.int32) if kpbwt > 0 else None
+        K = np.zeros(T, np.int32) if kpbwt > 0 else None
         _check(lib().gh_pbwt_select(self.h, T, _p(haps), _p(np.ascontiguousarray(stored, np.uint8)),
                                     _p(np.ascontiguousarray(start_idx, np.int32)), depth, ctypes.c_float(modulo),
-                                    _p(off), _p(cnt), _p(ids), cap))
-        return [[ids[off[t * depth + d]: off[t * depth + d] + cnt[t * depth + d]].copy() for d in range(depth)]
-                for t in range(T)]
+                                    _p(off), _p(cnt), _p(ids), cap, int(kpbwt), _p(st), _p(K)))
+        lists = [[ids[off[t * depth + d]: off[t * depth + d] + cnt[t * depth + d]].copy() for d in range(depth)]
+                 for t in range(T)]
+        if kpbwt > 0:
+            return lists, [st[t, :K[t]].copy() for t in range(T)]
+        return lists
 
     def phase_run(self, gl, flat, params, kpbwt_per_target=None):
         gl = np.ascontiguousarray(gl, np.uint8)

@@ -124,10 +124,14 @@ int gh_rephase_batch(gh_panel *p, int32_t n_items, const int32_t *state_ptr, con
 /* haplotype_set::matchHapsFromCompressedPBWTSmall
  * (phase/src/containers/haplotype_set.cpp:317-394) for 2T target haplotypes:
  * hap_alleles [2T][L], stored [n_common_hq], start_idx [2T].  Output: the
- * pbwt_states[T][depth] lists (off/cnt [T*depth], ids). */
+ * pbwt_states[T][depth] lists (off/cnt [T*depth], ids).  When out_states is
+ * not NULL, also the conditioning set conditioning_set::compactSelection
+ * takes from those lists (phase/src/containers/conditioning_set.cpp:96-104):
+ * out_K[t] sorted ids in out_states[t*kpbwt ..]. */
 int gh_pbwt_select(gh_panel *p, int32_t T, const uint8_t *hap_alleles, const uint8_t *stored,
                    const int32_t *start_idx, int32_t depth, float modulo_cm,
-                   int32_t *out_off, int32_t *out_cnt, int32_t *out_ids, int64_t cap);
+                   int32_t *out_off, int32_t *out_cnt, int32_t *out_ids, int64_t cap,
+                   int32_t kpbwt, int32_t *out_states, int32_t *out_K);
 
 /* One complete GLIMPSE2_phase Gibbs run on diploid targets
  * (phase/src/caller/caller_algorithm.cpp:53-243): INIT + burnin + main with
