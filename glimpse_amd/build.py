@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = [
-    "gh_capi.hip", "gh_panel.hip", "gh_condset.hip", "gh_impute.hip", "gh_phase.hip", "gh_pbwt.hip",
+    "gh_capi.hip", "gh_panel.hip", "gh_condset.hip", "gh_impute.hip", "gh_phase.hip", "gh_pbwt.hip", "gh_gibbs.hip",
 ]
 
 HIPCC_FLAGS = [

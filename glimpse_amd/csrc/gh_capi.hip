@@ -13,6 +13,9 @@
 namespace gh {
 void panel_create(const gh_panel_desc *d, gh_panel **out);
 void panel_destroy(gh_panel *p);
+void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
+               const int32_t *kpt, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
+               int32_t *nstates_h, gh_run_stats *stats);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -228,7 +231,7 @@ int gh_rephase_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int
             std::memset(&ph[i], 0, sizeof(PhItem));
             ph[i].cond = i;
             ph[i].tgt = tgt[i];
-            ph[i].hap_off = (int64_t)i * L;
+            ph[i].h0_off = ph[i].h1_off = ph[i].flat_off = (int64_t)i * L;
             ph[i].locus_off = (int64_t)i * L;
             maxK = std::max(maxK, B.h[i].K);
         }
@@ -359,10 +362,10 @@ int gh_pbwt_select(gh_panel *P, int32_t T, const uint8_t *hap_alleles, const uin
     });
 }
 
-int gh_phase_run(gh_panel *, int32_t, const uint8_t *, const uint8_t *, const gh_params *, const int32_t *, float *,
-                 float *, uint8_t *, uint8_t *, uint8_t *, int32_t *, gh_run_stats *) {
-    g_err = "gh_phase_run: not built in this library version";
-    return GH_ERR_ARG;
+int gh_phase_run(gh_panel *P, int32_t T, const uint8_t *gl, const uint8_t *flat, const gh_params *prm,
+                 const int32_t *kpbwt_per_target, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1,
+                 int32_t *nstates, gh_run_stats *stats) {
+    return guard([&] { phase_run(P, T, gl, flat, prm, kpbwt_per_target, gp0, gp1, hds, h0, h1, nstates, stats); });
 }
 
 }  // extern "C"

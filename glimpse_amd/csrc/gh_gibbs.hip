@@ -1,0 +1,520 @@
+// gh_gibbs.hip -- the device-resident Gibbs schedule of GLIMPSE2_phase
+// (caller::phase_loop / phase_iteration / phase_individual,
+//  phase/src/caller/caller_algorithm.cpp:53-243).
+//
+// Every target is processed in the same launches; per iteration:
+//   selection   : INIT -> rare-allele seeded random states (host, once);
+//                 BURN/MAIN -> sparse-PBWT walk + compaction (gh_pbwt.hip)
+//   condset     : conditioning sets (gh_condset.hip)
+//   H0          : haploid likelihoods -> forward-backward -> monomorphic closed form -> draw
+//   H1          : same, conditioned on the new H0
+//   phasing     : loci/segments prepass -> phasing HMM with segment draws
+//   MAIN only   : GP accumulation (dense form of genotype::storeGenotypePosteriorsAndHaplotypes)
+// then normalisation and best-guess genotypes (genotype::sortAndNormAndInferGenotype).
+//
+// Haplotypes live as bytes in H[2T][L] (row 2t = H0 of target t, 2t+1 = H1).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <set>
+
+#include "gh_device.hpp"
+#include "gh_internal.hpp"
+#include "gh_pbwt.hpp"
+#include "gh_phase.hpp"
+
+namespace gh {
+
+__constant__ float c_unphred[256];
+
+// common/src/utils/otools.h:97-98: 10^(-i/10) printed with 7 significant digits.
+static void unphred_table(float *t) {
+    char buf[64];
+    for (int i = 0; i < 256; i++) {
+        std::snprintf(buf, sizeof(buf), "%.6e", std::pow(10.0, -i / 10.0));
+        t[i] = std::strtof(buf, nullptr);
+    }
+}
+
+// genotype::initHaplotypeLikelihoods / makeHaplotypeLikelihoods (genotype.cpp:56-116)
+__global__ void hl_kernel(const uint8_t *__restrict__ gl, const uint8_t *__restrict__ flat,
+                          const uint8_t *__restrict__ H, int L, int T, int mode, int which, float min_gl,
+                          float *__restrict__ HL) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (l >= L) return;
+    const size_t o = (size_t)t * L + l;
+    float h0, h1;
+    if (flat[o]) {
+        h0 = 0.5f; h1 = 0.5f;
+    } else {
+        const uint8_t *g = gl + o * 3;
+        float t0 = c_unphred[g[0]], t1 = c_unphred[g[1]], t2 = c_unphred[g[2]];
+        const float s = t0 + t1 + t2;
+        t0 /= s; t1 /= s; t2 /= s;
+        if (mode == 0) {
+            h0 = (float)((double)t0 + 0.5 * (double)t1);
+            h1 = (float)(0.5 * (double)t1 + (double)t2);
+        } else {
+            // condition on the other haplotype of the target
+            const int c = H[(size_t)(2 * t + (which ^ 1)) * L + l];
+            const float a = c ? t1 : t0, b = c ? t2 : t1;
+            h0 = a / (a + b);
+            h1 = b / (a + b);
+        }
+    }
+    if (h0 < min_gl) { h0 = min_gl; h1 = 1.0f - min_gl; }
+    if (h1 < min_gl) { h0 = 1.0f - min_gl; h1 = min_gl; }
+    HL[2 * o] = h0;
+    HL[2 * o + 1] = h1;
+}
+
+// genotype::sampleHaplotypeH0/H1 (genotype.cpp:118-125) with counter draws
+__global__ void sample_kernel(const float *__restrict__ HP, uint8_t *__restrict__ H, int L, int which, uint32_t seed,
+                              uint32_t it) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (l >= L) return;
+    const float u = u01(philox_draw(seed, it, which ? P_H1 : P_H0, (uint32_t)t, (uint32_t)l));
+    H[(size_t)(2 * t + which) * L + l] = u > HP[((size_t)t * L + l) * 2];
+}
+
+// genotype::storeGenotypePosteriorsAndHaplotypes, diploid (genotype.cpp:168-209),
+// in dense form: a site not yet stored whose P(RR) >= 0.99999 contributes the
+// implicit (1, 0); from its first storage on, the actual normalised GP.
+__global__ void gp_kernel(const float *__restrict__ HP0, const float *__restrict__ HP1, int L,
+                          float *__restrict__ gp0, float *__restrict__ gp1, uint8_t *__restrict__ stored,
+                          uint8_t *__restrict__ hds) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (l >= L) return;
+    const size_t o = (size_t)t * L + l;
+    const float a0 = HP0[2 * o], a1 = HP0[2 * o + 1], b0 = HP1[2 * o], b1 = HP1[2 * o + 1];
+    const float p0 = fminf(fmaxf(a0 * b0, 0.0f), 1.0f);
+    const float p1 = fminf(fmaxf(a0 * b1 + a1 * b0, 0.0f), 1.0f);
+    const float p2 = fminf(fmaxf(a1 * b1, 0.0f), 1.0f);
+    const float n0 = p0 / (p0 + p1 + p2), n1 = p1 / (p0 + p1 + p2);
+    if (!stored[o] && !(n0 < 0.99999f)) {
+        gp0[o] += 1.0f;
+        return;
+    }
+    stored[o] = 1;
+    gp0[o] += n0;
+    gp1[o] += n1;
+    hds[o] = a1 < b1;
+}
+
+// genotype::sortAndNormAndInferGenotype + inferred_genotype::infer (genotype.h:30-36, genotype.cpp:211-244)
+__global__ void finalize_kernel(float *__restrict__ gp0, float *__restrict__ gp1, uint8_t *__restrict__ stored,
+                                uint8_t *__restrict__ hds, uint8_t *__restrict__ h0, uint8_t *__restrict__ h1,
+                                int L, int cnt) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (l >= L) return;
+    const size_t o = (size_t)t * L + l;
+    if (!stored[o] || cnt == 0) {
+        gp0[o] = 1.0f; gp1[o] = 0.0f; hds[o] = 0; h0[o] = 0; h1[o] = 0;
+        return;
+    }
+    const float a = gp0[o] / cnt, b = gp1[o] / cnt;
+    gp0[o] = a; gp1[o] = b;
+    const float c = 1.0f - b - a;
+    int g = 0;
+    if (a > b && a > c) g = 0;
+    else if (b > a && b > c) g = 1;
+    else if (c > a && c > b) g = 2;
+    h0[o] = g == 2;
+    h1[o] = g >= 1;
+}
+
+__global__ void set_k_kernel(CondItem *__restrict__ items, const int32_t *__restrict__ Kdev,
+                             const int32_t *__restrict__ targets, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = targets[i];
+    const int K = Kdev[t];
+    items[t].K = K;
+    items[t].W = (K + 63) / 64;
+}
+
+/* ------------------------------------------------------------------ host: INIT selection */
+static inline float init_u(uint32_t seed, uint32_t tgt, uint32_t ctr) {
+    return u01(philox_draw(seed, 0u, P_INIT_SEL, tgt, ctr));
+}
+// Selection sampling (Knuth, TAOCP vol. 2, Algorithm S) in place of std::sample.
+static void select_sample(const int32_t *pop, int n, int k, std::vector<int32_t> &out, uint32_t seed, uint32_t tgt,
+                          uint32_t &ctr) {
+    int need = std::min(k, n);
+    for (int i = 0; i < n && need > 0; i++) {
+        const float u = init_u(seed, tgt, ctr++);
+        if (u * (float)(n - i) < (float)need) { out.push_back(pop[i]); need--; }
+    }
+}
+
+// haplotype_set::initRareTar + performSelection_RARE_INIT_GL (haplotype_set.cpp:108-169, 775-833)
+static void init_states(gh_panel *P, int t, const uint8_t *gl, const uint8_t *flat, int kinit, uint32_t seed,
+                        std::vector<int32_t> &states, const float *unph) {
+    const int L = P->L, N = P->N;
+    std::vector<int> sind;
+    for (int v = 0; v < L; v++) {
+        if (flat[v]) continue;
+        const uint8_t f = P->flags[v];
+        const int mg = 2 * ((f & SF_MAJOR) ? 1 : 0);
+        if (!(f & SF_COMMON) && !(f & SF_LQ)) {
+            const uint8_t *g = gl + 3 * v;
+            if (g[1] <= g[mg] || g[2 - mg] <= g[mg]) {
+                float x[3] = {unph[g[0]], unph[g[1]], unph[g[2]]};
+                float s = x[0] + x[1] + x[2];
+                x[0] /= s; x[1] /= s; x[2] /= s;
+                const float af = P->calt[v] * 1.0 / N;
+                x[0] *= (1.0f - af) * (1.0f - af);
+                x[1] *= 2.0f * af * (1.0f - af);
+                x[2] *= (af * af);
+                s = x[0] + x[1] + x[2];
+                x[0] /= s; x[1] /= s; x[2] /= s;
+                if (x[1] + x[2 - mg] > x[mg]) sind.push_back(v);
+            }
+        }
+    }
+    const int k_init = kinit, k_init8 = (int)std::floor(k_init * 0.8f);
+    std::set<int32_t> S;
+    std::vector<int32_t> sampled;
+    uint32_t ctr = 0;
+    const int32_t *rp = P->rare_ptr_h.data(), *ri = P->rare_idx_h.data();
+    if (!sind.empty()) {
+        if ((int)sind.size() > k_init8) {
+            std::vector<std::pair<int, int>> mac;
+            for (int v : sind) mac.push_back({(int)std::min(P->cref[v], P->calt[v]), v});
+            std::sort(mac.begin(), mac.end());
+            std::vector<int32_t> io;
+            for (size_t r0 = 0; r0 < sind.size() && (int)S.size() < k_init8; r0++) {
+                const int v = mac[r0].second;
+                const int nc = rp[v + 1] - rp[v];
+                io.resize(nc);
+                std::iota(io.begin(), io.end(), 0);
+                sampled.clear();
+                select_sample(io.data(), nc, 5, sampled, seed, (uint32_t)t, ctr);
+                for (size_t i = 0; i < sampled.size() && (int)S.size() < k_init8; i++) S.insert(ri[rp[v] + sampled[i]]);
+            }
+        } else {
+            const int size_max = std::max((int)std::floor(k_init8 * 1.0 / sind.size()), 1);
+            for (int v : sind) {
+                sampled.clear();
+                select_sample(ri + rp[v], rp[v + 1] - rp[v], size_max, sampled, seed, (uint32_t)t, ctr);
+                for (size_t i = 0; i < sampled.size() && (int)S.size() < k_init8; i++) S.insert(sampled[i]);
+            }
+        }
+    }
+    if ((int)S.size() < k_init) {
+        std::vector<int32_t> io(N);
+        std::iota(io.begin(), io.end(), 0);
+        sampled.clear();
+        select_sample(io.data(), N, k_init, sampled, seed, (uint32_t)t, ctr);
+        for (size_t i = 0; i < sampled.size() && (int)S.size() < k_init; i++) S.insert(sampled[i]);
+    }
+    states.assign(S.begin(), S.end());
+}
+
+/* ------------------------------------------------------------------ driver */
+struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    void make() { GH_HIP(hipEventCreate(&a)); GH_HIP(hipEventCreate(&b)); }
+    ~EvPair() { if (a) (void)hipEventDestroy(a); if (b) (void)hipEventDestroy(b); }
+    float ms() { float x = 0.0f; (void)hipEventElapsedTime(&x, a, b); return x; }
+};
+
+void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
+               const int32_t *kpt, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
+               int32_t *nstates_h, gh_run_stats *stats) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    GH_CHECK(P && T > 0 && gl_h && flat_h && prm, GH_ERR_ARG, "gh_phase_run: bad arguments");
+    GH_CHECK(prm->burnin >= 0 && prm->main_iter >= 0 && prm->main_iter <= 15, GH_ERR_ARG,
+             "gh_phase_run: --main must be in 0..15 (caller_parameters.cpp:139-140)");
+    GH_CHECK(prm->pbwt_depth > 0 && prm->pbwt_depth <= 64, GH_ERR_ARG, "gh_phase_run: pbwt depth must be in 1..64");
+    hipStream_t s = P->stream;
+    const int L = P->L, N = P->N;
+    const uint32_t seed = prm->seed;
+    const int kinit = std::min(prm->kinit, N);
+    const float err_imp = std::min(std::max(prm->err_imp, 1e-12f), 1e-3f);
+    const float ed_i = err_imp, ee_i = (float)(1.0 - (double)err_imp);
+    const int n_iter = 1 + prm->burnin + prm->main_iter;
+    float unph[256];
+    unphred_table(unph);
+    GH_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_unphred), unph, sizeof(unph), 0, hipMemcpyHostToDevice, s));
+
+    // per-target K and PBWT classes (targets sharing an effective modulo share stored draws)
+    std::vector<int> kp(T);
+    for (int t = 0; t < T; t++) kp[t] = std::min(kpt ? kpt[t] : prm->kpbwt, N);
+    std::map<float, int> cls_of_mod;
+    std::vector<std::vector<int>> cls_grp;
+    std::vector<float> cls_mod;
+    std::vector<int> hap_cls(2 * T, 0), pbwt_targets, full_targets;
+    for (int t = 0; t < T; t++) {
+        GH_CHECK(kp[t] > 0, GH_ERR_STATE, "target " + std::to_string(t) + " has Kpbwt = 0 and no state list");
+        if (kp[t] >= N) { full_targets.push_back(t); continue; }
+        GH_CHECK(P->pbwt, GH_ERR_ARG, "gh_phase_run: Kpbwt < N needs a panel created with its sparse-PBWT stream");
+        std::vector<int> grp;
+        const float em = pbwt_groups(P, prm->pbwt_depth, prm->pbwt_modulo_cm, kp[t], grp);
+        auto it = cls_of_mod.find(em);
+        int c;
+        if (it == cls_of_mod.end()) {
+            c = (int)cls_mod.size();
+            cls_of_mod[em] = c;
+            cls_mod.push_back(em);
+            cls_grp.push_back(grp);
+        } else c = it->second;
+        hap_cls[2 * t] = hap_cls[2 * t + 1] = c;
+        pbwt_targets.push_back(t);
+    }
+    int kmax_t = 0;
+    std::vector<int> kmax(T);
+    for (int t = 0; t < T; t++) {
+        kmax[t] = std::max(kinit > 0 ? kinit : 0, kp[t]);
+        kmax_t = std::max(kmax_t, kmax[t]);
+    }
+    GH_CHECK(kmax_t <= 2048, GH_ERR_ARG, "gh_phase_run: more than 2048 conditioning states is not supported");
+
+    // ---------------- workspace
+    DBuf d_gl, d_flat, d_H, d_HL, d_HP0, d_HP1, d_states, d_K, d_items, d_fb, d_poly, d_pos, d_t, d_mark, d_hvar,
+        d_alpha, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_loci, d_seg, d_segsum, d_imp, d_impsum, d_of1,
+        d_dip, d_status, d_pst, d_pstart, d_pcls, d_pmod, d_ev, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
+    dalloc_copy(d_gl, gl_h, (size_t)T * L * 3, s);
+    dalloc_copy(d_flat, flat_h, (size_t)T * L, s);
+    uint8_t *H = (uint8_t *)d_H.reserve((size_t)2 * T * L);
+    GH_HIP(hipMemsetAsync(H, 0, (size_t)2 * T * L, s));
+    float *HL = (float *)d_HL.reserve(sizeof(float) * (size_t)T * 2 * L);
+    float *HP0 = (float *)d_HP0.reserve(sizeof(float) * (size_t)T * 2 * L);
+    float *HP1 = (float *)d_HP1.reserve(sizeof(float) * (size_t)T * 2 * L);
+    const int kstride = kmax_t;
+    int32_t *states = (int32_t *)d_states.reserve(sizeof(int32_t) * (size_t)T * kstride);
+    int32_t *Kdev = (int32_t *)d_K.reserve(sizeof(int32_t) * T);
+    std::vector<CondItem> items(T);
+    std::vector<FbItem> fb(T);
+    int64_t words = 0, aoff = 0;
+    int maxW = 0;
+    for (int t = 0; t < T; t++) {
+        const int Wt = (kmax[t] + 63) / 64;
+        maxW = std::max(maxW, Wt);
+        items[t] = CondItem{kmax[t], Wt, 0, 0, (int64_t)t * kstride, (int64_t)t * L, words};
+        words += (int64_t)L * Wt;
+        fb[t] = FbItem{0, 0, 0, t, aoff, (int64_t)t * 2 * L, (int64_t)t * L};
+        aoff += (int64_t)L * 64 * Wt;
+    }
+    CondItem *items_d = dalloc_copy(d_items, items.data(), T, s);
+    FbItem *fb_d = dalloc_copy(d_fb, fb.data(), T, s);
+    int32_t *poly = (int32_t *)d_poly.reserve(sizeof(int32_t) * (size_t)T * L);
+    int32_t *pos = (int32_t *)d_pos.reserve(sizeof(int32_t) * (size_t)T * L);
+    float *tr = (float *)d_t.reserve(sizeof(float) * (size_t)T * L);
+    uint8_t *mark = (uint8_t *)d_mark.reserve((size_t)T * L);
+    uint64_t *hvar = (uint64_t *)d_hvar.reserve(sizeof(uint64_t) * (size_t)words);
+    float *alpha = (float *)d_alpha.reserve(sizeof(float) * (size_t)aoff);
+    float *gp0 = (float *)d_gp0.reserve(sizeof(float) * (size_t)T * L);
+    float *gp1 = (float *)d_gp1.reserve(sizeof(float) * (size_t)T * L);
+    uint8_t *sto = (uint8_t *)d_stored.reserve((size_t)T * L);
+    uint8_t *hds = (uint8_t *)d_hds.reserve((size_t)T * L);
+    uint8_t *h0o = (uint8_t *)d_h0.reserve((size_t)T * L);
+    uint8_t *h1o = (uint8_t *)d_h1.reserve((size_t)T * L);
+    GH_HIP(hipMemsetAsync(gp0, 0, sizeof(float) * (size_t)T * L, s));
+    GH_HIP(hipMemsetAsync(gp1, 0, sizeof(float) * (size_t)T * L, s));
+    GH_HIP(hipMemsetAsync(sto, 0, (size_t)T * L, s));
+    GH_HIP(hipMemsetAsync(hds, 0, (size_t)T * L, s));
+    int32_t *status = (int32_t *)d_status.reserve(sizeof(int32_t) * 2);
+    GH_HIP(hipMemsetAsync(status, 0, sizeof(int32_t) * 2, s));
+    Locus *loci = (Locus *)d_loci.reserve(sizeof(Locus) * (size_t)T * L);
+    std::vector<PhItem> ph(T);
+    for (int t = 0; t < T; t++) {
+        std::memset(&ph[t], 0, sizeof(PhItem));
+        ph[t].cond = t;
+        ph[t].tgt = t;
+        ph[t].h0_off = (int64_t)(2 * t) * L;
+        ph[t].h1_off = (int64_t)(2 * t + 1) * L;
+        ph[t].flat_off = (int64_t)t * L;
+        ph[t].locus_off = (int64_t)t * L;
+    }
+    // PBWT launch constants
+    PbwtRun R;
+    std::memset(&R, 0, sizeof(R));
+    const int npt = (int)pbwt_targets.size();
+    std::vector<int32_t> hrow, srow;
+    if (npt > 0) {
+        int ngrp = 0;
+        for (auto &g : cls_grp) ngrp = std::max(ngrp, (int)g.size());
+        float minmod = *std::min_element(cls_mod.begin(), cls_mod.end());
+        R.n_haps = 2 * npt; R.n_samples = npt; R.depth = prm->pbwt_depth;
+        R.cap = pbwt_event_cap(P, minmod, ngrp);
+        for (int t : pbwt_targets) { hrow.push_back(2 * t); hrow.push_back(2 * t + 1); srow.push_back(t); }
+        std::vector<int32_t> pc(2 * npt), kpv(npt), sh(npt);
+        for (int i = 0; i < npt; i++) {
+            pc[2 * i] = pc[2 * i + 1] = hap_cls[2 * pbwt_targets[i]];
+            kpv[i] = kp[pbwt_targets[i]];
+            sh[i] = 2 * i;
+        }
+        R.hap_class = dalloc_copy(d_pcls, pc.data(), pc.size(), s);
+        R.modulo = dalloc_copy(d_pmod, cls_mod.data(), cls_mod.size(), s);
+        R.kpbwt = dalloc_copy(d_pkp, kpv.data(), kpv.size(), s);
+        R.sample_hap = dalloc_copy(d_psh, sh.data(), sh.size(), s);
+        R.hap_row = dalloc_copy(d_hrow, hrow.data(), hrow.size(), s);
+        R.sample_row = dalloc_copy(d_prow, srow.data(), srow.size(), s);
+        R.haps = H;
+        R.ev = (int32_t *)d_ev.reserve(sizeof(int32_t) * (size_t)2 * npt * R.cap * (2 + 2 * R.depth));
+        R.ev_cnt = (int32_t *)d_evcnt.reserve(sizeof(int32_t) * 2 * npt);
+        R.status = status + 1;
+        R.out_states = states;
+        R.out_stride = kstride;
+        R.out_K = Kdev;
+    }
+    int32_t *ptg = dalloc_copy(d_ptg, pbwt_targets.data(), pbwt_targets.size(), s);
+    std::vector<uint8_t> stored_h(std::max<size_t>(cls_mod.size(), 1) * std::max(P->Lhq, 1));
+    std::vector<int32_t> start_h(std::max(2 * npt, 1));
+    uint8_t *pst = (uint8_t *)d_pst.reserve(stored_h.size());
+    int32_t *pstart = (int32_t *)d_pstart.reserve(sizeof(int32_t) * start_h.size());
+    R.stored = pst;
+    R.start = pstart;
+
+    gh_run_stats st;
+    std::memset(&st, 0, sizeof(st));
+    EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1;
+    e_sel.make(); e_cond.make(); e_fb.make(); e_ph.make(); e_k0.make(); e_k1.make();
+    const dim3 gsite((L + 255) / 256, T), bsite(256);
+    int main_done = 0;
+    std::vector<int32_t> st_host;
+
+    for (int it = 0; it < n_iter; it++) {
+        const int stage = it == 0 ? 0 : (it <= prm->burnin ? 1 : 2);
+        GH_HIP(hipEventRecord(e_sel.a, s));
+        if (stage == 0 || it == 1) {
+            // host-provided state sets: INIT selection, and full-panel targets (fixed iota)
+            for (int t = 0; t < T; t++) {
+                std::vector<int32_t> S;
+                if (stage == 0 && kinit > 0) init_states(P, t, gl_h + (size_t)t * L * 3, flat_h + (size_t)t * L, kinit,
+                                                         seed, S, unph);
+                else if (kp[t] >= N) { S.resize(N); std::iota(S.begin(), S.end(), 0); }
+                else if (stage == 0) throw Error(GH_ERR_STATE, "target " + std::to_string(t) + ": Kinit = 0 leaves no INIT states");
+                else continue;
+                GH_CHECK(!S.empty(), GH_ERR_STATE, "target " + std::to_string(t) + ": zero conditioning states");
+                GH_HIP(hipMemcpyAsync(states + (size_t)t * kstride, S.data(), sizeof(int32_t) * S.size(),
+                                      hipMemcpyHostToDevice, s));
+                items[t].K = (int)S.size();
+                items[t].W = ((int)S.size() + 63) / 64;
+                GH_HIP(hipMemcpyAsync(&items_d[t], &items[t], sizeof(CondItem), hipMemcpyHostToDevice, s));
+                // the item array is read back after each iteration; keep this copy alive until then
+                GH_HIP(hipStreamSynchronize(s));
+            }
+        }
+        if (stage > 0 && npt > 0) {
+            // haplotype_set.cpp:335-348: one stored site per group and a random start per haplotype
+            std::fill(stored_h.begin(), stored_h.end(), 0);
+            for (size_t c = 0; c < cls_mod.size(); c++) {
+                int lo = 0;
+                for (size_t gi = 0; gi < cls_grp[c].size(); gi++) {
+                    const int hi = cls_grp[c][gi] - 1;
+                    const uint32_t x = philox_draw(seed, (uint32_t)it, P_PBWT_STORED, (uint32_t)gi, (uint32_t)c);
+                    stored_h[c * P->Lhq + urange(x, (uint32_t)lo, (uint32_t)hi)] = 1;
+                    lo = cls_grp[c][gi];
+                }
+            }
+            for (int i = 0; i < npt; i++)
+                for (int j = 0; j < 2; j++) {
+                    const uint32_t x = philox_draw(seed, (uint32_t)it, P_PBWT_START, (uint32_t)(2 * pbwt_targets[i] + j), 0u);
+                    start_h[2 * i + j] = (int32_t)urange(x, 0u, (uint32_t)(N - 1));
+                }
+            GH_HIP(hipMemcpyAsync(pst, stored_h.data(), stored_h.size(), hipMemcpyHostToDevice, s));
+            GH_HIP(hipMemcpyAsync(pstart, start_h.data(), sizeof(int32_t) * 2 * npt, hipMemcpyHostToDevice, s));
+            pbwt_select_launch(P, R, s);
+            hipLaunchKernelGGL(set_k_kernel, dim3((npt + 127) / 128), dim3(128), 0, s, items_d, Kdev, ptg, npt);
+            GH_HIP(hipGetLastError());
+            st.pbwt_hap_sites += (double)2 * npt * L;
+        }
+        GH_HIP(hipEventRecord(e_sel.b, s));
+        GH_HIP(hipEventRecord(e_cond.a, s));
+        condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, maxW, s);
+        GH_HIP(hipEventRecord(e_cond.b, s));
+        GH_HIP(hipEventRecord(e_fb.a, s));
+        // H0 (caller_algorithm.cpp:58-65)
+        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, d_gl.as<uint8_t>(), d_flat.as<uint8_t>(), H, L, T,
+                           stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
+        GH_HIP(hipEventRecord(e_k0.a, s));
+        fb_launch(P, T, fb_d, items_d, maxW, poly, tr, hvar, HL, d_flat.as<uint8_t>(), alpha, HP0, ee_i, ed_i, s);
+        GH_HIP(hipEventRecord(e_k0.b, s));
+        fb_mono_launch(P, T, fb_d, items_d, pos, HL, d_flat.as<uint8_t>(), HP0, ee_i, ed_i, s);
+        hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
+        // H1
+        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, d_gl.as<uint8_t>(), d_flat.as<uint8_t>(), H, L, T, 1, 1,
+                           prm->min_gl, HL);
+        GH_HIP(hipEventRecord(e_k1.a, s));
+        fb_launch(P, T, fb_d, items_d, maxW, poly, tr, hvar, HL, d_flat.as<uint8_t>(), alpha, HP1, ee_i, ed_i, s);
+        GH_HIP(hipEventRecord(e_k1.b, s));
+        fb_mono_launch(P, T, fb_d, items_d, pos, HL, d_flat.as<uint8_t>(), HP1, ee_i, ed_i, s);
+        hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
+        GH_HIP(hipGetLastError());
+        GH_HIP(hipEventRecord(e_fb.b, s));
+        // phasing (phasing_hmm::rephaseHaplotypes)
+        GH_HIP(hipEventRecord(e_ph.a, s));
+        PhItem *ph_d = dalloc_copy(d_ph, ph.data(), T, s);
+        phase_prep_launch(T, ph_d, items_d, poly, H, H, d_flat.as<uint8_t>(), P->d_flags.as<uint8_t>(), loci, s);
+        GH_HIP(hipMemcpyAsync(ph.data(), ph_d, sizeof(PhItem) * T, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+        int maxK = 1;
+        for (int t = 0; t < T; t++) maxK = std::max(maxK, items[t].K);
+        const int64_t KH = (int64_t)phase_kpt_states(maxK) * 8;
+        int64_t so = 0, ss = 0, io = 0, is = 0, dd = 0;
+        for (int t = 0; t < T; t++) {
+            ph[t].seg_off = so; so += ph[t].n_segs * KH;
+            ph[t].segsum_off = ss; ss += ph[t].n_segs * 9;
+            ph[t].imp_off = io; io += ph[t].n_miss * KH;
+            ph[t].impsum_off = is; is += ph[t].n_miss * 8;
+            ph[t].dip_off = dd; dd += ph[t].n_segs;
+            st.fb_state_sites += 2.0 * items[t].K * items[t].Lp;
+            st.fb_bytes += 2.0 * ((double)items[t].Lp * (2.0 * 8 * items[t].W + 8.0 * items[t].K + 16) + 8.0 * L);
+            st.fb_poly_sites += 2.0 * items[t].Lp;
+            st.ph_state_sites += (double)items[t].K * ph[t].n_loci;
+            st.ph_segments += ph[t].n_segs + ph[t].n_miss;
+            if (nstates_h) nstates_h[t * n_iter + it] = items[t].K;
+        }
+        GH_HIP(hipMemcpyAsync(ph_d, ph.data(), sizeof(PhItem) * T, hipMemcpyHostToDevice, s));
+        PhaseBufs PB;
+        PB.seg = (float *)d_seg.reserve(sizeof(float) * std::max<int64_t>(so, 1));
+        PB.segsum = (float *)d_segsum.reserve(sizeof(float) * std::max<int64_t>(ss, 1));
+        PB.imp = (float *)d_imp.reserve(sizeof(float) * std::max<int64_t>(io, 1));
+        PB.impsum = (float *)d_impsum.reserve(sizeof(float) * std::max<int64_t>(is, 1));
+        PB.of1 = (float *)d_of1.reserve(sizeof(float) * std::max<int64_t>(is, 1));
+        PB.dip = (int32_t *)d_dip.reserve(sizeof(int32_t) * std::max<int64_t>(dd, 1));
+        phase_launch(P, T, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
+                     (uint32_t)it, status, s);
+        GH_HIP(hipEventRecord(e_ph.b, s));
+        if (stage == 2) {
+            hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, gp0, gp1, sto, hds);
+            GH_HIP(hipGetLastError());
+            main_done++;
+        }
+        int32_t sflags[2] = {0, 0};
+        GH_HIP(hipMemcpyAsync(sflags, status, sizeof(sflags), hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+        GH_CHECK(sflags[0] == 0, GH_ERR_NUMERIC, "phasing HMM underflow (TRANS_HAP / SAMPLE_DIP)");
+        GH_CHECK(sflags[1] == 0, GH_ERR_CAPACITY, "PBWT selection record capacity exceeded");
+        st.ms_select += e_sel.ms();
+        st.ms_condset += e_cond.ms();
+        st.ms_fb += e_fb.ms();
+        st.ms_phase += e_ph.ms();
+        st.fb_kernel_ms += e_k0.ms() + e_k1.ms();
+        st.fb_launches += 2;
+    }
+    hipLaunchKernelGGL(finalize_kernel, gsite, bsite, 0, s, gp0, gp1, sto, hds, h0o, h1o, L, main_done);
+    GH_HIP(hipGetLastError());
+    const size_t TL = (size_t)T * L;
+    if (gp0_h) GH_HIP(hipMemcpyAsync(gp0_h, gp0, sizeof(float) * TL, hipMemcpyDeviceToHost, s));
+    if (gp1_h) GH_HIP(hipMemcpyAsync(gp1_h, gp1, sizeof(float) * TL, hipMemcpyDeviceToHost, s));
+    if (hds_h) GH_HIP(hipMemcpyAsync(hds_h, hds, TL, hipMemcpyDeviceToHost, s));
+    if (h0_h) GH_HIP(hipMemcpyAsync(h0_h, h0o, TL, hipMemcpyDeviceToHost, s));
+    if (h1_h) GH_HIP(hipMemcpyAsync(h1_h, h1o, TL, hipMemcpyDeviceToHost, s));
+    GH_HIP(hipStreamSynchronize(s));
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    st.ms_other = st.ms_total - st.ms_select - st.ms_condset - st.ms_fb - st.ms_phase;
+    if (stats) *stats = st;
+}
+
+}  // namespace gh

@@ -74,6 +74,7 @@ struct gh_panel {
     std::vector<uint8_t> flags;       // SF_* bits
     std::vector<int32_t> common2tot;
     std::vector<uint32_t> cref, calt;
+    std::vector<int32_t> rare_ptr_h, rare_idx_h;   // carriers of the non-major allele per site
     int64_t device_bytes = 0;
     hipStream_t stream = nullptr;
 

@@ -88,6 +88,8 @@ void panel_create(const gh_panel_desc *d, gh_panel **out) {
             for (int l = 0; l < L; l++)
                 for (int j = d->rare_ptr[l]; j < d->rare_ptr[l + 1]; j++) shap_idx[fill[d->rare_idx[j]]++] = l;
         }
+        P->rare_ptr_h.assign(d->rare_ptr, d->rare_ptr + L + 1);
+        P->rare_idx_h.assign(d->rare_idx, d->rare_idx + nr);
         hipStream_t s = P->stream;
         dalloc_copy(P->d_cm, P->cm.data(), L, s);
         dalloc_copy(P->d_cmpos, P->cm_pos.data(), L, s);

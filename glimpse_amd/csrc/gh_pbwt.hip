@@ -301,7 +301,8 @@ struct WalkArgs {
     const int32_t *aptr, *aidx;
     const float *cm_pos;
     const int32_t *c2t;
-    const uint8_t *haps;   // [n_haps][L]
+    const uint8_t *haps;   // rows of L
+    const int32_t *hap_row;
     int L;
     int n_haps;
     const int32_t *start;  // [n_haps]
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(64) void pbwt_walk_kernel(WalkArgs w) {
     const int h = blockIdx.x * 64 + threadIdx.x;
     if (h >= w.n_haps) return;
     const int N = w.N, depth = w.depth, rec = 2 + 2 * depth;
-    const uint8_t *hap = w.haps + (size_t)h * w.L;
+    const uint8_t *hap = w.haps + (size_t)(w.hap_row ? w.hap_row[h] : h) * w.L;
     const uint8_t *stored = w.stored + (size_t)w.hap_class[h] * w.Lhq;
     const float modulo = w.modulo[w.hap_class[h]];
     int32_t *ev = w.ev + (size_t)h * w.cap * rec;
@@ -480,6 +481,7 @@ struct CompactArgs {
     int cap, depth, N;
     const int32_t *kpbwt;       // [n_samples]
     const int32_t *sample_hap;  // [n_samples]: first haplotype index (second is +1)
+    const int32_t *sample_row;  // [n_samples]: output row
     int32_t *out_states;        // [n_samples][out_stride]
     int out_stride;
     int32_t *out_K;             // [n_samples]
@@ -522,7 +524,8 @@ __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
     __syncthreads();
     // sorted emission: wave prefix over per-word popcounts
     int base = 0;
-    int32_t *out = c.out_states + (size_t)t * c.out_stride;
+    const int row = c.sample_row ? c.sample_row[t] : t;
+    int32_t *out = c.out_states + (size_t)row * c.out_stride;
     for (int w0 = 0; w0 < Wn; w0 += 64) {
         const int wi = w0 + lane;
         const uint64_t x = wi < Wn ? bm[wi] : 0ull;
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
         }
         base += __shfl(incl, 63);
     }
-    if (lane == 0) c.out_K[t] = base;
+    if (lane == 0) c.out_K[row] = base;
 }
 
 void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
@@ -554,7 +557,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     w.A = D->A.as<int32_t>(); w.N = P->N;
     w.aptr = D->aptr.as<int32_t>(); w.aidx = D->aidx.as<int32_t>();
     w.cm_pos = P->d_cmpos.as<float>(); w.c2t = P->d_c2t.as<int32_t>();
-    w.haps = R.haps; w.L = P->L; w.n_haps = R.n_haps; w.start = R.start;
+    w.haps = R.haps; w.hap_row = R.hap_row; w.L = P->L; w.n_haps = R.n_haps; w.start = R.start;
     w.stored = R.stored; w.hap_class = R.hap_class; w.modulo = R.modulo; w.Lhq = P->Lhq;
     w.depth = R.depth; w.ev = R.ev; w.cap = R.cap; w.ev_cnt = R.ev_cnt; w.status = R.status;
     hipLaunchKernelGGL(pbwt_walk_kernel, dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
@@ -562,7 +565,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     if (R.out_states) {
         CompactArgs c;
         c.ev = R.ev; c.ev_cnt = R.ev_cnt; c.cap = R.cap; c.depth = R.depth; c.N = P->N;
-        c.kpbwt = R.kpbwt; c.sample_hap = R.sample_hap; c.out_states = R.out_states; c.out_stride = R.out_stride;
+        c.kpbwt = R.kpbwt; c.sample_hap = R.sample_hap; c.sample_row = R.sample_row; c.out_states = R.out_states; c.out_stride = R.out_stride;
         c.out_K = R.out_K;
         const size_t shm = sizeof(uint64_t) * ((P->N + 63) / 64);
         hipLaunchKernelGGL(pbwt_compact_kernel, dim3(R.n_samples), dim3(64), shm, s, c);

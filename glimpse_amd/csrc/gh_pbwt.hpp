@@ -28,7 +28,8 @@ struct SiteOp {
 // Arguments of one selection launch.
 struct PbwtRun {
     int n_haps, n_samples, depth, cap;
-    const uint8_t *haps;        // [n_haps][L] current alleles
+    const uint8_t *haps;        // current alleles, one row of L per haplotype
+    const int32_t *hap_row;     // [n_haps] row of each walked haplotype in haps (NULL: identity)
     const int32_t *start;       // [n_haps]
     const uint8_t *stored;      // [n_classes][Lhq]
     const int32_t *hap_class;   // [n_haps]
@@ -38,7 +39,8 @@ struct PbwtRun {
     int32_t *status;
     // optional compaction
     const int32_t *kpbwt;       // [n_samples]
-    const int32_t *sample_hap;  // [n_samples]
+    const int32_t *sample_hap;  // [n_samples] first walked haplotype of the sample
+    const int32_t *sample_row;  // [n_samples] output row (NULL: identity)
     int32_t *out_states;        // [n_samples][out_stride]
     int out_stride;
     int32_t *out_K;             // [n_samples]

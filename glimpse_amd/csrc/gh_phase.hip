@@ -78,7 +78,7 @@ __global__ __launch_bounds__(PREP_NT) void phase_prep_kernel(PhItem *__restrict_
     const CondItem c = cond[it.cond];
     const int Lp = c.Lp, tid = threadIdx.x;
     const int32_t *poly = poly_all + c.site_off;
-    const uint8_t *H0 = H0_all + it.hap_off, *H1 = H1_all + it.hap_off, *flat = flat_all + it.hap_off;
+    const uint8_t *H0 = H0_all + it.h0_off, *H1 = H1_all + it.h1_off, *flat = flat_all + it.flat_off;
     Locus *loci = loci_all + it.locus_off;
     const int C = (Lp + PREP_NT - 1) / PREP_NT;
     const int q0 = min(Lp, tid * C), q1 = min(Lp, q0 + C);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
     float *imps = impsum_all + (size_t)it.impsum_off;
     float *of1 = of1_all + (size_t)it.impsum_off;
     int32_t *dip = dip_all + it.dip_off;
-    uint8_t *H0 = H0_all + it.hap_off, *H1 = H1_all + it.hap_off;
+    uint8_t *H0 = H0_all + it.h0_off, *H1 = H1_all + it.h1_off;
     const float Kf = (float)K;
     const float mism = ed / ee;
     const size_t KH = (size_t)PH_NT * KPT * 8;  // stored message stride (padded states)

@@ -23,7 +23,9 @@ struct PhItem {
     int32_t cond;        // CondItem index
     int32_t tgt;         // target index (keys the counter draws)
     int32_t n_loci, n_segs, n_miss, pad;
-    int64_t hap_off;     // into H0/H1/flat ([L] per target)
+    int64_t h0_off;      // row of H0 in the H0 array
+    int64_t h1_off;      // row of H1 in the H1 array
+    int64_t flat_off;    // row of the flat flags
     int64_t locus_off;   // into loci (Lp reserved)
     int64_t seg_off;     // floats: n_segs * KH
     int64_t segsum_off;  // floats: n_segs * 9

@@ -39,7 +39,8 @@ class Params(ctypes.Structure):
 class RunStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "fb_state_sites", "fb_poly_sites", "ph_state_sites", "ph_segments", "pbwt_hap_sites",
-        "ms_select", "ms_condset", "ms_fb", "ms_phase", "ms_other", "ms_total")]
+        "ms_select", "ms_condset", "ms_fb", "ms_phase", "ms_other", "ms_total",
+        "fb_kernel_ms", "fb_launches", "fb_bytes")]
 
 
 def lib():

@@ -79,6 +79,9 @@ typedef struct {
     double ph_segments;        /* sum of (segments + flat hets) */
     double pbwt_hap_sites;     /* target haplotypes x sites swept */
     double ms_select, ms_condset, ms_fb, ms_phase, ms_other, ms_total;
+    double fb_kernel_ms;       /* summed HIP-event duration of the forward-backward kernel launches */
+    double fb_launches;        /* number of those launches */
+    double fb_bytes;           /* their algorithmic HBM bytes: per haplotype Lp*(2*8*ceil(K/64) + 8*K + 16) + 8*L */
 } gh_run_stats;
 
 const char *gh_last_error(void);
