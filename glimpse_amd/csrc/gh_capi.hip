@@ -16,6 +16,12 @@ void panel_destroy(gh_panel *p);
 void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
                const int32_t *kpt, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
                int32_t *nstates_h, gh_run_stats *stats);
+void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
+                const int32_t *kpt, gh_run **out);
+void run_execute(gh_run *G);
+void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
+               int32_t *nstates_h, gh_run_stats *stats);
+void run_destroy(gh_run *G);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -366,6 +372,21 @@ int gh_phase_run(gh_panel *P, int32_t T, const uint8_t *gl, const uint8_t *flat,
                  const int32_t *kpbwt_per_target, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1,
                  int32_t *nstates, gh_run_stats *stats) {
     return guard([&] { phase_run(P, T, gl, flat, prm, kpbwt_per_target, gp0, gp1, hds, h0, h1, nstates, stats); });
+}
+
+int gh_run_create(gh_panel *P, int32_t T, const uint8_t *gl, const uint8_t *flat, const gh_params *prm,
+                  const int32_t *kpbwt_per_target, gh_run **out) {
+    return guard([&] { run_create(P, T, gl, flat, prm, kpbwt_per_target, out); });
+}
+int gh_run_execute(gh_run *r) {
+    return guard([&] { run_execute(r); });
+}
+int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1, int32_t *nstates,
+                 gh_run_stats *stats) {
+    return guard([&] { run_fetch(r, gp0, gp1, hds, h0, h1, nstates, stats); });
+}
+int gh_run_destroy(gh_run *r) {
+    return guard([&] { run_destroy(r); });
 }
 
 }  // extern "C"

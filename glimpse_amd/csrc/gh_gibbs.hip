@@ -226,203 +226,266 @@ struct EvPair {
     float ms() { float x = 0.0f; (void)hipEventElapsedTime(&x, a, b); return x; }
 };
 
-void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
-               const int32_t *kpt, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
-               int32_t *nstates_h, gh_run_stats *stats) {
-    const auto wall0 = std::chrono::steady_clock::now();
-    GH_CHECK(P && T > 0 && gl_h && flat_h && prm, GH_ERR_ARG, "gh_phase_run: bad arguments");
-    GH_CHECK(prm->burnin >= 0 && prm->main_iter >= 0 && prm->main_iter <= 15, GH_ERR_ARG,
-             "gh_phase_run: --main must be in 0..15 (caller_parameters.cpp:139-140)");
-    GH_CHECK(prm->pbwt_depth > 0 && prm->pbwt_depth <= 64, GH_ERR_ARG, "gh_phase_run: pbwt depth must be in 1..64");
-    hipStream_t s = P->stream;
-    const int L = P->L, N = P->N;
-    const uint32_t seed = prm->seed;
-    const int kinit = std::min(prm->kinit, N);
-    const float err_imp = std::min(std::max(prm->err_imp, 1e-12f), 1e-3f);
-    const float ed_i = err_imp, ee_i = (float)(1.0 - (double)err_imp);
-    const int n_iter = 1 + prm->burnin + prm->main_iter;
-    float unph[256];
-    unphred_table(unph);
-    GH_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_unphred), unph, sizeof(unph), 0, hipMemcpyHostToDevice, s));
+}  // namespace gh
 
-    // per-target K and PBWT classes (targets sharing an effective modulo share stored draws)
-    std::vector<int> kp(T);
-    for (int t = 0; t < T; t++) kp[t] = std::min(kpt ? kpt[t] : prm->kpbwt, N);
-    std::map<float, int> cls_of_mod;
+// One prepared Gibbs run: inputs resident on the device, workspace allocated.
+struct gh_run {
+    gh_panel *P = nullptr;
+    int T = 0, L = 0, N = 0, n_iter = 0, kinit = 0, kstride = 0, maxW = 0, npt = 0;
+    gh_params prm;
+    float ee_i = 0, ed_i = 0;
+    float unph[256];
+    std::vector<uint8_t> gl_h, flat_h;   // host copies for the INIT selection
+    std::vector<int> kp, kmax, hap_cls, pbwt_targets;
     std::vector<std::vector<int>> cls_grp;
     std::vector<float> cls_mod;
-    std::vector<int> hap_cls(2 * T, 0), pbwt_targets, full_targets;
-    for (int t = 0; t < T; t++) {
-        GH_CHECK(kp[t] > 0, GH_ERR_STATE, "target " + std::to_string(t) + " has Kpbwt = 0 and no state list");
-        if (kp[t] >= N) { full_targets.push_back(t); continue; }
-        GH_CHECK(P->pbwt, GH_ERR_ARG, "gh_phase_run: Kpbwt < N needs a panel created with its sparse-PBWT stream");
-        std::vector<int> grp;
-        const float em = pbwt_groups(P, prm->pbwt_depth, prm->pbwt_modulo_cm, kp[t], grp);
-        auto it = cls_of_mod.find(em);
-        int c;
-        if (it == cls_of_mod.end()) {
-            c = (int)cls_mod.size();
-            cls_of_mod[em] = c;
-            cls_mod.push_back(em);
-            cls_grp.push_back(grp);
-        } else c = it->second;
-        hap_cls[2 * t] = hap_cls[2 * t + 1] = c;
-        pbwt_targets.push_back(t);
-    }
-    int kmax_t = 0;
-    std::vector<int> kmax(T);
-    for (int t = 0; t < T; t++) {
-        kmax[t] = std::max(kinit > 0 ? kinit : 0, kp[t]);
-        kmax_t = std::max(kmax_t, kmax[t]);
-    }
-    GH_CHECK(kmax_t <= 2048, GH_ERR_ARG, "gh_phase_run: more than 2048 conditioning states is not supported");
+    std::vector<gh::CondItem> items;
+    std::vector<gh::FbItem> fb;
+    std::vector<gh::PhItem> ph;
+    std::vector<uint8_t> stored_h;
+    std::vector<int32_t> start_h, nstates;   // nstates [T][n_iter]
+    gh::DBuf d_gl, d_flat, d_H, d_HL, d_HP0, d_HP1, d_states, d_K, d_items, d_fb, d_poly, d_pos, d_t, d_mark,
+        d_hvar, d_alpha, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_loci, d_seg, d_segsum, d_imp, d_impsum,
+        d_of1, d_dip, d_status, d_pst, d_pstart, d_pcls, d_pmod, d_ev, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
+    gh::PbwtRun R;
+    gh_run_stats st;
+    bool done = false;
+};
 
-    // ---------------- workspace
-    DBuf d_gl, d_flat, d_H, d_HL, d_HP0, d_HP1, d_states, d_K, d_items, d_fb, d_poly, d_pos, d_t, d_mark, d_hvar,
-        d_alpha, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_loci, d_seg, d_segsum, d_imp, d_impsum, d_of1,
-        d_dip, d_status, d_pst, d_pstart, d_pcls, d_pmod, d_ev, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
-    dalloc_copy(d_gl, gl_h, (size_t)T * L * 3, s);
-    dalloc_copy(d_flat, flat_h, (size_t)T * L, s);
-    uint8_t *H = (uint8_t *)d_H.reserve((size_t)2 * T * L);
-    GH_HIP(hipMemsetAsync(H, 0, (size_t)2 * T * L, s));
-    float *HL = (float *)d_HL.reserve(sizeof(float) * (size_t)T * 2 * L);
-    float *HP0 = (float *)d_HP0.reserve(sizeof(float) * (size_t)T * 2 * L);
-    float *HP1 = (float *)d_HP1.reserve(sizeof(float) * (size_t)T * 2 * L);
-    const int kstride = kmax_t;
-    int32_t *states = (int32_t *)d_states.reserve(sizeof(int32_t) * (size_t)T * kstride);
-    int32_t *Kdev = (int32_t *)d_K.reserve(sizeof(int32_t) * T);
-    std::vector<CondItem> items(T);
-    std::vector<FbItem> fb(T);
-    int64_t words = 0, aoff = 0;
-    int maxW = 0;
-    for (int t = 0; t < T; t++) {
-        const int Wt = (kmax[t] + 63) / 64;
-        maxW = std::max(maxW, Wt);
-        items[t] = CondItem{kmax[t], Wt, 0, 0, (int64_t)t * kstride, (int64_t)t * L, words};
-        words += (int64_t)L * Wt;
-        fb[t] = FbItem{0, 0, 0, t, aoff, (int64_t)t * 2 * L, (int64_t)t * L};
-        aoff += (int64_t)L * 64 * Wt;
+namespace gh {
+
+void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
+                const int32_t *kpt, gh_run **out) {
+    GH_CHECK(P && T > 0 && gl_h && flat_h && prm && out, GH_ERR_ARG, "gh_run_create: bad arguments");
+    GH_CHECK(prm->burnin >= 0 && prm->main_iter >= 0 && prm->main_iter <= 15, GH_ERR_ARG,
+             "gh_run_create: --main must be in 0..15 (caller_parameters.cpp:139-140)");
+    GH_CHECK(prm->pbwt_depth > 0 && prm->pbwt_depth <= 64, GH_ERR_ARG, "gh_run_create: pbwt depth must be in 1..64");
+    auto *G = new gh_run();
+    try {
+        hipStream_t s = P->stream;
+        G->P = P; G->T = T; G->L = P->L; G->N = P->N; G->prm = *prm;
+        const int L = P->L, N = P->N;
+        G->kinit = std::min(prm->kinit, N);
+        const float err_imp = std::min(std::max(prm->err_imp, 1e-12f), 1e-3f);   // caller_initialise.cpp:152
+        G->ed_i = err_imp;
+        G->ee_i = (float)(1.0 - (double)err_imp);
+        G->n_iter = 1 + prm->burnin + prm->main_iter;
+        unphred_table(G->unph);
+        G->gl_h.assign(gl_h, gl_h + (size_t)T * L * 3);
+        G->flat_h.assign(flat_h, flat_h + (size_t)T * L);
+        // per-target K (caller_initialise.cpp:150) and PBWT classes by effective modulo
+        G->kp.resize(T);
+        for (int t = 0; t < T; t++) G->kp[t] = std::min(kpt ? kpt[t] : prm->kpbwt, N);
+        std::map<float, int> cls_of_mod;
+        G->hap_cls.assign(2 * T, 0);
+        for (int t = 0; t < T; t++) {
+            GH_CHECK(G->kp[t] > 0, GH_ERR_STATE, "target " + std::to_string(t) + " has Kpbwt = 0 and no state list");
+            GH_CHECK(G->kinit > 0 || G->kp[t] >= N, GH_ERR_STATE,
+                     "target " + std::to_string(t) + ": Kinit = 0 leaves no INIT states");
+            if (G->kp[t] >= N) continue;
+            GH_CHECK(P->pbwt, GH_ERR_ARG, "Kpbwt < N needs a panel created with its sparse-PBWT stream");
+            std::vector<int> grp;
+            const float em = pbwt_groups(P, prm->pbwt_depth, prm->pbwt_modulo_cm, G->kp[t], grp);
+            auto it = cls_of_mod.find(em);
+            int c;
+            if (it == cls_of_mod.end()) {
+                c = (int)G->cls_mod.size();
+                cls_of_mod[em] = c;
+                G->cls_mod.push_back(em);
+                G->cls_grp.push_back(grp);
+            } else c = it->second;
+            G->hap_cls[2 * t] = G->hap_cls[2 * t + 1] = c;
+            G->pbwt_targets.push_back(t);
+        }
+        int kmax_t = 0;
+        G->kmax.resize(T);
+        for (int t = 0; t < T; t++) {
+            G->kmax[t] = std::max(G->kinit, G->kp[t]);
+            kmax_t = std::max(kmax_t, G->kmax[t]);
+        }
+        GH_CHECK(kmax_t <= 2048, GH_ERR_ARG, "more than 2048 conditioning states is not supported");
+        G->kstride = kmax_t;
+        // ---------------- device workspace
+        dalloc_copy(G->d_gl, gl_h, (size_t)T * L * 3, s);
+        dalloc_copy(G->d_flat, flat_h, (size_t)T * L, s);
+        G->d_H.reserve((size_t)2 * T * L);
+        G->d_HL.reserve(sizeof(float) * (size_t)T * 2 * L);
+        G->d_HP0.reserve(sizeof(float) * (size_t)T * 2 * L);
+        G->d_HP1.reserve(sizeof(float) * (size_t)T * 2 * L);
+        G->d_states.reserve(sizeof(int32_t) * (size_t)T * G->kstride);
+        G->d_K.reserve(sizeof(int32_t) * T);
+        G->items.resize(T);
+        G->fb.resize(T);
+        int64_t words = 0, aoff = 0;
+        for (int t = 0; t < T; t++) {
+            const int Wt = (G->kmax[t] + 63) / 64;
+            G->maxW = std::max(G->maxW, Wt);
+            G->items[t] = CondItem{G->kmax[t], Wt, 0, 0, (int64_t)t * G->kstride, (int64_t)t * L, words};
+            words += (int64_t)L * Wt;
+            G->fb[t] = FbItem{0, 0, 0, t, aoff, (int64_t)t * 2 * L, (int64_t)t * L};
+            aoff += (int64_t)L * 64 * Wt;
+        }
+        dalloc_copy(G->d_items, G->items.data(), T, s);
+        dalloc_copy(G->d_fb, G->fb.data(), T, s);
+        G->d_poly.reserve(sizeof(int32_t) * (size_t)T * L);
+        G->d_pos.reserve(sizeof(int32_t) * (size_t)T * L);
+        G->d_t.reserve(sizeof(float) * (size_t)T * L);
+        G->d_mark.reserve((size_t)T * L);
+        G->d_hvar.reserve(sizeof(uint64_t) * (size_t)words);
+        G->d_alpha.reserve(sizeof(float) * (size_t)aoff);
+        G->d_gp0.reserve(sizeof(float) * (size_t)T * L);
+        G->d_gp1.reserve(sizeof(float) * (size_t)T * L);
+        G->d_stored.reserve((size_t)T * L);
+        G->d_hds.reserve((size_t)T * L);
+        G->d_h0.reserve((size_t)T * L);
+        G->d_h1.reserve((size_t)T * L);
+        G->d_status.reserve(sizeof(int32_t) * 2);
+        G->d_loci.reserve(sizeof(Locus) * (size_t)T * L);
+        G->ph.resize(T);
+        for (int t = 0; t < T; t++) {
+            std::memset(&G->ph[t], 0, sizeof(PhItem));
+            G->ph[t].cond = t;
+            G->ph[t].tgt = t;
+            G->ph[t].h0_off = (int64_t)(2 * t) * L;
+            G->ph[t].h1_off = (int64_t)(2 * t + 1) * L;
+            G->ph[t].flat_off = (int64_t)t * L;
+            G->ph[t].locus_off = (int64_t)t * L;
+        }
+        PbwtRun &R = G->R;
+        std::memset(&R, 0, sizeof(R));
+        G->npt = (int)G->pbwt_targets.size();
+        const int npt = G->npt;
+        if (npt > 0) {
+            int ngrp = 0;
+            for (auto &g : G->cls_grp) ngrp = std::max(ngrp, (int)g.size());
+            const float minmod = *std::min_element(G->cls_mod.begin(), G->cls_mod.end());
+            R.n_haps = 2 * npt; R.n_samples = npt; R.depth = prm->pbwt_depth;
+            R.cap = pbwt_event_cap(P, minmod, ngrp);
+            std::vector<int32_t> hrow, srow, pc(2 * npt), kpv(npt), sh(npt);
+            for (int i = 0; i < npt; i++) {
+                const int t = G->pbwt_targets[i];
+                hrow.push_back(2 * t); hrow.push_back(2 * t + 1); srow.push_back(t);
+                pc[2 * i] = pc[2 * i + 1] = G->hap_cls[2 * t];
+                kpv[i] = G->kp[t];
+                sh[i] = 2 * i;
+            }
+            R.hap_class = dalloc_copy(G->d_pcls, pc.data(), pc.size(), s);
+            R.modulo = dalloc_copy(G->d_pmod, G->cls_mod.data(), G->cls_mod.size(), s);
+            R.kpbwt = dalloc_copy(G->d_pkp, kpv.data(), kpv.size(), s);
+            R.sample_hap = dalloc_copy(G->d_psh, sh.data(), sh.size(), s);
+            R.hap_row = dalloc_copy(G->d_hrow, hrow.data(), hrow.size(), s);
+            R.sample_row = dalloc_copy(G->d_prow, srow.data(), srow.size(), s);
+            R.ev = (int32_t *)G->d_ev.reserve(sizeof(int32_t) * (size_t)2 * npt * R.cap * (2 + 2 * R.depth));
+            R.ev_cnt = (int32_t *)G->d_evcnt.reserve(sizeof(int32_t) * 2 * npt);
+        }
+        dalloc_copy(G->d_ptg, G->pbwt_targets.data(), G->pbwt_targets.size(), s);
+        G->stored_h.assign(std::max<size_t>(G->cls_mod.size(), 1) * std::max(P->Lhq, 1), 0);
+        G->start_h.assign(std::max(2 * npt, 1), 0);
+        G->d_pst.reserve(G->stored_h.size());
+        G->d_pstart.reserve(sizeof(int32_t) * G->start_h.size());
+        G->nstates.assign((size_t)T * G->n_iter, 0);
+        GH_HIP(hipStreamSynchronize(s));
+        *out = G;
+    } catch (...) {
+        delete G;
+        throw;
     }
-    CondItem *items_d = dalloc_copy(d_items, items.data(), T, s);
-    FbItem *fb_d = dalloc_copy(d_fb, fb.data(), T, s);
-    int32_t *poly = (int32_t *)d_poly.reserve(sizeof(int32_t) * (size_t)T * L);
-    int32_t *pos = (int32_t *)d_pos.reserve(sizeof(int32_t) * (size_t)T * L);
-    float *tr = (float *)d_t.reserve(sizeof(float) * (size_t)T * L);
-    uint8_t *mark = (uint8_t *)d_mark.reserve((size_t)T * L);
-    uint64_t *hvar = (uint64_t *)d_hvar.reserve(sizeof(uint64_t) * (size_t)words);
-    float *alpha = (float *)d_alpha.reserve(sizeof(float) * (size_t)aoff);
-    float *gp0 = (float *)d_gp0.reserve(sizeof(float) * (size_t)T * L);
-    float *gp1 = (float *)d_gp1.reserve(sizeof(float) * (size_t)T * L);
-    uint8_t *sto = (uint8_t *)d_stored.reserve((size_t)T * L);
-    uint8_t *hds = (uint8_t *)d_hds.reserve((size_t)T * L);
-    uint8_t *h0o = (uint8_t *)d_h0.reserve((size_t)T * L);
-    uint8_t *h1o = (uint8_t *)d_h1.reserve((size_t)T * L);
+}
+
+void run_execute(gh_run *G) {
+    GH_CHECK(G, GH_ERR_ARG, "gh_run_execute: null run");
+    const auto wall0 = std::chrono::steady_clock::now();
+    gh_panel *P = G->P;
+    hipStream_t s = P->stream;
+    const int T = G->T, L = G->L, N = G->N, n_iter = G->n_iter, npt = G->npt;
+    const gh_params *prm = &G->prm;
+    const uint32_t seed = prm->seed;
+    GH_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_unphred), G->unph, sizeof(G->unph), 0, hipMemcpyHostToDevice, s));
+    uint8_t *H = G->d_H.as<uint8_t>();
+    float *HL = G->d_HL.as<float>(), *HP0 = G->d_HP0.as<float>(), *HP1 = G->d_HP1.as<float>();
+    int32_t *states = G->d_states.as<int32_t>(), *Kdev = G->d_K.as<int32_t>();
+    CondItem *items_d = G->d_items.as<CondItem>();
+    FbItem *fb_d = G->d_fb.as<FbItem>();
+    int32_t *poly = G->d_poly.as<int32_t>(), *pos = G->d_pos.as<int32_t>();
+    float *tr = G->d_t.as<float>();
+    uint8_t *mark = G->d_mark.as<uint8_t>();
+    uint64_t *hvar = G->d_hvar.as<uint64_t>();
+    float *alpha = G->d_alpha.as<float>();
+    float *gp0 = G->d_gp0.as<float>(), *gp1 = G->d_gp1.as<float>();
+    uint8_t *sto = G->d_stored.as<uint8_t>(), *hds = G->d_hds.as<uint8_t>();
+    int32_t *status = G->d_status.as<int32_t>();
+    Locus *loci = G->d_loci.as<Locus>();
+    const uint8_t *gl_d = G->d_gl.as<uint8_t>(), *flat_d = G->d_flat.as<uint8_t>();
+    int32_t *ptg = G->d_ptg.as<int32_t>();
+    uint8_t *pst = G->d_pst.as<uint8_t>();
+    int32_t *pstart = G->d_pstart.as<int32_t>();
+    PbwtRun R = G->R;
+    R.haps = H;
+    R.status = status + 1;
+    R.out_states = states;
+    R.out_stride = G->kstride;
+    R.out_K = Kdev;
+    R.stored = pst;
+    R.start = pstart;
+    // fresh chain: haplotypes, accumulators, item sizes
+    GH_HIP(hipMemsetAsync(H, 0, (size_t)2 * T * L, s));
     GH_HIP(hipMemsetAsync(gp0, 0, sizeof(float) * (size_t)T * L, s));
     GH_HIP(hipMemsetAsync(gp1, 0, sizeof(float) * (size_t)T * L, s));
     GH_HIP(hipMemsetAsync(sto, 0, (size_t)T * L, s));
     GH_HIP(hipMemsetAsync(hds, 0, (size_t)T * L, s));
-    int32_t *status = (int32_t *)d_status.reserve(sizeof(int32_t) * 2);
     GH_HIP(hipMemsetAsync(status, 0, sizeof(int32_t) * 2, s));
-    Locus *loci = (Locus *)d_loci.reserve(sizeof(Locus) * (size_t)T * L);
-    std::vector<PhItem> ph(T);
-    for (int t = 0; t < T; t++) {
-        std::memset(&ph[t], 0, sizeof(PhItem));
-        ph[t].cond = t;
-        ph[t].tgt = t;
-        ph[t].h0_off = (int64_t)(2 * t) * L;
-        ph[t].h1_off = (int64_t)(2 * t + 1) * L;
-        ph[t].flat_off = (int64_t)t * L;
-        ph[t].locus_off = (int64_t)t * L;
-    }
-    // PBWT launch constants
-    PbwtRun R;
-    std::memset(&R, 0, sizeof(R));
-    const int npt = (int)pbwt_targets.size();
-    std::vector<int32_t> hrow, srow;
-    if (npt > 0) {
-        int ngrp = 0;
-        for (auto &g : cls_grp) ngrp = std::max(ngrp, (int)g.size());
-        float minmod = *std::min_element(cls_mod.begin(), cls_mod.end());
-        R.n_haps = 2 * npt; R.n_samples = npt; R.depth = prm->pbwt_depth;
-        R.cap = pbwt_event_cap(P, minmod, ngrp);
-        for (int t : pbwt_targets) { hrow.push_back(2 * t); hrow.push_back(2 * t + 1); srow.push_back(t); }
-        std::vector<int32_t> pc(2 * npt), kpv(npt), sh(npt);
-        for (int i = 0; i < npt; i++) {
-            pc[2 * i] = pc[2 * i + 1] = hap_cls[2 * pbwt_targets[i]];
-            kpv[i] = kp[pbwt_targets[i]];
-            sh[i] = 2 * i;
-        }
-        R.hap_class = dalloc_copy(d_pcls, pc.data(), pc.size(), s);
-        R.modulo = dalloc_copy(d_pmod, cls_mod.data(), cls_mod.size(), s);
-        R.kpbwt = dalloc_copy(d_pkp, kpv.data(), kpv.size(), s);
-        R.sample_hap = dalloc_copy(d_psh, sh.data(), sh.size(), s);
-        R.hap_row = dalloc_copy(d_hrow, hrow.data(), hrow.size(), s);
-        R.sample_row = dalloc_copy(d_prow, srow.data(), srow.size(), s);
-        R.haps = H;
-        R.ev = (int32_t *)d_ev.reserve(sizeof(int32_t) * (size_t)2 * npt * R.cap * (2 + 2 * R.depth));
-        R.ev_cnt = (int32_t *)d_evcnt.reserve(sizeof(int32_t) * 2 * npt);
-        R.status = status + 1;
-        R.out_states = states;
-        R.out_stride = kstride;
-        R.out_K = Kdev;
-    }
-    int32_t *ptg = dalloc_copy(d_ptg, pbwt_targets.data(), pbwt_targets.size(), s);
-    std::vector<uint8_t> stored_h(std::max<size_t>(cls_mod.size(), 1) * std::max(P->Lhq, 1));
-    std::vector<int32_t> start_h(std::max(2 * npt, 1));
-    uint8_t *pst = (uint8_t *)d_pst.reserve(stored_h.size());
-    int32_t *pstart = (int32_t *)d_pstart.reserve(sizeof(int32_t) * start_h.size());
-    R.stored = pst;
-    R.start = pstart;
-
     gh_run_stats st;
     std::memset(&st, 0, sizeof(st));
     EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1;
     e_sel.make(); e_cond.make(); e_fb.make(); e_ph.make(); e_k0.make(); e_k1.make();
     const dim3 gsite((L + 255) / 256, T), bsite(256);
     int main_done = 0;
-    std::vector<int32_t> st_host;
+    std::vector<CondItem> &items = G->items;
+    std::vector<PhItem> &ph = G->ph;
 
     for (int it = 0; it < n_iter; it++) {
         const int stage = it == 0 ? 0 : (it <= prm->burnin ? 1 : 2);
         GH_HIP(hipEventRecord(e_sel.a, s));
         if (stage == 0 || it == 1) {
-            // host-provided state sets: INIT selection, and full-panel targets (fixed iota)
+            // host-side state sets: INIT selection, and full-panel targets (fixed iota)
+            bool any = false;
             for (int t = 0; t < T; t++) {
                 std::vector<int32_t> S;
-                if (stage == 0 && kinit > 0) init_states(P, t, gl_h + (size_t)t * L * 3, flat_h + (size_t)t * L, kinit,
-                                                         seed, S, unph);
-                else if (kp[t] >= N) { S.resize(N); std::iota(S.begin(), S.end(), 0); }
-                else if (stage == 0) throw Error(GH_ERR_STATE, "target " + std::to_string(t) + ": Kinit = 0 leaves no INIT states");
+                if (stage == 0 && G->kinit > 0)
+                    init_states(P, t, G->gl_h.data() + (size_t)t * L * 3, G->flat_h.data() + (size_t)t * L,
+                                G->kinit, seed, S, G->unph);
+                else if (G->kp[t] >= N) { S.resize(N); std::iota(S.begin(), S.end(), 0); }
                 else continue;
                 GH_CHECK(!S.empty(), GH_ERR_STATE, "target " + std::to_string(t) + ": zero conditioning states");
-                GH_HIP(hipMemcpyAsync(states + (size_t)t * kstride, S.data(), sizeof(int32_t) * S.size(),
+                GH_HIP(hipMemcpyAsync(states + (size_t)t * G->kstride, S.data(), sizeof(int32_t) * S.size(),
                                       hipMemcpyHostToDevice, s));
                 items[t].K = (int)S.size();
                 items[t].W = ((int)S.size() + 63) / 64;
-                GH_HIP(hipMemcpyAsync(&items_d[t], &items[t], sizeof(CondItem), hipMemcpyHostToDevice, s));
-                // the item array is read back after each iteration; keep this copy alive until then
-                GH_HIP(hipStreamSynchronize(s));
+                GH_HIP(hipStreamSynchronize(s));   // S is a temporary
+                any = true;
             }
+            if (any) GH_HIP(hipMemcpyAsync(items_d, items.data(), sizeof(CondItem) * T, hipMemcpyHostToDevice, s));
         }
         if (stage > 0 && npt > 0) {
-            // haplotype_set.cpp:335-348: one stored site per group and a random start per haplotype
-            std::fill(stored_h.begin(), stored_h.end(), 0);
-            for (size_t c = 0; c < cls_mod.size(); c++) {
+            // haplotype_set.cpp:335-348: one stored site per group, random start per haplotype
+            std::fill(G->stored_h.begin(), G->stored_h.end(), 0);
+            for (size_t c = 0; c < G->cls_mod.size(); c++) {
                 int lo = 0;
-                for (size_t gi = 0; gi < cls_grp[c].size(); gi++) {
-                    const int hi = cls_grp[c][gi] - 1;
+                for (size_t gi = 0; gi < G->cls_grp[c].size(); gi++) {
+                    const int hi = G->cls_grp[c][gi] - 1;
                     const uint32_t x = philox_draw(seed, (uint32_t)it, P_PBWT_STORED, (uint32_t)gi, (uint32_t)c);
-                    stored_h[c * P->Lhq + urange(x, (uint32_t)lo, (uint32_t)hi)] = 1;
-                    lo = cls_grp[c][gi];
+                    G->stored_h[c * P->Lhq + urange(x, (uint32_t)lo, (uint32_t)hi)] = 1;
+                    lo = G->cls_grp[c][gi];
                 }
             }
             for (int i = 0; i < npt; i++)
                 for (int j = 0; j < 2; j++) {
-                    const uint32_t x = philox_draw(seed, (uint32_t)it, P_PBWT_START, (uint32_t)(2 * pbwt_targets[i] + j), 0u);
-                    start_h[2 * i + j] = (int32_t)urange(x, 0u, (uint32_t)(N - 1));
+                    const uint32_t x =
+                        philox_draw(seed, (uint32_t)it, P_PBWT_START, (uint32_t)(2 * G->pbwt_targets[i] + j), 0u);
+                    G->start_h[2 * i + j] = (int32_t)urange(x, 0u, (uint32_t)(N - 1));
                 }
-            GH_HIP(hipMemcpyAsync(pst, stored_h.data(), stored_h.size(), hipMemcpyHostToDevice, s));
-            GH_HIP(hipMemcpyAsync(pstart, start_h.data(), sizeof(int32_t) * 2 * npt, hipMemcpyHostToDevice, s));
+            GH_HIP(hipMemcpyAsync(pst, G->stored_h.data(), G->stored_h.size(), hipMemcpyHostToDevice, s));
+            GH_HIP(hipMemcpyAsync(pstart, G->start_h.data(), sizeof(int32_t) * 2 * npt, hipMemcpyHostToDevice, s));
             pbwt_select_launch(P, R, s);
             hipLaunchKernelGGL(set_k_kernel, dim3((npt + 127) / 128), dim3(128), 0, s, items_d, Kdev, ptg, npt);
             GH_HIP(hipGetLastError());
@@ -430,31 +493,29 @@ void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, c
         }
         GH_HIP(hipEventRecord(e_sel.b, s));
         GH_HIP(hipEventRecord(e_cond.a, s));
-        condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, maxW, s);
+        condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, G->maxW, s);
         GH_HIP(hipEventRecord(e_cond.b, s));
         GH_HIP(hipEventRecord(e_fb.a, s));
         // H0 (caller_algorithm.cpp:58-65)
-        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, d_gl.as<uint8_t>(), d_flat.as<uint8_t>(), H, L, T,
-                           stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
+        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k0.a, s));
-        fb_launch(P, T, fb_d, items_d, maxW, poly, tr, hvar, HL, d_flat.as<uint8_t>(), alpha, HP0, ee_i, ed_i, s);
+        fb_launch(P, T, fb_d, items_d, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HP0, G->ee_i, G->ed_i, s);
         GH_HIP(hipEventRecord(e_k0.b, s));
-        fb_mono_launch(P, T, fb_d, items_d, pos, HL, d_flat.as<uint8_t>(), HP0, ee_i, ed_i, s);
+        fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
-        // H1
-        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, d_gl.as<uint8_t>(), d_flat.as<uint8_t>(), H, L, T, 1, 1,
-                           prm->min_gl, HL);
+        // H1 (caller_algorithm.cpp:66-71)
+        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k1.a, s));
-        fb_launch(P, T, fb_d, items_d, maxW, poly, tr, hvar, HL, d_flat.as<uint8_t>(), alpha, HP1, ee_i, ed_i, s);
+        fb_launch(P, T, fb_d, items_d, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i, G->ed_i, s);
         GH_HIP(hipEventRecord(e_k1.b, s));
-        fb_mono_launch(P, T, fb_d, items_d, pos, HL, d_flat.as<uint8_t>(), HP1, ee_i, ed_i, s);
+        fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
         GH_HIP(hipGetLastError());
         GH_HIP(hipEventRecord(e_fb.b, s));
         // phasing (phasing_hmm::rephaseHaplotypes)
         GH_HIP(hipEventRecord(e_ph.a, s));
-        PhItem *ph_d = dalloc_copy(d_ph, ph.data(), T, s);
-        phase_prep_launch(T, ph_d, items_d, poly, H, H, d_flat.as<uint8_t>(), P->d_flags.as<uint8_t>(), loci, s);
+        PhItem *ph_d = dalloc_copy(G->d_ph, ph.data(), T, s);
+        phase_prep_launch(T, ph_d, items_d, poly, H, H, flat_d, P->d_flags.as<uint8_t>(), loci, s);
         GH_HIP(hipMemcpyAsync(ph.data(), ph_d, sizeof(PhItem) * T, hipMemcpyDeviceToHost, s));
         GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
         GH_HIP(hipStreamSynchronize(s));
@@ -473,16 +534,16 @@ void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, c
             st.fb_poly_sites += 2.0 * items[t].Lp;
             st.ph_state_sites += (double)items[t].K * ph[t].n_loci;
             st.ph_segments += ph[t].n_segs + ph[t].n_miss;
-            if (nstates_h) nstates_h[t * n_iter + it] = items[t].K;
+            G->nstates[(size_t)t * n_iter + it] = items[t].K;
         }
         GH_HIP(hipMemcpyAsync(ph_d, ph.data(), sizeof(PhItem) * T, hipMemcpyHostToDevice, s));
         PhaseBufs PB;
-        PB.seg = (float *)d_seg.reserve(sizeof(float) * std::max<int64_t>(so, 1));
-        PB.segsum = (float *)d_segsum.reserve(sizeof(float) * std::max<int64_t>(ss, 1));
-        PB.imp = (float *)d_imp.reserve(sizeof(float) * std::max<int64_t>(io, 1));
-        PB.impsum = (float *)d_impsum.reserve(sizeof(float) * std::max<int64_t>(is, 1));
-        PB.of1 = (float *)d_of1.reserve(sizeof(float) * std::max<int64_t>(is, 1));
-        PB.dip = (int32_t *)d_dip.reserve(sizeof(int32_t) * std::max<int64_t>(dd, 1));
+        PB.seg = (float *)G->d_seg.reserve(sizeof(float) * std::max<int64_t>(so, 1));
+        PB.segsum = (float *)G->d_segsum.reserve(sizeof(float) * std::max<int64_t>(ss, 1));
+        PB.imp = (float *)G->d_imp.reserve(sizeof(float) * std::max<int64_t>(io, 1));
+        PB.impsum = (float *)G->d_impsum.reserve(sizeof(float) * std::max<int64_t>(is, 1));
+        PB.of1 = (float *)G->d_of1.reserve(sizeof(float) * std::max<int64_t>(is, 1));
+        PB.dip = (int32_t *)G->d_dip.reserve(sizeof(int32_t) * std::max<int64_t>(dd, 1));
         phase_launch(P, T, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
                      (uint32_t)it, status, s);
         GH_HIP(hipEventRecord(e_ph.b, s));
@@ -503,18 +564,46 @@ void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, c
         st.fb_kernel_ms += e_k0.ms() + e_k1.ms();
         st.fb_launches += 2;
     }
-    hipLaunchKernelGGL(finalize_kernel, gsite, bsite, 0, s, gp0, gp1, sto, hds, h0o, h1o, L, main_done);
+    hipLaunchKernelGGL(finalize_kernel, gsite, bsite, 0, s, gp0, gp1, sto, hds, G->d_h0.as<uint8_t>(),
+                       G->d_h1.as<uint8_t>(), L, main_done);
     GH_HIP(hipGetLastError());
-    const size_t TL = (size_t)T * L;
-    if (gp0_h) GH_HIP(hipMemcpyAsync(gp0_h, gp0, sizeof(float) * TL, hipMemcpyDeviceToHost, s));
-    if (gp1_h) GH_HIP(hipMemcpyAsync(gp1_h, gp1, sizeof(float) * TL, hipMemcpyDeviceToHost, s));
-    if (hds_h) GH_HIP(hipMemcpyAsync(hds_h, hds, TL, hipMemcpyDeviceToHost, s));
-    if (h0_h) GH_HIP(hipMemcpyAsync(h0_h, h0o, TL, hipMemcpyDeviceToHost, s));
-    if (h1_h) GH_HIP(hipMemcpyAsync(h1_h, h1o, TL, hipMemcpyDeviceToHost, s));
     GH_HIP(hipStreamSynchronize(s));
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     st.ms_other = st.ms_total - st.ms_select - st.ms_condset - st.ms_fb - st.ms_phase;
-    if (stats) *stats = st;
+    G->st = st;
+    G->done = true;
+}
+
+void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
+               int32_t *nstates_h, gh_run_stats *stats) {
+    GH_CHECK(G && G->done, GH_ERR_ARG, "gh_run_fetch: the run has not been executed");
+    hipStream_t s = G->P->stream;
+    const size_t TL = (size_t)G->T * G->L;
+    if (gp0_h) GH_HIP(hipMemcpyAsync(gp0_h, G->d_gp0.p, sizeof(float) * TL, hipMemcpyDeviceToHost, s));
+    if (gp1_h) GH_HIP(hipMemcpyAsync(gp1_h, G->d_gp1.p, sizeof(float) * TL, hipMemcpyDeviceToHost, s));
+    if (hds_h) GH_HIP(hipMemcpyAsync(hds_h, G->d_hds.p, TL, hipMemcpyDeviceToHost, s));
+    if (h0_h) GH_HIP(hipMemcpyAsync(h0_h, G->d_h0.p, TL, hipMemcpyDeviceToHost, s));
+    if (h1_h) GH_HIP(hipMemcpyAsync(h1_h, G->d_h1.p, TL, hipMemcpyDeviceToHost, s));
+    GH_HIP(hipStreamSynchronize(s));
+    if (nstates_h) std::memcpy(nstates_h, G->nstates.data(), sizeof(int32_t) * G->nstates.size());
+    if (stats) *stats = G->st;
+}
+
+void run_destroy(gh_run *G) { delete G; }
+
+void phase_run(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
+               const int32_t *kpt, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
+               int32_t *nstates_h, gh_run_stats *stats) {
+    gh_run *G = nullptr;
+    run_create(P, T, gl_h, flat_h, prm, kpt, &G);
+    try {
+        run_execute(G);
+        run_fetch(G, gp0_h, gp1_h, hds_h, h0_h, h1_h, nstates_h, stats);
+    } catch (...) {
+        run_destroy(G);
+        throw;
+    }
+    run_destroy(G);
 }
 
 }  // namespace gh

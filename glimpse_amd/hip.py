@@ -62,6 +62,10 @@ def lib():
         L.gh_pbwt_select.argtypes = [vp, i32, vp, vp, vp, i32, f32, vp, vp, vp, i64, i32, vp, vp]
         L.gh_phase_run.argtypes = [vp, i32, vp, vp, ctypes.POINTER(Params), vp, vp, vp, vp, vp, vp, vp,
                                    ctypes.POINTER(RunStats)]
+        L.gh_run_create.argtypes = [vp, i32, vp, vp, ctypes.POINTER(Params), vp, ctypes.POINTER(vp)]
+        L.gh_run_execute.argtypes = [vp]
+        L.gh_run_fetch.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(RunStats)]
+        L.gh_run_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -236,3 +240,54 @@ class Panel:
                                   _p(out["nstates"]), ctypes.byref(st)))
         out["stats"] = {n: getattr(st, n) for n, _ in RunStats._fields_}
         return out
+
+
+class Run:
+    """A prepared Gibbs run (gh_run_*): inputs resident on the device.
+    execute() runs the whole INIT + burn-in + main schedule on the GPU and
+    releases the GIL while it runs, so runs on different panels (streams) can
+    be driven concurrently from Python threads."""
+
+    def __init__(self, panel, gl, flat, params, kpbwt_per_target=None):
+        self.panel = panel
+        self.gl = np.ascontiguousarray(gl, np.uint8)
+        self.flat = np.ascontiguousarray(flat, np.uint8)
+        self.T, self.L = self.flat.shape
+        self.params = params
+        self.n_iter = 1 + params.burnin + params.main_iter
+        kpt = None if kpbwt_per_target is None else np.ascontiguousarray(kpbwt_per_target, np.int32)
+        self._kpt = kpt
+        h = vp()
+        _check(lib().gh_run_create(panel.h, self.T, _p(self.gl), _p(self.flat), ctypes.byref(params), _p(kpt),
+                                   ctypes.byref(h)))
+        self.h = h
+
+    def execute(self):
+        _check(lib().gh_run_execute(self.h))
+
+    def fetch(self):
+        T, L = self.T, self.L
+        out = dict(gp0=np.zeros((T, L), np.float32), gp1=np.zeros((T, L), np.float32),
+                   hds=np.zeros((T, L), np.uint8), h0=np.zeros((T, L), np.uint8), h1=np.zeros((T, L), np.uint8),
+                   nstates=np.zeros((T, self.n_iter), np.int32))
+        st = RunStats()
+        _check(lib().gh_run_fetch(self.h, _p(out["gp0"]), _p(out["gp1"]), _p(out["hds"]), _p(out["h0"]),
+                                  _p(out["h1"]), _p(out["nstates"]), ctypes.byref(st)))
+        out["stats"] = {n: getattr(st, n) for n, _ in RunStats._fields_}
+        return out
+
+    def stats(self):
+        st = RunStats()
+        _check(lib().gh_run_fetch(self.h, None, None, None, None, None, None, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in RunStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().gh_run_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -146,6 +146,21 @@ int gh_phase_run(gh_panel *p, int32_t T, const uint8_t *gl, const uint8_t *flat,
                  const int32_t *kpbwt_per_target, float *gp0, float *gp1, uint8_t *hds,
                  uint8_t *h0, uint8_t *h1, int32_t *nstates, gh_run_stats *stats);
 
+/* The same run split in three so a caller can time or repeat the device part
+ * with its inputs resident: create uploads the inputs and allocates the
+ * workspace (everything gh_phase_run allocates), execute runs the full Gibbs
+ * schedule on the device (a fresh chain each call, same seed, same result),
+ * fetch copies the outputs back.  These have no single reference counterpart:
+ * together they are caller::phase_loop split at its I/O boundaries
+ * (phase/src/caller/caller_management.cpp:47-56). */
+typedef struct gh_run gh_run;
+int gh_run_create(gh_panel *p, int32_t T, const uint8_t *gl, const uint8_t *flat, const gh_params *prm,
+                  const int32_t *kpbwt_per_target, gh_run **out);
+int gh_run_execute(gh_run *r);
+int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1, int32_t *nstates,
+                 gh_run_stats *stats);
+int gh_run_destroy(gh_run *r);
+
 #ifdef __cplusplus
 }
 #endif
