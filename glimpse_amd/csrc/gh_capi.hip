@@ -171,6 +171,7 @@ int gh_impute_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int3
         B.build(P, n, state_ptr, states, ne, s);
         std::vector<FbItem> fb(n);
         int64_t aoff = 0;
+        const int C = fb_block_sites(B.maxW);
         for (int i = 0; i < n; i++) {
             fb[i].K = B.h[i].K;
             fb[i].W = B.h[i].W;
@@ -179,7 +180,7 @@ int gh_impute_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int3
             fb[i].alpha_off = aoff;
             fb[i].hl_off = (int64_t)i * 2 * L;
             fb[i].flat_off = (int64_t)i * L;
-            aoff += (int64_t)B.h[i].Lp * 64 * B.h[i].W;
+            aoff += (int64_t)((B.h[i].Lp + C - 1) / C) * 64 * B.h[i].W;
         }
         DBuf d_fb, d_HL, d_flat, d_alpha, d_HP;
         dalloc_copy(d_fb, fb.data(), n, s);

@@ -317,13 +317,14 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         G->items.resize(T);
         G->fb.resize(T);
         int64_t words = 0, aoff = 0;
+        for (int t = 0; t < T; t++) G->maxW = std::max(G->maxW, (G->kmax[t] + 63) / 64);
+        const int C = fb_block_sites(G->maxW);
         for (int t = 0; t < T; t++) {
             const int Wt = (G->kmax[t] + 63) / 64;
-            G->maxW = std::max(G->maxW, Wt);
             G->items[t] = CondItem{G->kmax[t], Wt, 0, 0, (int64_t)t * G->kstride, (int64_t)t * L, words};
             words += (int64_t)L * Wt;
             G->fb[t] = FbItem{0, 0, 0, t, aoff, (int64_t)t * 2 * L, (int64_t)t * L};
-            aoff += (int64_t)L * 64 * Wt;
+            aoff += (int64_t)((L + C - 1) / C) * 64 * Wt;
         }
         dalloc_copy(G->d_items, G->items.data(), T, s);
         dalloc_copy(G->d_fb, G->fb.data(), T, s);

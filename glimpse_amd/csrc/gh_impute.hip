@@ -1,157 +1,289 @@
 // gh_impute.hip -- haploid Li-Stephens forward-backward
 // (imputation_hmm::forward / backward, phase/src/models/imputation_hmm.cpp:143-430).
 //
-// One wavefront per haplotype item.  Lane j owns states j, 64+j, 128+j, ...
-// (MW registers), so word w of a Hvar row gives lane j its state's allele as
-// bit j: every row word is a wave-uniform (scalar) load and alpha rows are
-// written/read as contiguous 256-byte segments per word.  Per-site scaling
-// sums use DPP row reductions plus v_readlane (no LDS, no barriers).
+// One workgroup of NW wavefronts per haplotype item.  Word w of a Hvar row (64
+// states) belongs to wave w % NW; lane j of that wave owns state 64w + j, so a
+// row word gives every lane its state's allele as bit j.
 //
-// Arithmetic follows the reference per state: alpha_l = (alpha_{l-1} * nt/S +
-// t/K) * e (one FMA then a multiply), beta~ = beta * nt/S + t/K, posteriors
-// divide out the site's own emission.  Only the order of the cross-state sums
-// differs from the reference's 8-lane AVX2 order.
+// Latency: the per-site metadata (site id, flat/LQ flags, emissions,
+// transitions, likelihoods) of a 64-site tile is gathered once, one site per
+// lane, and broadcast with v_readlane inside the serial site loop; the tile's
+// Hvar rows are staged in LDS.  Per-site scaling sums use DPP row reductions
+// and v_readlane, then one LDS exchange (double-buffered by site parity, one
+// barrier) across the NW waves.
+//
+// Memory: alpha never round-trips through HBM.  The forward pass keeps alpha
+// in registers and writes a checkpoint (the full alpha vector) every C sites;
+// the backward pass walks blocks of C sites from the end, recomputes the
+// block's alpha from its checkpoint into LDS (thread-private slots, the same
+// operations in the same order, so bit-identical to the forward values) and
+// runs the backward recursion against it.  HBM traffic per site is the Hvar row
+// (three reads) plus 1/C of an alpha vector; the arithmetic per state-site is
+// about 1.5x a plain forward-backward.
+//
+// Arithmetic per state follows the reference: alpha_l = (alpha_{l-1}*nt/S +
+// t/K)*e (one FMA, one multiply), beta~ = beta*nt/S + t/K, the site's own
+// emission divided out of the allele posteriors; only the order of the sums
+// over states differs from the reference's 8-lane AVX2 order.
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
 
 namespace gh {
 
-template <int MW>
-__global__ __launch_bounds__(64) void fb_kernel(const FbItem *__restrict__ fb, const CondItem *__restrict__ cond,
-                                               const int32_t *__restrict__ poly_all, const float *__restrict__ t_all,
-                                               const uint64_t *__restrict__ hvar_all, const float *__restrict__ HL_all,
-                                               const uint8_t *__restrict__ flat_all,
-                                               const uint8_t *__restrict__ flags, float *__restrict__ alpha_all,
-                                               float *__restrict__ HP_all, float ee, float ed) {
-    const FbItem f = fb[blockIdx.x];
-    const CondItem c = cond[f.cond];
-    const int K = c.K, W = c.W, Lp = c.Lp;
-    const int lane = threadIdx.x;
-    const int32_t *poly = poly_all + c.site_off;
-    const float *t = t_all + c.site_off;
-    const uint64_t *hv = hvar_all + c.hvar_off;
-    const float *HL = HL_all + f.hl_off;
-    const uint8_t *flat = flat_all + f.flat_off;
-    float *al = alpha_all + f.alpha_off;
-    float *HP = HP_all + f.hl_off;
-    const float Kf = (float)K;
-    const float invK = 1.0f / Kf;
-    const size_t rstride = (size_t)64 * W;
+constexpr int FB_TILE = 64;
+constexpr int FB_LDS_BYTES = 156 * 1024;
 
-    bool valid[MW];
-#pragma unroll
-    for (int i = 0; i < MW; i++) valid[i] = (i < W) && (64 * i + lane < K);
+struct FbArgs {
+    const FbItem *fb;
+    const CondItem *cond;
+    const int32_t *poly_all;
+    const float *t_all;
+    const uint64_t *hvar_all;
+    const float *HL_all;
+    const uint8_t *flat_all;
+    const uint8_t *flags;
+    float *ckpt_all;
+    float *HP_all;
+    float *mass_all;   // optional: per-state posterior mass (summed over sites)
+    float ee, ed;
+    int C;             // sites per checkpoint block (divides FB_TILE)
+    int maxW;          // LDS row stride bound
+};
 
-    // ---------------- forward
-    float a[MW];
-    float sum_prev = 1.0f;
-    for (int l = 0; l < Lp; l++) {
-        const int site = poly[l];
-        const bool fl = flat[site] || (flags[site] & SF_LQ);
-        const uint64_t *row = hv + (size_t)l * W;
-        float s = 0.0f;
-        if (fl) {
-            if (l == 0) {
+__device__ __forceinline__ float rlf(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ int rli(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+
+template <int NW, int MWL>
+__global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const FbItem f = A.fb[blockIdx.x];
+    const CondItem c = A.cond[f.cond];
+    const int K = c.K, W = c.W, Lp = c.Lp, C = A.C;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (Lp <= 0) return;
+    uint64_t *rows = (uint64_t *)smem;                                    // [FB_TILE][W]
+    float *ablk = (float *)(smem + (size_t)FB_TILE * A.maxW * 8);        // [C][NW*MWL][64]
+    float *red = ablk + (size_t)C * NW * MWL * 64;                        // [2][NW][4]
+    const int32_t *poly = A.poly_all + c.site_off;
+    const float *tr = A.t_all + c.site_off;
+    const uint64_t *hv = A.hvar_all + c.hvar_off;
+    const float *HL = A.HL_all + f.hl_off;
+    const uint8_t *flat = A.flat_all + f.flat_off;
+    float *ck = A.ckpt_all + f.alpha_off;
+    float *HP = A.HP_all + f.hl_off;
+    const float ee = A.ee, ed = A.ed;
+    const float Kf = (float)K, invK = 1.0f / Kf;
+    const size_t vstride = (size_t)64 * W;   // one alpha vector (checkpoint) in floats
+
+    int wid[MWL];
+    bool wok[MWL], valid[MWL];
 #pragma unroll
-                for (int i = 0; i < MW; i++) a[i] = valid[i] ? invK : 0.0f;
-            } else {
-                const float tt = t[l - 1];
-                const float f1 = tt / Kf, f2 = (1.0f - tt) / sum_prev;
-#pragma unroll
-                for (int i = 0; i < MW; i++) {
-                    a[i] = valid[i] ? fmaf(a[i], f2, f1) : 0.0f;
-                    s += a[i];
-                }
-            }
-        } else {
-            const float h0 = HL[2 * site], h1 = HL[2 * site + 1];
-            const float p0 = h0 * ee + h1 * ed, p1 = h0 * ed + h1 * ee;
-            const float e0 = p0 / (p0 + p1), e1 = p1 / (p0 + p1);
-            if (l == 0) {
-#pragma unroll
-                for (int i = 0; i < MW; i++) {
-                    if (i < W) {
-                        const bool b = (row[i] >> lane) & 1ull;
-                        a[i] = valid[i] ? (b ? e1 : e0) * invK : 0.0f;
-                        s += a[i];
-                    }
-                }
-            } else {
-                const float tt = t[l - 1];
-                const float f1 = tt / Kf, f2 = (1.0f - tt) / sum_prev;
-#pragma unroll
-                for (int i = 0; i < MW; i++) {
-                    if (i < W) {
-                        const bool b = (row[i] >> lane) & 1ull;
-                        a[i] = valid[i] ? fmaf(a[i], f2, f1) * (b ? e1 : e0) : 0.0f;
-                        s += a[i];
-                    }
-                }
-            }
-        }
-        sum_prev = (fl && l == 0) ? 1.0f : wave_sum(s);
-        float *dst = al + (size_t)l * rstride + lane;
-#pragma unroll
-        for (int i = 0; i < MW; i++)
-            if (i < W) dst[64 * i] = a[i];
+    for (int i = 0; i < MWL; i++) {
+        wok[i] = i * NW + wv < W;
+        wid[i] = wok[i] ? i * NW + wv : 0;   // clamped so row reads stay inside the tile
+        valid[i] = wok[i] && (64 * wid[i] + lane < K);
     }
 
-    // ---------------- backward + posteriors
-    float b[MW];
+    // cross-wave sums (fixed order, so the forward and the recomputation agree bit for bit).
+    // The LDS slot alternates with every reduction, so one barrier per reduction suffices.
+    int par = 0;
+    auto xsum1 = [&](float v) -> float {
+        v = wave_sum(v);
+        if (NW == 1) return v;
+        par ^= 1;
+        if (lane == 0) red[(par * NW + wv) * 4] = v;
+        __syncthreads();
+        float s = 0.0f;
 #pragma unroll
-    for (int i = 0; i < MW; i++) b[i] = 0.0f;
-    float bsum_next = 1.0f;
-    for (int l = Lp - 1; l >= 0; l--) {
-        const int site = poly[l];
-        const bool isflat = flat[site] != 0;
-        const bool fl = isflat || (flags[site] & SF_LQ);
-        const uint64_t *row = hv + (size_t)l * W;
-        const float *src = al + (size_t)l * rstride + lane;
-        float p0 = 0.0f, p1 = 0.0f, bs = 0.0f;
-        float e0 = 1.0f, e1 = 1.0f;
-        const float h0 = HL[2 * site], h1 = HL[2 * site + 1];
-        if (!fl) {
-            const float q0 = h0 * ee + h1 * ed, q1 = h0 * ed + h1 * ee;
-            e0 = q0 / (q0 + q1);
-            e1 = q1 / (q0 + q1);
-        }
-        if (l == Lp - 1) {
-#pragma unroll
-            for (int i = 0; i < MW; i++) {
-                if (i < W) {
-                    const bool bit = (row[i] >> lane) & 1ull;
-                    const float av = src[64 * i];
-                    b[i] = valid[i] ? (fl ? invK : (bit ? e1 : e0) * invK) : 0.0f;
-                    if (bit) p1 += av; else p0 += av;
-                    bs += b[i];
-                }
-            }
-        } else {
-            const float tt = t[l];
-            const float f1 = tt / Kf, f2 = (1.0f - tt) / bsum_next;
-#pragma unroll
-            for (int i = 0; i < MW; i++) {
-                if (i < W) {
-                    const bool bit = (row[i] >> lane) & 1ull;
-                    const float av = src[64 * i];
-                    const float bt = valid[i] ? fmaf(b[i], f2, f1) : 0.0f;
-                    const float dp = av * bt;
-                    if (bit) p1 += dp; else p0 += dp;
-                    b[i] = fl ? bt : bt * (bit ? e1 : e0);
-                    bs += b[i];
-                }
-            }
-        }
-        wave_sum3(p0, p1, bs);
-        if (fl && l == Lp - 1) bs = 1.0f;
-        if (!fl) { p0 /= e0; p1 /= e1; }
-        float o0 = p0 * ee + p1 * ed, o1 = p0 * ed + p1 * ee;
-        if (!isflat) { o0 *= h0; o1 *= h1; }
+        for (int q = 0; q < NW; q++) s += red[(par * NW + q) * 4];
+        return s;
+    };
+    auto xsum3 = [&](float &x, float &y, float &z) {
+        wave_sum3(x, y, z);
+        if (NW == 1) return;
+        par ^= 1;
         if (lane == 0) {
-            HP[2 * site] = o0 / (o0 + o1);
-            HP[2 * site + 1] = o1 / (o0 + o1);
+            red[(par * NW + wv) * 4 + 0] = x;
+            red[(par * NW + wv) * 4 + 1] = y;
+            red[(par * NW + wv) * 4 + 2] = z;
         }
-        bsum_next = bs;
+        __syncthreads();
+        x = 0.0f; y = 0.0f; z = 0.0f;
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            x += red[(par * NW + q) * 4 + 0];
+            y += red[(par * NW + q) * 4 + 1];
+            z += red[(par * NW + q) * 4 + 2];
+        }
+    };
+
+    // tile metadata: lane j of every wave describes site l0 + j
+    int m_site = 0, m_code = 0;
+    float m_e0 = 1.0f, m_e1 = 1.0f, m_h0 = 0.5f, m_h1 = 0.5f, m_tin = 0.0f, m_tout = 0.0f;
+    auto load_tile = [&](int l0, int n) {
+        __syncthreads();   // previous tile's LDS rows are no longer read
+        if (lane < n) {
+            const int l = l0 + lane;
+            const int site = poly[l];
+            const bool isflat = flat[site] != 0;
+            const bool fl = isflat || (A.flags[site] & SF_LQ);
+            const float h0 = HL[2 * site], h1 = HL[2 * site + 1];
+            const float p0 = h0 * ee + h1 * ed, p1 = h0 * ed + h1 * ee;
+            m_site = site;
+            m_code = (fl ? 1 : 0) | (isflat ? 2 : 0);
+            m_e0 = p0 / (p0 + p1);
+            m_e1 = p1 / (p0 + p1);
+            m_h0 = h0;
+            m_h1 = h1;
+            m_tin = l > 0 ? tr[l - 1] : 0.0f;
+            m_tout = l < Lp - 1 ? tr[l] : 0.0f;
+        }
+        const uint64_t *src = hv + (size_t)l0 * W;
+        for (int e = tid; e < n * W; e += 64 * NW) rows[e] = src[e];
+        __syncthreads();
+    };
+
+    float a[MWL];
+    // forward step at site l (tile offset j): alpha in a[], scaling sum in sp
+    auto fwd_site = [&](int j, int l, float &sp) {
+        const int code = rli(m_code, j);
+        const bool fl = code & 1;
+        const float e0 = fl ? 1.0f : rlf(m_e0, j), e1 = fl ? 1.0f : rlf(m_e1, j);
+        float s = 0.0f;
+        if (l == 0) {
+#pragma unroll
+            for (int i = 0; i < MWL; i++) {
+                const bool b = (rows[j * W + wid[i]] >> lane) & 1ull;
+                const float x = fl ? invK : (b ? e1 : e0) * invK;
+                a[i] = valid[i] ? x : 0.0f;
+                s += a[i];
+            }
+            sp = fl ? 1.0f : xsum1(s);
+        } else {
+            const float tt = rlf(m_tin, j);
+            const float f1 = tt / Kf, f2 = (1.0f - tt) / sp;
+#pragma unroll
+            for (int i = 0; i < MWL; i++) {
+                const bool b = (rows[j * W + wid[i]] >> lane) & 1ull;
+                const float x = fmaf(a[i], f2, f1) * (b ? e1 : e0);
+                a[i] = valid[i] ? x : 0.0f;
+                s += a[i];
+            }
+            sp = xsum1(s);
+        }
+    };
+
+    // ---------------- forward: checkpoints every C sites
+    {
+        float sp = 1.0f;
+        for (int l0 = 0; l0 < Lp; l0 += FB_TILE) {
+            const int n = min(FB_TILE, Lp - l0);
+            load_tile(l0, n);
+            for (int j = 0; j < n; j++) {
+                const int l = l0 + j;
+                fwd_site(j, l, sp);
+                if (l % C == 0) {
+                    float *dst = ck + (size_t)(l / C) * vstride + lane;
+#pragma unroll
+                    for (int i = 0; i < MWL; i++)
+                        if (wok[i]) dst[64 * wid[i]] = a[i];
+                }
+            }
+        }
+    }
+
+    // ---------------- backward over checkpoint blocks, alpha recomputed into LDS
+    float b[MWL], ms[MWL];
+#pragma unroll
+    for (int i = 0; i < MWL; i++) { b[i] = 0.0f; ms[i] = 0.0f; }
+    float bsum_next = 1.0f;
+    const int last_tile = ((Lp - 1) / FB_TILE) * FB_TILE;
+    for (int l0 = last_tile; l0 >= 0; l0 -= FB_TILE) {
+        const int n = min(FB_TILE, Lp - l0);
+        load_tile(l0, n);
+        for (int q = (n - 1) / C; q >= 0; q--) {
+            const int bs = l0 + q * C, be = min(bs + C, l0 + n);
+            // recompute alpha on [bs, be) from the checkpoint at bs
+            const float *srcv = ck + (size_t)(bs / C) * vstride + lane;
+#pragma unroll
+            for (int i = 0; i < MWL; i++) a[i] = wok[i] ? srcv[64 * wid[i]] : 0.0f;
+            float sp;
+            {
+                const bool fl0 = rli(m_code, bs - l0) & 1;
+                float s = 0.0f;
+#pragma unroll
+                for (int i = 0; i < MWL; i++) s += a[i];
+                sp = (bs == 0 && fl0) ? 1.0f : xsum1(s);
+            }
+#pragma unroll
+            for (int i = 0; i < MWL; i++) ablk[((size_t)0 * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
+            for (int l = bs + 1; l < be; l++) {
+                fwd_site(l - l0, l, sp);
+#pragma unroll
+                for (int i = 0; i < MWL; i++) ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
+            }
+            // backward recursion on the block
+            for (int l = be - 1; l >= bs; l--) {
+                const int j = l - l0;
+                const int code = rli(m_code, j);
+                const bool fl = code & 1, isflat = code & 2;
+                const float e0 = fl ? 1.0f : rlf(m_e0, j), e1 = fl ? 1.0f : rlf(m_e1, j);
+                float p0 = 0.0f, p1 = 0.0f, bs_ = 0.0f;
+                float dpv[MWL];
+                if (l == Lp - 1) {
+#pragma unroll
+                    for (int i = 0; i < MWL; i++) {
+                        const bool bit = (rows[j * W + wid[i]] >> lane) & 1ull;
+                        const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
+                        const float bv = fl ? invK : (bit ? e1 : e0) * invK;
+                        b[i] = valid[i] ? bv : 0.0f;
+                        dpv[i] = av;
+                        p1 += bit ? av : 0.0f;
+                        p0 += bit ? 0.0f : av;
+                        bs_ += b[i];
+                    }
+                } else {
+                    const float tt = rlf(m_tout, j);
+                    const float f1 = tt / Kf, f2 = (1.0f - tt) / bsum_next;
+#pragma unroll
+                    for (int i = 0; i < MWL; i++) {
+                        const bool bit = (rows[j * W + wid[i]] >> lane) & 1ull;
+                        const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
+                        const float bt = valid[i] ? fmaf(b[i], f2, f1) : 0.0f;
+                        const float dp = av * bt;
+                        dpv[i] = dp;
+                        p1 += bit ? dp : 0.0f;
+                        p0 += bit ? 0.0f : dp;
+                        b[i] = bt * (bit ? e1 : e0);
+                        bs_ += b[i];
+                    }
+                }
+                xsum3(p0, p1, bs_);
+                if (fl && l == Lp - 1) bs_ = 1.0f;
+                if (A.mass_all) {
+                    const float inv = 1.0f / (p0 + p1);
+#pragma unroll
+                    for (int i = 0; i < MWL; i++) ms[i] = fmaf(dpv[i], inv, ms[i]);
+                }
+                if (!fl) { p0 /= e0; p1 /= e1; }
+                float o0 = p0 * ee + p1 * ed, o1 = p0 * ed + p1 * ee;
+                if (!isflat) { o0 *= rlf(m_h0, j); o1 *= rlf(m_h1, j); }
+                if (tid == 0) {
+                    const int site = rli(m_site, j);
+                    HP[2 * site] = o0 / (o0 + o1);
+                    HP[2 * site + 1] = o1 / (o0 + o1);
+                }
+                bsum_next = bs_;
+            }
+        }
+    }
+    if (A.mass_all) {
+        float *mo = A.mass_all + (size_t)c.state_off;
+#pragma unroll
+        for (int i = 0; i < MWL; i++)
+            if (valid[i]) mo[64 * wid[i] + lane] = ms[i];
     }
 }
 
@@ -177,25 +309,53 @@ __global__ void fb_mono_kernel(const FbItem *__restrict__ fb, const CondItem *__
     HP_all[f.hl_off + 2 * site + 1] = o[1] / (o[0] + o[1]);
 }
 
-template <int MW>
-static void fb_go(int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_poly, const float *d_t,
-                  const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, const uint8_t *d_flags,
-                  float *d_alpha, float *d_HP, float ee, float ed, hipStream_t s) {
-    hipLaunchKernelGGL(fb_kernel<MW>, dim3(n), dim3(64), 0, s, d_fb, d_cond, d_poly, d_t, d_hvar, d_HL, d_flat,
-                       d_flags, d_alpha, d_HP, ee, ed);
+// Launch shape: waves per item and words per lane by the largest conditioning set.
+struct FbShape {
+    int NW, MWL, C;
+    size_t smem;
+};
+static FbShape fb_shape(int maxW) {
+    FbShape s;
+    if (maxW <= 4) { s.NW = 1; s.MWL = 4; }
+    else if (maxW <= 8) { s.NW = 2; s.MWL = 4; }
+    else if (maxW <= 16) { s.NW = 4; s.MWL = 4; }
+    else { s.NW = 4; s.MWL = 8; }
+    const size_t rows = (size_t)FB_TILE * maxW * 8;
+    const size_t per_site = (size_t)s.NW * s.MWL * 64 * 4;
+    int C = 64;
+    while (C > 1 && rows + C * per_site + 2 * 4 * 4 * 4 > (size_t)FB_LDS_BYTES) C >>= 1;
+    s.C = C;
+    s.smem = rows + C * per_site + 2 * 4 * 4 * 4;
+    return s;
+}
+
+int fb_block_sites(int maxW) { return fb_shape(maxW).C; }
+
+template <int NW, int MWL>
+static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        GH_HIP(hipFuncSetAttribute((const void *)fb_kernel<NW, MWL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   FB_LDS_BYTES));
+        attr = true;
+    }
+    hipLaunchKernelGGL((fb_kernel<NW, MWL>), dim3(n), dim3(64 * NW), smem, s, A);
 }
 
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
-               const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_alpha,
-               float *d_HP, float ee, float ed, hipStream_t s) {
+               const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
+               float *d_HP, float ee, float ed, hipStream_t s, float *d_mass) {
     if (n <= 0) return;
-    const uint8_t *fl = P->d_flags.as<uint8_t>();
-    if (maxW <= 2) fb_go<2>(n, d_fb, d_cond, d_poly, d_t, d_hvar, d_HL, d_flat, fl, d_alpha, d_HP, ee, ed, s);
-    else if (maxW <= 4) fb_go<4>(n, d_fb, d_cond, d_poly, d_t, d_hvar, d_HL, d_flat, fl, d_alpha, d_HP, ee, ed, s);
-    else if (maxW <= 8) fb_go<8>(n, d_fb, d_cond, d_poly, d_t, d_hvar, d_HL, d_flat, fl, d_alpha, d_HP, ee, ed, s);
-    else if (maxW <= 16) fb_go<16>(n, d_fb, d_cond, d_poly, d_t, d_hvar, d_HL, d_flat, fl, d_alpha, d_HP, ee, ed, s);
-    else if (maxW <= 32) fb_go<32>(n, d_fb, d_cond, d_poly, d_t, d_hvar, d_HL, d_flat, fl, d_alpha, d_HP, ee, ed, s);
-    else throw Error(GH_ERR_ARG, "fb: more than 2048 conditioning states per haplotype is not supported");
+    GH_CHECK(maxW <= 32, GH_ERR_ARG, "fb: more than 2048 conditioning states per haplotype is not supported");
+    const FbShape sh = fb_shape(maxW);
+    FbArgs A;
+    A.fb = d_fb; A.cond = d_cond; A.poly_all = d_poly; A.t_all = d_t; A.hvar_all = d_hvar; A.HL_all = d_HL;
+    A.flat_all = d_flat; A.flags = P->d_flags.as<uint8_t>(); A.ckpt_all = d_ckpt; A.HP_all = d_HP;
+    A.mass_all = d_mass; A.ee = ee; A.ed = ed; A.C = sh.C; A.maxW = maxW;
+    if (sh.NW == 1) fb_go<1, 4>(A, n, sh.smem, s);
+    else if (sh.NW == 2) fb_go<2, 4>(A, n, sh.smem, s);
+    else if (sh.MWL == 4) fb_go<4, 4>(A, n, sh.smem, s);
+    else fb_go<4, 8>(A, n, sh.smem, s);
     GH_HIP(hipGetLastError());
 }
 

@@ -151,6 +151,8 @@ __device__ __forceinline__ float transition(const double *cm, int prev, int next
     return (float)x;
 }
 
+constexpr int PH_TILE = 64;
+
 template <int KPT>
 __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__ items, const CondItem *__restrict__ cond,
                                                      const uint64_t *__restrict__ hvar_all,
@@ -163,10 +165,15 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
                                                      uint8_t *__restrict__ H0_all, uint8_t *__restrict__ H1_all,
                                                      int L, float ee, float ed, uint32_t seed, uint32_t iter,
                                                      int32_t *__restrict__ status) {
-    __shared__ float lds[32];
+    extern __shared__ __align__(16) uint64_t s_rows[];   // [PH_TILE][W] Hvar rows of the current tile
+    __shared__ float lds8[2][32];
     __shared__ float red64[4][64];
     __shared__ float hprobs[64];
     __shared__ int dip_sh;
+    __shared__ int s_rel[PH_TILE], s_seg[PH_TILE], s_miss[PH_TILE];
+    __shared__ signed char s_type[PH_TILE];
+    __shared__ unsigned char s_alt[PH_TILE], s_first[PH_TILE], s_last[PH_TILE];
+    __shared__ float s_yin[PH_TILE], s_yout[PH_TILE];
     const PhItem it = items[blockIdx.x];
     const CondItem c = cond[it.cond];
     const int K = c.K, W = c.W, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -193,30 +200,64 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
     float sumK[KPT];
     float sumH[8];
     float sumT = 1.0f;
+    int par = 0;
 #pragma unroll
     for (int i = 0; i < KPT; i++) sumK[i] = 0.0f;
 
-    auto bit_of = [&](int rel, int i) -> bool {
-        const int k = kk[i];
-        if (!valid[i]) return false;
-        return (hv[(size_t)rel * W + (k >> 6)] >> (k & 63)) & 1ull;
+    // per-hypothesis sums over the workgroup, one barrier (LDS slot alternates)
+    auto bsum8 = [&](float v[8]) {
+#pragma unroll
+        for (int h = 0; h < 8; h++) v[h] = wave_sum(v[h]);
+        par ^= 1;
+        if (lane == 0) {
+#pragma unroll
+            for (int h = 0; h < 8; h++) lds8[par][wid * 8 + h] = v[h];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 8; h++) v[h] = (lds8[par][h] + lds8[par][8 + h]) + (lds8[par][16 + h] + lds8[par][24 + h]);
     };
 
-    // One locus update (INIT / RUN / COLLAPSE of the three kinds).
-    auto step = [&](const Locus &Lc, int mode, float yt) {
-        const float nt = 1.0f - yt;
-        float tF[8];
-        float NT = 0.0f;
-        if (mode != 0) {
-            const float sc = yt / (Kf * sumT);
-#pragma unroll
-            for (int h = 0; h < 8; h++) tF[h] = sumH[h] * sc;
-            NT = nt / sumT;
+    // tile of loci [q0, q0+n): scalars and Hvar rows into LDS
+    auto load_tile = [&](int q0, int n) {
+        __syncthreads();
+        if (tid < n) {
+            const int q = q0 + tid;
+            const Locus Lc = loci[q];
+            s_rel[tid] = Lc.rel;
+            s_seg[tid] = Lc.seg;
+            s_miss[tid] = Lc.miss;
+            s_type[tid] = Lc.type;
+            s_alt[tid] = Lc.alt;
+            const bool has_prev = q > 0, has_next = q < nloc - 1;
+            const Locus Lp_ = has_prev ? loci[q - 1] : Lc;
+            const Locus Ln_ = has_next ? loci[q + 1] : Lc;
+            s_yin[tid] = has_prev ? transition(cm, Lp_.abs, Lc.abs, nrho) : 0.0f;
+            s_yout[tid] = has_next ? transition(cm, Lc.abs, Ln_.abs, nrho) : 0.0f;
+            s_first[tid] = !has_prev || Lp_.seg != Lc.seg;
+            s_last[tid] = !has_next || Ln_.seg != Lc.seg;
         }
-        float acc[8];
-#pragma unroll
-        for (int h = 0; h < 8; h++) acc[h] = 0.0f;
-        if (Lc.type == -2 && mode == 0) {
+        __syncthreads();
+        for (int e = tid; e < n * W; e += PH_NT) {
+            const int r = e / W, w = e - r * W;
+            s_rows[e] = hv[(size_t)s_rel[r] * W + w];
+        }
+        __syncthreads();
+    };
+    auto bit_of = [&](int j, int i) -> bool {
+        const int k = kk[i];
+        return (s_rows[j * W + (k >> 6)] >> (k & 63)) & 1ull;
+    };
+
+    // One locus update (INIT / RUN / COLLAPSE of the three kinds), tile offset j.
+    // Emissions are an 8-vector per allele for the locus; a state's update is
+    // base(h) * e_allele(h) with base = 1 (INIT) or fma(src, nt/S, tFreq(h)).
+    // Multiplying by exactly 1.0 where the reference skips the multiply keeps
+    // every value identical to the reference's per-kind kernels.
+    auto step = [&](int j, int mode, float yt) {
+        const int type = s_type[j];
+        const bool alt = s_alt[j];
+        if (type == -2 && mode == 0) {
             const float v = 1.0f / (8 * (unsigned)K);
 #pragma unroll
             for (int i = 0; i < KPT; i++)
@@ -227,31 +268,46 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
             sumT = 1.0f;
             return;
         }
+        float e0v[8], e1v[8];
+#pragma unroll
+        for (int h = 0; h < 8; h++) {
+            if (type >= 0) {
+                const bool hyp0 = (h >> (2 - type)) & 1;   // hypothesis carries allele 0 at this het
+                e0v[h] = hyp0 ? ee : ed;
+                e1v[h] = hyp0 ? ed : ee;
+            } else if (type == -1) {
+                e0v[h] = alt ? mism : 1.0f;
+                e1v[h] = alt ? 1.0f : mism;
+            } else {
+                e0v[h] = 1.0f;
+                e1v[h] = 1.0f;
+            }
+        }
+        const float nt = 1.0f - yt;
+        float tF[8];
+        float NT = 0.0f;
+        const float sc = (mode != 0) ? yt / (Kf * sumT) : 0.0f;
+        if (mode != 0) NT = nt / sumT;
+#pragma unroll
+        for (int h = 0; h < 8; h++) tF[h] = sumH[h] * sc;
+        const bool run = mode == 1, init = mode == 0;
+        float acc[8];
+#pragma unroll
+        for (int h = 0; h < 8; h++) acc[h] = 0.0f;
 #pragma unroll
         for (int i = 0; i < KPT; i++) {
-            if (!valid[i]) {
-#pragma unroll
-                for (int h = 0; h < 8; h++) p[i][h] = 0.0f;
-                continue;
-            }
-            const bool ah = bit_of(Lc.rel, i);
+            const bool ah = valid[i] && bit_of(j, i);
 #pragma unroll
             for (int h = 0; h < 8; h++) {
-                float x;
-                if (mode == 0) x = 1.0f;
-                else if (mode == 1) x = fmaf(p[i][h], NT, tF[h]);
-                else x = fmaf(sumK[i], NT, tF[h]);
-                if (Lc.type >= 0) {
-                    const float e = emit_het(Lc.type, h, ah, ee, ed);
-                    x = (mode == 0) ? e : x * e;
-                } else if (Lc.type == -1) {
-                    if (ah != (bool)Lc.alt) x = (mode == 0) ? mism : x * mism;
-                }
+                const float src = run ? p[i][h] : sumK[i];
+                const float base = init ? 1.0f : fmaf(src, NT, tF[h]);
+                float x = base * (ah ? e1v[h] : e0v[h]);
+                x = valid[i] ? x : 0.0f;
                 p[i][h] = x;
                 acc[h] += x;
             }
         }
-        block_sum8(acc, lds);
+        bsum8(acc);
 #pragma unroll
         for (int h = 0; h < 8; h++) sumH[h] = acc[h];
         sumT = hadd8(sumH);
@@ -272,21 +328,22 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
     };
 
     // ---------------- backward (phasing_hmm.cpp:193-245)
-    {
-        int seg = it.n_segs - 1;
-        for (int q = nloc - 1; q >= 0; q--) {
-            const Locus Lc = loci[q];
-            const bool last = q == nloc - 1;
-            const float yt = last ? 0.0f : transition(cm, Lc.abs, loci[q + 1].abs, nrho);
-            const bool seg_last = last || loci[q + 1].seg != Lc.seg;   // last locus of its segment
-            const bool seg_first = q == 0 || loci[q - 1].seg != Lc.seg;
-            step(Lc, last ? 0 : (seg_last ? 2 : 1), yt);
-            if (seg_first) {
-                sumk();
-                store_msg(segp + (size_t)Lc.seg * KH, segs + Lc.seg * 9, segs + Lc.seg * 9 + 8);
+    if (nloc > 0) {
+        const int last_tile = ((nloc - 1) / PH_TILE) * PH_TILE;
+        for (int q0 = last_tile; q0 >= 0; q0 -= PH_TILE) {
+            const int n = min(PH_TILE, nloc - q0);
+            load_tile(q0, n);
+            for (int j = n - 1; j >= 0; j--) {
+                const int q = q0 + j;
+                const bool last = q == nloc - 1;
+                step(j, last ? 0 : (s_last[j] ? 2 : 1), s_yout[j]);
+                if (s_first[j]) {
+                    sumk();
+                    const int sg = s_seg[j];
+                    store_msg(segp + (size_t)sg * KH, segs + sg * 9, segs + sg * 9 + 8);
+                }
+                if (s_type[j] == -2) store_msg(impp + (size_t)s_miss[j] * KH, imps + s_miss[j] * 8, nullptr);
             }
-            if (Lc.type == -2) store_msg(impp + (size_t)Lc.miss * KH, imps + Lc.miss * 8, nullptr);
-            (void)seg;
         }
     }
     __syncthreads();
@@ -298,7 +355,6 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
             for (int d = 0; d < 8; d++) { D[d] = (segs[d] / s0) * (segs[7 - d] / s0); S += D[d]; }
         } else {
             for (int d = 0; d < 8; d++) D[d] = 0.0f;
-            S = 0.0f;
         }
         const float u = u01(philox_draw(seed, iter, P_DIP_ROOT, it.tgt, 0u)) * S;
         float cs = D[0];
@@ -313,99 +369,104 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
     __syncthreads();
     int cur_dip = dip_sh;
     // ---------------- forward (phasing_hmm.cpp:142-191)
-    for (int q = 0; q < nloc; q++) {
-        const Locus Lc = loci[q];
-        const float yt = q == 0 ? 0.0f : transition(cm, loci[q - 1].abs, Lc.abs, nrho);
-        const bool seg_first = q == 0 || loci[q - 1].seg != Lc.seg;
-        const bool seg_last = q == nloc - 1 || loci[q + 1].seg != Lc.seg;
-        step(Lc, q == 0 ? 0 : (seg_first ? 2 : 1), yt);
-        if (seg_last) sumk();
-        if (seg_last && q != nloc - 1) {
-            // TRANS_HAP
-            const float yt2 = transition(cm, Lc.abs, loci[q + 1].abs, nrho);
-            const float nt2 = 1.0f - yt2;
-            const float F2 = nt2 / sumT;
-            const float *nxt = segp + (size_t)(Lc.seg + 1) * KH;
-            float part[64];
+    for (int q0 = 0; q0 < nloc; q0 += PH_TILE) {
+        const int n = min(PH_TILE, nloc - q0);
+        load_tile(q0, n);
+        for (int j = 0; j < n; j++) {
+            const int q = q0 + j;
+            step(j, q == 0 ? 0 : (s_first[j] ? 2 : 1), s_yin[j]);
+            const bool seg_last = s_last[j];
+            if (seg_last) sumk();
+            if (seg_last && q != nloc - 1) {
+                // TRANS_HAP (phasing_hmm.h:351-373)
+                const float yt2 = s_yout[j];
+                const float nt2 = 1.0f - yt2;
+                const float F2 = nt2 / sumT;
+                const float *nxt = segp + (size_t)(s_seg[j] + 1) * KH;
+                float part[64];
 #pragma unroll
-            for (int e = 0; e < 64; e++) part[e] = 0.0f;
+                for (int e = 0; e < 64; e++) part[e] = 0.0f;
 #pragma unroll
-            for (int i = 0; i < KPT; i++) {
-                if (!valid[i]) continue;
-                const float4 *n4 = (const float4 *)(nxt + ((size_t)i * PH_NT + tid) * 8);
-                const float4 a = n4[0], b = n4[1];
-                const float nv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+                for (int i = 0; i < KPT; i++) {
+                    const float4 *n4 = (const float4 *)(nxt + ((size_t)i * PH_NT + tid) * 8);
+                    float4 a = n4[0], b = n4[1];
+                    if (!valid[i]) { a = make_float4(0, 0, 0, 0); b = a; }
+                    const float nv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-                for (int h1 = 0; h1 < 8; h1++) {
-                    const float F1 = (sumH[h1] / sumT) * yt2 / Kf;
-                    const float pv = fmaf(p[i][h1], F2, F1);
+                    for (int h1 = 0; h1 < 8; h1++) {
+                        const float F1 = (sumH[h1] / sumT) * yt2 / Kf;
+                        const float pv = fmaf(p[i][h1], F2, F1);
 #pragma unroll
-                    for (int h2 = 0; h2 < 8; h2++) part[h1 * 8 + h2] += pv * nv[h2];
-                }
-            }
-#pragma unroll
-            for (int e = 0; e < 64; e++) {
-                const float s = wave_sum(part[e]);
-                if (lane == 0) red64[wid][e] = s;
-            }
-            __syncthreads();
-            if (tid < 64) hprobs[tid] = (red64[0][tid] + red64[1][tid]) + (red64[2][tid] + red64[3][tid]);
-            __syncthreads();
-            if (tid == 0) {
-                float sumHP = 0.0f;
-                for (int h1 = 0; h1 < 8; h1++) sumHP += hadd8(&hprobs[h1 * 8]);
-                float D[8], S = 0.0f;
-                const int p0 = cur_dip, p1 = 7 - cur_dip;
-                for (int d = 0; d < 8; d++) {
-                    D[d] = (hprobs[p0 * 8 + d] / sumHP) * (hprobs[p1 * 8 + (7 - d)] / sumHP);
-                    S += D[d];
-                }
-                int r = 7;
-                if (!(S >= 1.17549435e-38f) || !(sumHP >= 1.17549435e-38f) || isinf(S) || isinf(sumHP)) {
-                    atomicExch(status, 1);
-                } else {
-                    const float u = u01(philox_draw(seed, iter, P_DIP_SEG, it.tgt, (uint32_t)(Lc.seg + 1))) * S;
-                    float cs = D[0];
-                    for (int i = 0; i < 7; i++) {
-                        if (u <= cs) { r = i; break; }
-                        cs += D[i + 1];
+                        for (int h2 = 0; h2 < 8; h2++) part[h1 * 8 + h2] += pv * nv[h2];
                     }
                 }
-                dip[Lc.seg + 1] = r;
-                dip_sh = r;
-            }
-            __syncthreads();
-            cur_dip = dip_sh;
-        }
-        if (Lc.type == -2) {
-            // IMPUTE_FLAT_HET (phasing_hmm.h:389-423)
-            const float *ip = impp + (size_t)Lc.miss * KH;
-            float sR[8], sL[8], s0[8], s1[8];
 #pragma unroll
-            for (int h = 0; h < 8; h++) {
-                sR[h] = 1.0f / imps[Lc.miss * 8 + h];
-                sL[h] = 1.0f / sumH[h];
-                s0[h] = 0.0f;
-                s1[h] = 0.0f;
+                for (int e = 0; e < 64; e++) {
+                    const float sm = wave_sum(part[e]);
+                    if (lane == 0) red64[wid][e] = sm;
+                }
+                __syncthreads();
+                if (tid < 64) hprobs[tid] = (red64[0][tid] + red64[1][tid]) + (red64[2][tid] + red64[3][tid]);
+                __syncthreads();
+                if (tid == 0) {
+                    float sumHP = 0.0f;
+                    for (int h1 = 0; h1 < 8; h1++) sumHP += hadd8(&hprobs[h1 * 8]);
+                    float D[8], S = 0.0f;
+                    const int d0 = cur_dip, d1 = 7 - cur_dip;
+                    for (int d = 0; d < 8; d++) {
+                        D[d] = (hprobs[d0 * 8 + d] / sumHP) * (hprobs[d1 * 8 + (7 - d)] / sumHP);
+                        S += D[d];
+                    }
+                    int r = 7;
+                    if (!(S >= 1.17549435e-38f) || !(sumHP >= 1.17549435e-38f) || isinf(S) || isinf(sumHP)) {
+                        atomicExch(status, 1);
+                    } else {
+                        const float u =
+                            u01(philox_draw(seed, iter, P_DIP_SEG, it.tgt, (uint32_t)(s_seg[j] + 1))) * S;
+                        float cs = D[0];
+                        for (int i = 0; i < 7; i++) {
+                            if (u <= cs) { r = i; break; }
+                            cs += D[i + 1];
+                        }
+                    }
+                    dip[s_seg[j] + 1] = r;
+                    dip_sh = r;
+                }
+                __syncthreads();
+                cur_dip = dip_sh;
             }
-#pragma unroll
-            for (int i = 0; i < KPT; i++) {
-                if (!valid[i]) continue;
-                const bool ah = bit_of(Lc.rel, i);
-                const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
-                const float4 a = i4[0], b = i4[1];
-                const float iv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            if (s_type[j] == -2) {
+                // IMPUTE_FLAT_HET (phasing_hmm.h:389-423)
+                const int mi = s_miss[j];
+                const float *ip = impp + (size_t)mi * KH;
+                float sR[8], sL[8], s0[8], s1[8];
 #pragma unroll
                 for (int h = 0; h < 8; h++) {
-                    const float v = (iv[h] * sR[h]) * (p[i][h] * sL[h]);
-                    if (ah) s1[h] += v; else s0[h] += v;
+                    sR[h] = 1.0f / imps[mi * 8 + h];
+                    sL[h] = 1.0f / sumH[h];
+                    s0[h] = 0.0f;
+                    s1[h] = 0.0f;
                 }
-            }
-            block_sum8(s0, lds);
-            block_sum8(s1, lds);
-            if (tid < 8) {
-                const float n = s1[tid] / (s0[tid] + s1[tid]);
-                of1[Lc.miss * 8 + tid] = fmaxf(0.0f, fminf(1.0f, n));
+#pragma unroll
+                for (int i = 0; i < KPT; i++) {
+                    const bool ah = valid[i] && bit_of(j, i);
+                    const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
+                    const float4 a = i4[0], b = i4[1];
+                    const float iv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+                    for (int h = 0; h < 8; h++) {
+                        float v = (iv[h] * sR[h]) * (p[i][h] * sL[h]);
+                        v = valid[i] ? v : 0.0f;
+                        s1[h] += ah ? v : 0.0f;
+                        s0[h] += ah ? 0.0f : v;
+                    }
+                }
+                bsum8(s0);
+                bsum8(s1);
+                if (tid < 8) {
+                    const float nn = s1[tid] / (s0[tid] + s1[tid]);
+                    of1[mi * 8 + tid] = fmaxf(0.0f, fminf(1.0f, nn));
+                }
             }
         }
     }
@@ -457,7 +518,8 @@ static void phase_go(gh_panel *P, int n, const PhItem *d_items, const CondItem *
                      const Locus *d_loci, const int32_t *d_pos, double nrho, PhaseBufs &B, uint8_t *d_H0,
                      uint8_t *d_H1, float ee, float ed, uint32_t seed, uint32_t iter, int32_t *d_status,
                      hipStream_t s) {
-    hipLaunchKernelGGL(phase_kernel<KPT>, dim3(n), dim3(PH_NT), 0, s, d_items, d_cond, d_hvar, d_loci, d_pos,
+    const size_t smem = (size_t)PH_TILE * ((KPT * PH_NT + 63) / 64) * 8;
+    hipLaunchKernelGGL(phase_kernel<KPT>, dim3(n), dim3(PH_NT), smem, s, d_items, d_cond, d_hvar, d_loci, d_pos,
                        P->d_cm.as<double>(), nrho, B.seg, B.segsum, B.imp, B.impsum, B.of1, B.dip, d_H0, d_H1,
                        P->L, ee, ed, seed, iter, d_status);
 }
