@@ -121,7 +121,8 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
 
     // tile metadata: lane j of every wave describes site l0 + j
     int m_site = 0, m_code = 0;
-    float m_e0 = 1.0f, m_e1 = 1.0f, m_h0 = 0.5f, m_h1 = 0.5f, m_tin = 0.0f, m_tout = 0.0f;
+    float m_e0 = 1.0f, m_e1 = 1.0f, m_h0 = 0.5f, m_h1 = 0.5f;
+    float m_f1in = 0.0f, m_ntin = 1.0f, m_f1out = 0.0f, m_ntout = 1.0f;   // t/K and 1-t into / out of the site
     auto load_tile = [&](int l0, int n) {
         __syncthreads();   // previous tile's LDS rows are no longer read
         if (lane < n) {
@@ -137,8 +138,11 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
             m_e1 = p1 / (p0 + p1);
             m_h0 = h0;
             m_h1 = h1;
-            m_tin = l > 0 ? tr[l - 1] : 0.0f;
-            m_tout = l < Lp - 1 ? tr[l] : 0.0f;
+            const float tin = l > 0 ? tr[l - 1] : 0.0f, tout = l < Lp - 1 ? tr[l] : 0.0f;
+            m_f1in = tin / Kf;
+            m_ntin = 1.0f - tin;
+            m_f1out = tout / Kf;
+            m_ntout = 1.0f - tout;
         }
         const uint64_t *src = hv + (size_t)l0 * W;
         for (int e = tid; e < n * W; e += 64 * NW) rows[e] = src[e];
@@ -162,8 +166,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
             }
             sp = fl ? 1.0f : xsum1(s);
         } else {
-            const float tt = rlf(m_tin, j);
-            const float f1 = tt / Kf, f2 = (1.0f - tt) / sp;
+            const float f1 = rlf(m_f1in, j), f2 = rlf(m_ntin, j) * __builtin_amdgcn_rcpf(sp);
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
                 const bool b = (rows[j * W + wid[i]] >> lane) & 1ull;
@@ -245,8 +248,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                         bs_ += b[i];
                     }
                 } else {
-                    const float tt = rlf(m_tout, j);
-                    const float f1 = tt / Kf, f2 = (1.0f - tt) / bsum_next;
+                    const float f1 = rlf(m_f1out, j), f2 = rlf(m_ntout, j) * __builtin_amdgcn_rcpf(bsum_next);
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
                         const bool bit = (rows[j * W + wid[i]] >> lane) & 1ull;

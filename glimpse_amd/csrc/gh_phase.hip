@@ -187,7 +187,7 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
     float *of1 = of1_all + (size_t)it.impsum_off;
     int32_t *dip = dip_all + it.dip_off;
     uint8_t *H0 = H0_all + it.h0_off, *H1 = H1_all + it.h1_off;
-    const float Kf = (float)K;
+    const float Kf = (float)K, invK = 1.0f / Kf;
     const float mism = ed / ee;
     const size_t KH = (size_t)PH_NT * KPT * 8;  // stored message stride (padded states)
 
@@ -285,24 +285,34 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
         }
         const float nt = 1.0f - yt;
         float tF[8];
-        float NT = 0.0f;
-        const float sc = (mode != 0) ? yt / (Kf * sumT) : 0.0f;
-        if (mode != 0) NT = nt / sumT;
+        // one reciprocal of the running total serves both scale factors
+        const float rS = __builtin_amdgcn_rcpf(sumT);
+        const float NT = (mode != 0) ? nt * rS : 0.0f;
+        const float sc = (mode != 0) ? yt * rS * invK : 0.0f;
 #pragma unroll
         for (int h = 0; h < 8; h++) tF[h] = sumH[h] * sc;
         const bool run = mode == 1, init = mode == 0;
         float acc[8];
 #pragma unroll
         for (int h = 0; h < 8; h++) acc[h] = 0.0f;
+        // states tid + 256*i: a wave covers 64 consecutive states, so a wave whose
+        // first state is >= K skips the block entirely (uniform branch)
 #pragma unroll
         for (int i = 0; i < KPT; i++) {
+            if (PH_NT * i + 64 * wid >= K) {
+#pragma unroll
+                for (int h = 0; h < 8; h++) p[i][h] = 0.0f;
+                continue;
+            }
             const bool ah = valid[i] && bit_of(j, i);
+            const float vm = valid[i] ? 1.0f : 0.0f;
+            const float sk = sumK[i];
 #pragma unroll
             for (int h = 0; h < 8; h++) {
-                const float src = run ? p[i][h] : sumK[i];
+                const float e = (ah ? e1v[h] : e0v[h]) * vm;
+                const float src = run ? p[i][h] : sk;
                 const float base = init ? 1.0f : fmaf(src, NT, tF[h]);
-                float x = base * (ah ? e1v[h] : e0v[h]);
-                x = valid[i] ? x : 0.0f;
+                const float x = base * e;
                 p[i][h] = x;
                 acc[h] += x;
             }
@@ -369,11 +379,31 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
     __syncthreads();
     int cur_dip = dip_sh;
     // ---------------- forward (phasing_hmm.cpp:142-191)
+    // the stored backward message of the next segment is fetched into registers
+    // when a segment starts, so TRANS_HAP at the segment's end does not wait on HBM
+    float nxtv[KPT][8];
+    auto fetch_next = [&](int seg) {
+        if (seg + 1 >= it.n_segs) return;
+        const float *nxt = segp + (size_t)(seg + 1) * KH;
+#pragma unroll
+        for (int i = 0; i < KPT; i++) {
+            const float4 *n4 = (const float4 *)(nxt + ((size_t)i * PH_NT + tid) * 8);
+            float4 a = make_float4(0, 0, 0, 0), b = a;
+            if (valid[i]) { a = n4[0]; b = n4[1]; }
+            nxtv[i][0] = a.x; nxtv[i][1] = a.y; nxtv[i][2] = a.z; nxtv[i][3] = a.w;
+            nxtv[i][4] = b.x; nxtv[i][5] = b.y; nxtv[i][6] = b.z; nxtv[i][7] = b.w;
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < KPT; i++)
+#pragma unroll
+        for (int h = 0; h < 8; h++) nxtv[i][h] = 0.0f;
     for (int q0 = 0; q0 < nloc; q0 += PH_TILE) {
         const int n = min(PH_TILE, nloc - q0);
         load_tile(q0, n);
         for (int j = 0; j < n; j++) {
             const int q = q0 + j;
+            if (s_first[j]) fetch_next(s_seg[j]);
             step(j, q == 0 ? 0 : (s_first[j] ? 2 : 1), s_yin[j]);
             const bool seg_last = s_last[j];
             if (seg_last) sumk();
@@ -382,16 +412,12 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
                 const float yt2 = s_yout[j];
                 const float nt2 = 1.0f - yt2;
                 const float F2 = nt2 / sumT;
-                const float *nxt = segp + (size_t)(s_seg[j] + 1) * KH;
                 float part[64];
 #pragma unroll
                 for (int e = 0; e < 64; e++) part[e] = 0.0f;
 #pragma unroll
                 for (int i = 0; i < KPT; i++) {
-                    const float4 *n4 = (const float4 *)(nxt + ((size_t)i * PH_NT + tid) * 8);
-                    float4 a = n4[0], b = n4[1];
-                    if (!valid[i]) { a = make_float4(0, 0, 0, 0); b = a; }
-                    const float nv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+                    const float *nv = nxtv[i];
 #pragma unroll
                     for (int h1 = 0; h1 < 8; h1++) {
                         const float F1 = (sumH[h1] / sumT) * yt2 / Kf;
