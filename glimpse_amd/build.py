@@ -15,10 +15,11 @@ ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = [
     "gh_capi.hip", "gh_panel.hip", "gh_condset.hip", "gh_impute.hip", "gh_phase.hip", "gh_pbwt.hip", "gh_gibbs.hip",
+    "gh_mlp.hip", "gh_select.hip",
 ]
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
 ]
@@ -37,13 +38,23 @@ def build_synth():
 
 
 def build_hip():
+    """One object per source (compiled in parallel), then one shared library."""
     out = os.path.join(HERE, "libglimpse_hip.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    srcs = [s for s in srcs if os.path.exists(s)]
-    if not srcs:
-        return None
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", out] + srcs)
+    jobs, objs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", "-o", obj,
+                                       os.path.join(CSRC, src)]
+        print("[build]", " ".join(cmd), flush=True)
+        jobs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+    bad = [src for src, j in zip(HIP_SOURCES, jobs) if j.wait() != 0]
+    if bad:
+        raise RuntimeError("hipcc failed on " + ", ".join(bad))
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs)
     return out
 
 

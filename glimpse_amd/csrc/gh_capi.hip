@@ -7,6 +7,8 @@
 
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
+#include "gh_mlp.hpp"
+#include "gh_select.hpp"
 #include "gh_phase.hpp"
 #include "gh_pbwt.hpp"
 
@@ -22,6 +24,10 @@ void run_execute(gh_run *G);
 void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
                int32_t *nstates_h, gh_run_stats *stats);
 void run_destroy(gh_run *G);
+void run_collect(gh_run *G, int from_iter);
+void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w);
+void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss);
+void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -388,6 +394,50 @@ int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, u
 }
 int gh_run_destroy(gh_run *r) {
     return guard([&] { run_destroy(r); });
+}
+
+int gh_mlp_create(uint32_t seed, gh_mlp **out) {
+    return guard([&] { mlp_create(seed, out); });
+}
+int gh_mlp_destroy(gh_mlp *m) {
+    return guard([&] { mlp_destroy(m); });
+}
+int gh_mlp_get_params(gh_mlp *m, float *w) {
+    return guard([&] { mlp_get(m, w); });
+}
+int gh_mlp_set_params(gh_mlp *m, const float *w) {
+    return guard([&] { mlp_set(m, w); });
+}
+int gh_mlp_forward(gh_mlp *m, int64_t n, const float *X, float *y) {
+    return guard([&] { mlp_forward_host(m, n, X, y); });
+}
+int gh_mlp_backward(gh_mlp *m, int32_t n, const float *X, const float *t, const float *w, float *grad,
+                    float *loss) {
+    return guard([&] { mlp_backward_host(m, n, X, t, w, grad, loss); });
+}
+int gh_mlp_step(gh_mlp *m, int32_t n, const float *X, const float *t, const float *w, float lr, float *loss) {
+    return guard([&] { mlp_step_host(m, n, X, t, w, lr, loss); });
+}
+int gh_mlp_train_run(gh_mlp *m, gh_run *r, int32_t epochs, int32_t batch, float lr, uint32_t seed,
+                     float *loss_per_epoch) {
+    return guard([&] { run_train(m, r, epochs, batch, lr, seed, loss_per_epoch); });
+}
+int gh_run_collect(gh_run *r, int32_t from_iter) {
+    return guard([&] { run_collect(r, from_iter); });
+}
+int gh_run_collected(gh_run *r, int64_t *n_rows, float *X, float *y, float *w) {
+    return guard([&] { run_collected(r, n_rows, X, y, w); });
+}
+int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass) {
+    return guard([&] { run_set_selector(r, m, k_learn, min_mass); });
+}
+int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *gl, const uint8_t *flat,
+                       const int32_t *pool, const int32_t *K, int32_t kstride, int32_t depth, float *X) {
+    return guard([&] { select_features_batch(p, T, H, gl, flat, pool, K, kstride, depth, X); });
+}
+int gh_select_topk(int32_t T, const float *scores, const int32_t *pool, const int32_t *K, const int32_t *k_learn,
+                   int32_t kstride, float min_mass, int32_t *states, int32_t *K_out, float *coverage) {
+    return guard([&] { select_topk_batch(T, scores, pool, K, k_learn, kstride, min_mass, states, K_out, coverage); });
 }
 
 }  // extern "C"

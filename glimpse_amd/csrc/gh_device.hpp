@@ -75,7 +75,7 @@ __host__ __device__ __forceinline__ uint32_t urange(uint32_t x, uint32_t lo, uin
 
 enum Purpose {
     P_H0 = 0, P_H1 = 1, P_DIP_ROOT = 2, P_DIP_SEG = 3, P_FLAT_HET = 4,
-    P_MONO_HET = 5, P_PBWT_STORED = 6, P_PBWT_START = 7, P_INIT_SEL = 8
+    P_MONO_HET = 5, P_PBWT_STORED = 6, P_PBWT_START = 7, P_INIT_SEL = 8, P_MLP_INIT = 9, P_MLP_PERM = 10
 };
 
 // site flag bits in the resident panel

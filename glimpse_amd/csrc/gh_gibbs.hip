@@ -25,6 +25,7 @@
 #include "gh_internal.hpp"
 #include "gh_pbwt.hpp"
 #include "gh_phase.hpp"
+#include "gh_select.hpp"
 
 namespace gh {
 
@@ -232,6 +233,7 @@ struct EvPair {
 struct gh_run {
     gh_panel *P = nullptr;
     int T = 0, L = 0, N = 0, n_iter = 0, kinit = 0, kstride = 0, maxW = 0, npt = 0;
+    int pstride = 1;   // selector rows per PBWT target: the largest Kpbwt
     gh_params prm;
     float ee_i = 0, ed_i = 0;
     float unph[256];
@@ -250,6 +252,13 @@ struct gh_run {
     gh::PbwtRun R;
     gh_run_stats st;
     bool done = false;
+    // learned selection / training-row collection
+    gh_mlp *mlp = nullptr;
+    float min_mass = 0.95f;
+    int collect_from = -1;
+    int64_t col_rows = 0;
+    bool masks_ready = false;
+    gh::DBuf d_klearn, d_ins, d_fx, d_fy, d_cov, d_cnt, d_tmask, d_mass0, d_mass1, d_cx, d_cy, d_cw;
 };
 
 namespace gh {
@@ -297,6 +306,7 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             G->hap_cls[2 * t] = G->hap_cls[2 * t + 1] = c;
             G->pbwt_targets.push_back(t);
         }
+        for (int t : G->pbwt_targets) G->pstride = std::max(G->pstride, G->kp[t]);
         int kmax_t = 0;
         G->kmax.resize(T);
         for (int t = 0; t < T; t++) {
@@ -420,7 +430,34 @@ void run_execute(gh_run *G) {
     int32_t *ptg = G->d_ptg.as<int32_t>();
     uint8_t *pst = G->d_pst.as<uint8_t>();
     int32_t *pstart = G->d_pstart.as<int32_t>();
+    const bool learned = G->mlp != nullptr, collecting = G->collect_from >= 1;
+    GH_CHECK(!(learned && collecting), GH_ERR_ARG, "a run either records training rows or uses the learned selector");
+    const int kst = G->kstride, prs = G->pstride;
+    if ((learned || collecting) && npt > 0) {
+        GH_CHECK(feature_lds_bytes(P) <= FEAT_LDS_MAX, GH_ERR_CAPACITY,
+                 "learned selection: chunk too large for the feature kernel");
+        GH_CHECK(N < (1 << 20), GH_ERR_ARG, "learned selection supports at most 2^20 panel haplotypes");
+        G->d_ins.reserve(sizeof(int32_t) * (size_t)T * kst);
+        if (!G->masks_ready) {
+            G->d_tmask.reserve(sizeof(uint64_t) * (size_t)T * 4 * std::max(P->Wc, 1));
+            target_mask_launch(P, T, gl_d, flat_d, G->d_tmask.as<uint64_t>(), s);
+            G->masks_ready = true;
+        }
+        G->d_cnt.reserve(sizeof(int32_t) * 2);
+        GH_HIP(hipMemsetAsync(G->d_cnt.p, 0, sizeof(int32_t) * 2, s));
+    }
+    if (learned) {
+        G->d_fx.reserve(sizeof(float) * (size_t)npt * prs * MLP_FEAT);
+        G->d_fy.reserve(sizeof(float) * (size_t)npt * prs);
+        G->d_cov.reserve(sizeof(float) * (size_t)T);
+    }
+    if (collecting) {
+        G->d_mass0.reserve(sizeof(float) * (size_t)T * kst);
+        G->d_mass1.reserve(sizeof(float) * (size_t)T * kst);
+    }
+    G->col_rows = 0;
     PbwtRun R = G->R;
+    R.ins = (learned || collecting) ? G->d_ins.as<int32_t>() : nullptr;
     R.haps = H;
     R.status = status + 1;
     R.out_states = states;
@@ -437,8 +474,8 @@ void run_execute(gh_run *G) {
     GH_HIP(hipMemsetAsync(status, 0, sizeof(int32_t) * 2, s));
     gh_run_stats st;
     std::memset(&st, 0, sizeof(st));
-    EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1;
-    e_sel.make(); e_cond.make(); e_fb.make(); e_ph.make(); e_k0.make(); e_k1.make();
+    EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1, e_learn;
+    e_sel.make(); e_cond.make(); e_fb.make(); e_ph.make(); e_k0.make(); e_k1.make(); e_learn.make();
     const dim3 gsite((L + 255) / 256, T), bsite(256);
     int main_done = 0;
     std::vector<CondItem> &items = G->items;
@@ -446,6 +483,7 @@ void run_execute(gh_run *G) {
 
     for (int it = 0; it < n_iter; it++) {
         const int stage = it == 0 ? 0 : (it <= prm->burnin ? 1 : 2);
+        bool learn_ev = false;
         GH_HIP(hipEventRecord(e_sel.a, s));
         if (stage == 0 || it == 1) {
             // host-side state sets: INIT selection, and full-panel targets (fixed iota)
@@ -491,28 +529,88 @@ void run_execute(gh_run *G) {
             hipLaunchKernelGGL(set_k_kernel, dim3((npt + 127) / 128), dim3(128), 0, s, items_d, Kdev, ptg, npt);
             GH_HIP(hipGetLastError());
             st.pbwt_hap_sites += (double)2 * npt * L;
+            const bool col_it = collecting && it >= G->collect_from;
+            if (learned || col_it) {
+                GH_HIP(hipEventRecord(e_learn.a, s));
+                FeatArgs fa;
+                fa.hm = P->d_hm.as<uint64_t>(); fa.Wc = P->Wc; fa.Lc = P->Lc; fa.N = N; fa.L = L;
+                fa.depth = prm->pbwt_depth; fa.kstride = kst; fa.rstride = prs;
+                fa.c2t = P->d_c2t.as<int32_t>(); fa.flags = P->d_flags.as<uint8_t>();
+                fa.rare_ptr = P->d_rare_ptr.as<int32_t>(); fa.rare_idx = P->d_rare_idx.as<int32_t>();
+                fa.H = H; fa.tmask = G->d_tmask.as<uint64_t>(); fa.targets = ptg; fa.Kpool = Kdev;
+                fa.ins = G->d_ins.as<int32_t>();
+                if (learned) {
+                    fa.X = G->d_fx.as<float>();
+                    fa.row0 = 0;
+                } else {
+                    const int64_t need = G->col_rows + (int64_t)npt * prs;
+                    if ((int64_t)(G->d_cx.bytes / (sizeof(float) * MLP_FEAT)) < need) {
+                        // grow the row store (rare: sized for all collected iterations up front)
+                        const int64_t cap = (int64_t)npt * prs * std::max(1, n_iter - G->collect_from);
+                        GH_CHECK(G->col_rows == 0, GH_ERR_CAPACITY, "training-row store overflow");
+                        G->d_cx.reserve(sizeof(float) * (size_t)cap * MLP_FEAT);
+                        G->d_cy.reserve(sizeof(float) * (size_t)cap);
+                        G->d_cw.reserve(sizeof(float) * (size_t)cap);
+                    }
+                    fa.X = G->d_cx.as<float>();
+                    fa.row0 = G->col_rows;
+                }
+                feature_launch(P, npt, fa, s);
+                if (learned) {
+                    mlp_forward_launch(G->mlp->w.as<float>(), G->mlp->wb.as<uint16_t>(), G->d_fx.as<float>(),
+                                       (int64_t)npt * prs, G->d_fy.as<float>(), s);
+                    TopkArgs ta;
+                    ta.targets = ptg; ta.Kpool = Kdev; ta.klearn = G->d_klearn.as<int32_t>();
+                    ta.y = G->d_fy.as<float>(); ta.row0 = 0; ta.kstride = kst; ta.rstride = prs;
+                    ta.min_mass = G->min_mass;
+                    ta.ins = G->d_ins.as<int32_t>(); ta.states = states; ta.Kdev = Kdev; ta.items = items_d;
+                    ta.coverage = G->d_cov.as<float>(); ta.counters = G->d_cnt.as<int32_t>();
+                    topk_launch(npt, ta, s);
+                }
+                GH_HIP(hipEventRecord(e_learn.b, s));
+                learn_ev = true;
+            }
+        }
+        int maxW_it = G->maxW;
+        if (learned && stage > 0) {
+            // the selection shrank the sets on the device: size this iteration's launches to them
+            GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
+            GH_HIP(hipStreamSynchronize(s));
+            maxW_it = 1;
+            for (int t = 0; t < T; t++) maxW_it = std::max(maxW_it, items[t].W);
         }
         GH_HIP(hipEventRecord(e_sel.b, s));
         GH_HIP(hipEventRecord(e_cond.a, s));
-        condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, G->maxW, s);
+        condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, maxW_it, s);
+        const bool label_it = collecting && stage > 0 && npt > 0 && it >= G->collect_from;
+        float *mass0 = label_it ? G->d_mass0.as<float>() : nullptr, *mass1 = label_it ? G->d_mass1.as<float>() : nullptr;
         GH_HIP(hipEventRecord(e_cond.b, s));
         GH_HIP(hipEventRecord(e_fb.a, s));
         // H0 (caller_algorithm.cpp:58-65)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k0.a, s));
-        fb_launch(P, T, fb_d, items_d, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HP0, G->ee_i, G->ed_i, s);
+        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP0, G->ee_i, G->ed_i, s, mass0);
         GH_HIP(hipEventRecord(e_k0.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
         // H1 (caller_algorithm.cpp:66-71)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k1.a, s));
-        fb_launch(P, T, fb_d, items_d, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i, G->ed_i, s);
+        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i, G->ed_i, s, mass1);
         GH_HIP(hipEventRecord(e_k1.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
         GH_HIP(hipGetLastError());
         GH_HIP(hipEventRecord(e_fb.b, s));
+        if (label_it) {
+            LabelArgs la;
+            la.targets = ptg; la.items = items_d; la.states = states; la.ins = G->d_ins.as<int32_t>();
+            la.mass0 = mass0; la.mass1 = mass1; la.kstride = kst; la.rstride = prs;
+            la.X = G->d_cx.as<float>(); la.Y = G->d_cy.as<float>(); la.W = G->d_cw.as<float>();
+            la.row0 = G->col_rows;
+            label_launch(npt, la, s);
+            G->col_rows += (int64_t)npt * prs;
+        }
         // phasing (phasing_hmm::rephaseHaplotypes)
         GH_HIP(hipEventRecord(e_ph.a, s));
         PhItem *ph_d = dalloc_copy(G->d_ph, ph.data(), T, s);
@@ -564,7 +662,16 @@ void run_execute(gh_run *G) {
         st.ms_phase += e_ph.ms();
         st.fb_kernel_ms += e_k0.ms() + e_k1.ms();
         st.fb_launches += 2;
+        if (learn_ev) st.ms_learn += e_learn.ms();
     }
+    if ((learned || collecting) && npt > 0) {
+        int32_t cnt[2] = {0, 0};
+        GH_HIP(hipMemcpyAsync(cnt, G->d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+        st.learned_sel = cnt[0];
+        st.pool_kept = cnt[1];
+    }
+    st.collected_rows = (double)G->col_rows;
     hipLaunchKernelGGL(finalize_kernel, gsite, bsite, 0, s, gp0, gp1, sto, hds, G->d_h0.as<uint8_t>(),
                        G->d_h1.as<uint8_t>(), L, main_done);
     GH_HIP(hipGetLastError());
@@ -573,6 +680,52 @@ void run_execute(gh_run *G) {
     st.ms_other = st.ms_total - st.ms_select - st.ms_condset - st.ms_fb - st.ms_phase;
     G->st = st;
     G->done = true;
+}
+
+void run_collect(gh_run *G, int from_iter) {
+    GH_CHECK(G, GH_ERR_ARG, "gh_run_collect: null run");
+    GH_CHECK(from_iter >= 1 || from_iter < 0, GH_ERR_ARG, "gh_run_collect: rows come from PBWT iterations (>= 1)");
+    GH_CHECK(from_iter < 0 || !G->mlp, GH_ERR_ARG, "gh_run_collect: the run uses the learned selector");
+    G->collect_from = from_iter < 0 ? -1 : from_iter;
+    if (G->collect_from >= 1 && G->npt > 0) {
+        const int64_t cap = (int64_t)G->npt * G->pstride * std::max(1, G->n_iter - G->collect_from);
+        G->d_cx.reserve(sizeof(float) * (size_t)cap * MLP_FEAT);
+        G->d_cy.reserve(sizeof(float) * (size_t)cap);
+        G->d_cw.reserve(sizeof(float) * (size_t)cap);
+    }
+}
+
+void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w) {
+    GH_CHECK(G && n_rows, GH_ERR_ARG, "gh_run_collected: bad arguments");
+    *n_rows = G->col_rows;
+    hipStream_t s = G->P->stream;
+    const size_t n = (size_t)G->col_rows;
+    if (n == 0) return;
+    if (X) GH_HIP(hipMemcpyAsync(X, G->d_cx.p, sizeof(float) * n * MLP_FEAT, hipMemcpyDeviceToHost, s));
+    if (y) GH_HIP(hipMemcpyAsync(y, G->d_cy.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (w) GH_HIP(hipMemcpyAsync(w, G->d_cw.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    GH_HIP(hipStreamSynchronize(s));
+}
+
+void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss) {
+    GH_CHECK(M && G, GH_ERR_ARG, "gh_mlp_train_run: bad arguments");
+    GH_CHECK(G->col_rows > 0, GH_ERR_STATE, "gh_mlp_train_run: the run recorded no training rows");
+    GH_CHECK(G->mlp != M, GH_ERR_ARG, "gh_mlp_train_run: the network is this run's selector");
+    mlp_train_device(M, G->d_cx.as<float>(), G->d_cy.as<float>(), G->d_cw.as<float>(), G->col_rows, epochs, batch,
+                     lr, seed, G->P->stream, loss);
+}
+
+void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass) {
+    GH_CHECK(G, GH_ERR_ARG, "gh_run_set_selector: null run");
+    if (!M) { G->mlp = nullptr; return; }
+    GH_CHECK(klearn, GH_ERR_ARG, "gh_run_set_selector: k_learn is required");
+    GH_CHECK(G->collect_from < 1, GH_ERR_ARG, "gh_run_set_selector: the run records training rows");
+    GH_CHECK(min_mass >= 0.0f && min_mass <= 1.0f, GH_ERR_ARG, "gh_run_set_selector: min_mass must be in [0, 1]");
+    for (int t = 0; t < G->T; t++) GH_CHECK(klearn[t] > 0, GH_ERR_ARG, "gh_run_set_selector: k_learn must be > 0");
+    dalloc_copy(G->d_klearn, klearn, (size_t)G->T, G->P->stream);
+    GH_HIP(hipStreamSynchronize(G->P->stream));
+    G->mlp = M;
+    G->min_mass = min_mass;
 }
 
 void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,

@@ -485,6 +485,7 @@ struct CompactArgs {
     int32_t *out_states;        // [n_samples][out_stride]
     int out_stride;
     int32_t *out_K;             // [n_samples]
+    int32_t *ins;               // [n_samples][out_stride] or NULL
 };
 
 __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
@@ -498,25 +499,41 @@ __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
     const int32_t *eA = c.ev + (size_t)h0 * c.cap * rec, *eB = c.ev + (size_t)h1 * c.cap * rec;
     const int nA = c.ev_cnt[h0], nB = c.ev_cnt[h1];
     const int kp = c.kpbwt[t];
+    const int orow = c.sample_row ? c.sample_row[t] : t;
+    int32_t *ins = c.ins ? c.ins + (size_t)orow * c.out_stride : nullptr;
     if (lane == 0) {
         int count = 0;
         for (int o = 0; o < c.depth && count < kp; o++) {
             int ia = nA - 1, ib = nB - 1;
             while ((ia >= 0 || ib >= 0) && count < kp) {
                 const int32_t *r;
+                int fromB;
                 // newest first; at equal key the second haplotype pushed last
-                if (ib >= 0 && (ia < 0 || eB[(size_t)ib * rec] >= eA[(size_t)ia * rec])) r = eB + (size_t)ib-- * rec;
-                else r = eA + (size_t)ia-- * rec;
+                if (ib >= 0 && (ia < 0 || eB[(size_t)ib * rec] >= eA[(size_t)ia * rec])) {
+                    r = eB + (size_t)ib-- * rec;
+                    fromB = 1;
+                } else {
+                    r = eA + (size_t)ia-- * rec;
+                    fromB = 0;
+                }
                 const int nd = r[1] & 0xffff, nu = r[1] >> 16;
                 if (o < nu) {
                     const int x = r[2 + c.depth + o];
                     const uint64_t m = 1ull << (x & 63);
-                    if (!(bm[x >> 6] & m)) { bm[x >> 6] |= m; count++; }
+                    if (!(bm[x >> 6] & m)) {
+                        bm[x >> 6] |= m;
+                        if (ins) ins[count] = x | (o << 20) | (fromB << 26) | (1 << 27);
+                        count++;
+                    }
                 }
                 if (o < nd && count < kp) {
                     const int x = r[2 + o];
                     const uint64_t m = 1ull << (x & 63);
-                    if (!(bm[x >> 6] & m)) { bm[x >> 6] |= m; count++; }
+                    if (!(bm[x >> 6] & m)) {
+                        bm[x >> 6] |= m;
+                        if (ins) ins[count] = x | (o << 20) | (fromB << 26);
+                        count++;
+                    }
                 }
             }
         }
@@ -567,6 +584,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
         c.ev = R.ev; c.ev_cnt = R.ev_cnt; c.cap = R.cap; c.depth = R.depth; c.N = P->N;
         c.kpbwt = R.kpbwt; c.sample_hap = R.sample_hap; c.sample_row = R.sample_row; c.out_states = R.out_states; c.out_stride = R.out_stride;
         c.out_K = R.out_K;
+        c.ins = R.ins;
         const size_t shm = sizeof(uint64_t) * ((P->N + 63) / 64);
         hipLaunchKernelGGL(pbwt_compact_kernel, dim3(R.n_samples), dim3(64), shm, s, c);
         GH_HIP(hipGetLastError());

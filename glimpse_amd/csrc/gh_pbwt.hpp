@@ -44,6 +44,9 @@ struct PbwtRun {
     int32_t *out_states;        // [n_samples][out_stride]
     int out_stride;
     int32_t *out_K;             // [n_samples]
+    // optional: the same states in insertion order, id | rank << 20 | second
+    // haplotype << 26 | from the upper neighbours << 27  ([n_samples][out_stride])
+    int32_t *ins;
 };
 
 void build_sparse_pbwt(const gh_panel_desc *d, std::vector<uint8_t> &Y, std::vector<int32_t> &aptr,

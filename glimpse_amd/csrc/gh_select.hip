@@ -1,0 +1,466 @@
+// gh_select.hip -- learned conditioning-state selection (no counterpart in the
+// reference): per-candidate features of a target's PBWT pool, top-K by the
+// network's predicted posterior mass with a fallback to the whole pool, and the
+// training labels taken from the forward-backward posterior mass.
+//
+//   target_mask_kernel  once per run: per target, bit masks over the common
+//                       sites (words of 64 common rows, the transposed panel's
+//                       layout): has data, confident RR, AA, RA genotype.
+//   feature_kernel      one workgroup per target: packs the current H0/H1 at the
+//                       common sites into LDS (ballot), marks panel haplotypes
+//                       that carry a rare allele the target carries, then one
+//                       wave per candidate streams the candidate's transposed
+//                       panel row (coalesced 64-bit words) against them.
+//   topk_kernel         one workgroup per target: bitonic sort of the pool by
+//                       predicted score, mass coverage of the top k, then the
+//                       kept ids sorted ascending (the order the conditioning
+//                       set builder expects).
+//   label_kernel        log posterior mass of every pool candidate.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gh_device.hpp"
+#include "gh_internal.hpp"
+#include "gh_select.hpp"
+
+namespace gh {
+
+/* ------------------------------------------------------------------ masks */
+__global__ __launch_bounds__(64) void target_mask_kernel(const uint8_t *__restrict__ gl,
+                                                         const uint8_t *__restrict__ flat,
+                                                         const int32_t *__restrict__ c2t, int L, int Lc, int Wc,
+                                                         uint64_t *__restrict__ tmask) {
+    const int wi = blockIdx.x, t = blockIdx.y, lane = threadIdx.x;
+    const int c = wi * 64 + lane;
+    bool d = false, rr = false, aa = false, ra = false;
+    if (c < Lc) {
+        const int site = c2t[c];
+        const size_t o = (size_t)t * L + site;
+        if (!flat[o]) {
+            d = true;
+            const uint8_t *g = gl + 3 * o;   // phred-scaled, normalised to min 0
+            rr = g[0] == 0 && g[1] >= FEAT_CONF_PHRED && g[2] >= FEAT_CONF_PHRED;
+            ra = g[1] == 0 && g[0] >= FEAT_CONF_PHRED && g[2] >= FEAT_CONF_PHRED;
+            aa = g[2] == 0 && g[0] >= FEAT_CONF_PHRED && g[1] >= FEAT_CONF_PHRED;
+        }
+    }
+    const uint64_t md = __ballot(d), mr = __ballot(rr), ma = __ballot(aa), mh = __ballot(ra);
+    if (lane == 0) {
+        uint64_t *m = tmask + (size_t)t * 4 * Wc;
+        m[wi] = md;
+        m[Wc + wi] = mr;
+        m[2 * Wc + wi] = ma;
+        m[3 * Wc + wi] = mh;
+    }
+}
+
+/* ------------------------------------------------------------------ features */
+// Longest run of set bits in z (z != all ones is not required).
+__device__ __forceinline__ int longest_ones(uint64_t z) {
+    if (z == ~0ull) return 64;
+    // A_k = bits that start a run of >= k ones; A_{a+b} = A_a & (A_b >> a)
+    const uint64_t p1 = z, p2 = p1 & (p1 >> 1), p4 = p2 & (p2 >> 2), p8 = p4 & (p4 >> 4), p16 = p8 & (p8 >> 8),
+                   p32 = p16 & (p16 >> 16);
+    uint64_t acc = ~0ull;
+    int len = 0;
+    const uint64_t P[6] = {p32, p16, p8, p4, p2, p1};
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int k = 32 >> i;
+        const uint64_t cand = acc & (P[i] >> len);
+        if (cand) { acc = cand; len += k; }
+    }
+    return len;
+}
+
+// Run state over consecutive chunks of n <= 64 words; z = words that match.
+struct RunState {
+    int cur = 0, best = 0;
+    __device__ __forceinline__ void push(uint64_t z, int n) {
+        const uint64_t vm = n == 64 ? ~0ull : ((1ull << n) - 1ull);
+        z &= vm;
+        if (z == vm) { cur += n; best = max(best, cur); return; }
+        const int lead = __builtin_ctzll(~z);
+        best = max(best, cur + lead);
+        best = max(best, longest_ones(z));
+        const int hz = 63 - __builtin_clzll(~z & vm);
+        cur = n - 1 - hz;
+    }
+};
+
+__device__ __forceinline__ float pick32(int lane, const float *f) {
+    float v = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MLP_FEAT; i++) v = lane == i ? f[i] : v;
+    return v;
+}
+
+__global__ __launch_bounds__(FEAT_NT) void feature_kernel(FeatArgs a) {
+    extern __shared__ uint64_t fs[];
+    const int Wc = a.Wc, Wn = (a.N + 63) / 64;
+    uint64_t *s_h0 = fs, *s_h1 = fs + Wc, *s_d = fs + 2 * Wc, *s_rr = fs + 3 * Wc, *s_aa = fs + 4 * Wc,
+             *s_ra = fs + 5 * Wc, *bm0 = fs + 6 * Wc, *bm1 = bm0 + Wn;
+    __shared__ float s_cnt[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = FEAT_NT / 64;
+    const int i = blockIdx.x, t = a.targets[i];
+    const uint8_t *H0 = a.H + (size_t)(2 * t) * a.L, *H1 = a.H + (size_t)(2 * t + 1) * a.L;
+    const uint64_t *tm = a.tmask + (size_t)t * 4 * Wc;
+    // current haplotypes at the common sites, one ballot per 64 common rows
+    for (int c0 = wv * 64; c0 < Wc * 64; c0 += nw * 64) {
+        const int c = c0 + lane;
+        bool b0 = false, b1 = false;
+        if (c < a.Lc) {
+            const int site = a.c2t[c];
+            b0 = H0[site] != 0;
+            b1 = H1[site] != 0;
+        }
+        const uint64_t m0 = __ballot(b0), m1 = __ballot(b1);
+        if (lane == 0) { s_h0[c0 >> 6] = m0; s_h1[c0 >> 6] = m1; }
+    }
+    for (int w = tid; w < Wc; w += FEAT_NT) {
+        s_d[w] = tm[w];
+        s_rr[w] = tm[Wc + w];
+        s_aa[w] = tm[2 * Wc + w];
+        s_ra[w] = tm[3 * Wc + w];
+    }
+    for (int w = tid; w < 2 * Wn; w += FEAT_NT) bm0[w] = 0ull;
+    __syncthreads();
+    // panel haplotypes sharing a rare allele with H0 / H1
+    for (int l = tid; l < a.L; l += FEAT_NT) {
+        const uint8_t f = a.flags[l];
+        if (f & SF_COMMON) continue;
+        const uint8_t maj = (f & SF_MAJOR) ? 1 : 0;
+        const bool r0 = H0[l] != maj, r1 = H1[l] != maj;
+        if (!r0 && !r1) continue;
+        for (int p = a.rare_ptr[l]; p < a.rare_ptr[l + 1]; p++) {
+            const int c = a.rare_idx[p];
+            const uint64_t m = 1ull << (c & 63);
+            if (r0) atomicOr((unsigned long long *)&bm0[c >> 6], (unsigned long long)m);
+            if (r1) atomicOr((unsigned long long *)&bm1[c >> 6], (unsigned long long)m);
+        }
+    }
+    if (wv == 0) {
+        float nd = 0, nrr = 0, naa = 0, nra = 0;
+        for (int w = lane; w < Wc; w += 64) {
+            nd += __popcll(s_d[w]);
+            nrr += __popcll(s_rr[w]);
+            naa += __popcll(s_aa[w]);
+            nra += __popcll(s_ra[w]);
+        }
+        nd = wave_sum(nd); nrr = wave_sum(nrr); naa = wave_sum(naa); nra = wave_sum(nra);
+        if (lane == 0) { s_cnt[0] = nd; s_cnt[1] = nrr; s_cnt[2] = naa; s_cnt[3] = nra; }
+    }
+    __syncthreads();
+    const float inv_d = 1.0f / fmaxf(s_cnt[0], 1.0f), inv_rr = 1.0f / fmaxf(s_cnt[1], 1.0f),
+                inv_aa = 1.0f / fmaxf(s_cnt[2], 1.0f), inv_ra = 1.0f / fmaxf(s_cnt[3], 1.0f);
+    const float inv_lc = 1.0f / (float)max(a.Lc, 1), inv_wc = 1.0f / (float)max(Wc, 1);
+    const int K = a.Kpool[t];
+    const int tail = a.Lc - 64 * (Wc - 1);   // valid bits of the last word
+    const uint64_t lastm = tail >= 64 ? ~0ull : ((1ull << tail) - 1ull);
+    const int32_t *ins = a.ins + (size_t)t * a.kstride;
+    float *Xo = a.X + ((size_t)a.row0 + (size_t)i * a.rstride) * MLP_FEAT;
+    const float inv_k = 1.0f / (float)max(K, 1);
+    for (int j = wv; j < K; j += nw) {
+        const int v = ins[j];
+        const int id = v & 0xFFFFF;
+        const uint64_t *row = a.hm + (size_t)id * Wc;
+        float f[MLP_FEAT];
+        f[0] = (float)((v >> 20) & 63) / (float)a.depth;
+        f[1] = (float)j * inv_k;
+        f[2] = (float)((v >> 26) & 1);
+        f[3] = (float)((v >> 27) & 1);
+        float mD0 = 0, mD1 = 0, cRR = 0, cAA = 0, cRA = 0, alt = 0;
+        float all0 = 0, all1 = 0;
+        RunState run0, run1;
+#pragma unroll
+        for (int win = 0; win < FEAT_WIN; win++) {
+            const int ws = (win * Wc) / FEAT_WIN, we = ((win + 1) * Wc) / FEAT_WIN;
+            float m0 = 0, m1 = 0;
+            for (int cw = ws; cw < we; cw += 64) {
+                const int w = cw + lane;
+                const bool valid = w < we;
+                uint64_t x0 = 1, x1 = 1;
+                if (valid) {
+                    const uint64_t vm = w == Wc - 1 ? lastm : ~0ull;
+                    const uint64_t R = row[w] & vm;
+                    x0 = (R ^ s_h0[w]) & vm;
+                    x1 = (R ^ s_h1[w]) & vm;
+                    const uint64_t D = s_d[w];
+                    m0 += __popcll(x0);
+                    m1 += __popcll(x1);
+                    mD0 += __popcll(x0 & D);
+                    mD1 += __popcll(x1 & D);
+                    cRR += __popcll(R & s_rr[w]);
+                    cAA += __popcll(~R & s_aa[w] & vm);
+                    cRA += __popcll(R & s_ra[w]);
+                    alt += __popcll(R);
+                }
+                const int n = min(64, we - cw);
+                run0.push(__ballot(valid && x0 == 0ull), n);
+                run1.push(__ballot(valid && x1 == 0ull), n);
+            }
+            const float nsite = (float)max(min(64 * we, a.Lc) - 64 * ws, 1);
+            m0 = wave_sum(m0);
+            m1 = wave_sum(m1);
+            all0 += m0;
+            all1 += m1;
+            f[16 + win] = m0 / nsite;
+            f[24 + win] = m1 / nsite;
+        }
+        wave_sum3(mD0, mD1, cRR);
+        wave_sum3(cAA, cRA, alt);
+        f[4] = mD0 * inv_d;
+        f[5] = mD1 * inv_d;
+        f[6] = all0 * inv_lc;
+        f[7] = all1 * inv_lc;
+        f[8] = (float)run0.best * inv_wc;
+        f[9] = (float)run1.best * inv_wc;
+        f[10] = cRR * inv_rr;
+        f[11] = cAA * inv_aa;
+        f[12] = cRA * inv_ra;
+        f[13] = alt * inv_lc;
+        f[14] = (float)((bm0[id >> 6] >> (id & 63)) & 1ull);
+        f[15] = (float)((bm1[id >> 6] >> (id & 63)) & 1ull);
+        const float out = pick32(lane, f);
+        if (lane < MLP_FEAT) Xo[(size_t)j * MLP_FEAT + lane] = out;
+    }
+}
+
+/* ------------------------------------------------------------------ top-k */
+// Bitonic sort of s[0..n2) (n2 a power of two), descending when DESC.
+template <bool DESC, typename T>
+__device__ void bitonic(T *s, int n2) {
+    for (int size = 2; size <= n2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = threadIdx.x; q < (n2 >> 1); q += blockDim.x) {
+                const int p = 2 * q - (q & (stride - 1));
+                const int r = p + stride;
+                const bool up = (p & size) == 0;
+                const T x = s[p], y = s[r];
+                const bool sw = DESC ? ((x < y) == up) : ((x > y) == up);
+                if (sw && x != y) { s[p] = y; s[r] = x; }
+            }
+            __syncthreads();
+        }
+}
+
+__device__ __forceinline__ uint32_t ord_key(float y) {
+    if (!(y == y)) y = -INFINITY;   // NaN sorts last
+    const uint32_t u = __float_as_uint(y);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_score(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+__device__ __forceinline__ float block_sum(float v, float *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    float s = 0.0f;
+    for (int k = 0; k < nw; k++) s += red[k];
+    return s;
+}
+
+__global__ __launch_bounds__(TOPK_NT) void topk_kernel(TopkArgs a) {
+    __shared__ uint64_t s_key[TOPK_MAX];
+    __shared__ float red[TOPK_NT / 64];
+    const int i = blockIdx.x, t = a.targets[i];
+    const int K = a.Kpool[t];
+    const int k = min(a.klearn[t], K);
+    if (k <= 0 || k >= K) return;   // nothing to drop
+    const float *y = a.y + (size_t)a.row0 + (size_t)i * a.rstride;
+    int n2 = 1;
+    while (n2 < K) n2 <<= 1;
+    for (int j = threadIdx.x; j < n2; j += TOPK_NT)
+        s_key[j] = j < K ? (((uint64_t)ord_key(y[j]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)j)) : 0ull;
+    __syncthreads();
+    bitonic<true>(s_key, n2);
+    float top = 0.0f, all = 0.0f;
+    for (int j = threadIdx.x; j < K; j += TOPK_NT) {
+        const float m = fmaxf(expf(key_score((uint32_t)(s_key[j] >> 32))) - LABEL_EPS, 0.0f);
+        all += m;
+        top += j < k ? m : 0.0f;
+    }
+    top = block_sum(top, red);
+    all = block_sum(all, red);
+    const bool keep_pool = !(all > 0.0f) || top < a.min_mass * all;
+    if (threadIdx.x == 0) {
+        a.coverage[t] = all > 0.0f ? top / all : 0.0f;
+        atomicAdd(keep_pool ? &a.counters[1] : &a.counters[0], 1);
+    }
+    if (keep_pool) return;   // plain PBWT selection stays
+    const int32_t *ins = a.ins + (size_t)t * a.kstride;
+    __syncthreads();
+    for (int j = threadIdx.x; j < n2; j += TOPK_NT)
+        s_key[j] = j < k ? (uint64_t)(ins[0xFFFFFFFFu - (uint32_t)(s_key[j] & 0xFFFFFFFFull)] & 0xFFFFF) : ~0ull;
+    __syncthreads();
+    bitonic<false>(s_key, n2);
+    int32_t *st = a.states + (size_t)t * a.kstride;
+    for (int j = threadIdx.x; j < k; j += TOPK_NT) st[j] = (int32_t)s_key[j];
+    if (threadIdx.x == 0) {
+        a.Kdev[t] = k;
+        a.items[t].K = k;
+        a.items[t].W = (k + 63) / 64;
+    }
+}
+
+/* ------------------------------------------------------------------ labels */
+__global__ __launch_bounds__(256) void label_kernel(LabelArgs a) {
+    const int i = blockIdx.x, t = a.targets[i];
+    const CondItem c = a.items[t];
+    const int K = c.K;
+    const int32_t *st = a.states + c.state_off;
+    const int32_t *ins = a.ins + (size_t)t * a.kstride;
+    const float *m0 = a.mass0 + c.state_off, *m1 = a.mass1 + c.state_off;
+    const float inv = 1.0f / (2.0f * (float)max(c.Lp, 1));
+    const size_t r0 = (size_t)a.row0 + (size_t)i * a.rstride;
+    for (int j = threadIdx.x; j < a.rstride; j += blockDim.x) {
+        float lab = 0.0f, wt = 0.0f;
+        if (j < K) {
+            const int id = ins[j] & 0xFFFFF;
+            int lo = 0, hi = K;   // lower_bound in the sorted state list
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (st[mid] < id) lo = mid + 1; else hi = mid;
+            }
+            const float m = (m0[lo] + m1[lo]) * inv;
+            lab = __logf((float)K * m + LABEL_EPS);
+            wt = 1.0f;
+        } else {
+            for (int f = 0; f < MLP_FEAT; f++) a.X[(r0 + j) * MLP_FEAT + f] = 0.0f;
+        }
+        a.Y[r0 + j] = lab;
+        a.W[r0 + j] = wt;
+    }
+}
+
+/* ------------------------------------------------------------------ launchers */
+void target_mask_launch(gh_panel *P, int T, const uint8_t *gl, const uint8_t *flat, uint64_t *tmask,
+                        hipStream_t s) {
+    if (P->Wc == 0) return;
+    hipLaunchKernelGGL(target_mask_kernel, dim3(P->Wc, T), dim3(64), 0, s, gl, flat, P->d_c2t.as<int32_t>(), P->L,
+                       P->Lc, P->Wc, tmask);
+    GH_HIP(hipGetLastError());
+}
+
+size_t feature_lds_bytes(const gh_panel *P) {
+    return sizeof(uint64_t) * (6 * (size_t)P->Wc + 2 * (size_t)((P->N + 63) / 64));
+}
+
+void feature_launch(gh_panel *P, int n, const FeatArgs &a, hipStream_t s) {
+    const size_t lds = feature_lds_bytes(P);
+    GH_CHECK(lds <= FEAT_LDS_MAX, GH_ERR_CAPACITY,
+             "learned selection: chunk too large for the feature kernel (common sites x 6 + haplotypes x 2 bits "
+             "must fit 144 KiB of LDS)");
+    static const bool attr = [] {
+        GH_HIP(hipFuncSetAttribute((const void *)feature_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)FEAT_LDS_MAX));
+        return true;
+    }();
+    (void)attr;
+    if (n <= 0 || P->Wc == 0) return;
+    hipLaunchKernelGGL(feature_kernel, dim3(n), dim3(FEAT_NT), lds, s, a);
+    GH_HIP(hipGetLastError());
+}
+
+void topk_launch(int n, const TopkArgs &a, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(topk_kernel, dim3(n), dim3(TOPK_NT), 0, s, a);
+    GH_HIP(hipGetLastError());
+}
+
+void label_launch(int n, const LabelArgs &a, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(label_kernel, dim3(n), dim3(256), 0, s, a);
+    GH_HIP(hipGetLastError());
+}
+
+/* ------------------------------------------------------------------ batch entry points */
+void select_features_batch(gh_panel *P, int T, const uint8_t *H_h, const uint8_t *gl_h, const uint8_t *flat_h,
+                           const int32_t *ins_h, const int32_t *K_h, int kstride, int depth, float *X_h) {
+    GH_CHECK(P && T > 0 && H_h && gl_h && flat_h && ins_h && K_h && X_h && kstride > 0 && depth > 0, GH_ERR_ARG,
+             "gh_select_features: bad arguments");
+    GH_CHECK(P->N < (1 << 20), GH_ERR_ARG, "gh_select_features: at most 2^20 panel haplotypes");
+    for (int t = 0; t < T; t++) {
+        GH_CHECK(K_h[t] >= 0 && K_h[t] <= kstride, GH_ERR_ARG, "gh_select_features: K out of range");
+        for (int j = 0; j < K_h[t]; j++)
+            GH_CHECK((ins_h[(size_t)t * kstride + j] & 0xFFFFF) < P->N, GH_ERR_ARG, "gh_select_features: bad id");
+    }
+    hipStream_t s = P->stream;
+    const int L = P->L;
+    DBuf H, gl, flat, ins, K, X, tm, tg;
+    std::vector<int32_t> tv(T);
+    for (int t = 0; t < T; t++) tv[t] = t;
+    FeatArgs a;
+    a.hm = P->d_hm.as<uint64_t>(); a.Wc = P->Wc; a.Lc = P->Lc; a.N = P->N; a.L = L; a.depth = depth;
+    a.kstride = kstride; a.rstride = kstride; a.c2t = P->d_c2t.as<int32_t>(); a.flags = P->d_flags.as<uint8_t>();
+    a.rare_ptr = P->d_rare_ptr.as<int32_t>(); a.rare_idx = P->d_rare_idx.as<int32_t>();
+    a.H = dalloc_copy(H, H_h, (size_t)2 * T * L, s);
+    const uint8_t *gld = dalloc_copy(gl, gl_h, (size_t)3 * T * L, s);
+    const uint8_t *fld = dalloc_copy(flat, flat_h, (size_t)T * L, s);
+    tm.reserve(sizeof(uint64_t) * (size_t)T * 4 * std::max(P->Wc, 1));
+    target_mask_launch(P, T, gld, fld, tm.as<uint64_t>(), s);
+    a.tmask = tm.as<uint64_t>();
+    a.targets = dalloc_copy(tg, tv.data(), (size_t)T, s);
+    a.Kpool = dalloc_copy(K, K_h, (size_t)T, s);
+    a.ins = dalloc_copy(ins, ins_h, (size_t)T * kstride, s);
+    X.reserve(sizeof(float) * (size_t)T * kstride * MLP_FEAT);
+    GH_HIP(hipMemsetAsync(X.p, 0, sizeof(float) * (size_t)T * kstride * MLP_FEAT, s));
+    a.X = X.as<float>();
+    a.row0 = 0;
+    feature_launch(P, T, a, s);
+    GH_HIP(hipMemcpyAsync(X_h, X.p, sizeof(float) * (size_t)T * kstride * MLP_FEAT, hipMemcpyDeviceToHost, s));
+    GH_HIP(hipStreamSynchronize(s));
+}
+
+void select_topk_batch(int T, const float *y_h, const int32_t *ins_h, const int32_t *K_h, const int32_t *klearn_h,
+                       int kstride, float min_mass, int32_t *states_h, int32_t *Kout_h, float *cov_h) {
+    GH_CHECK(T > 0 && y_h && ins_h && K_h && klearn_h && states_h && Kout_h && kstride > 0 && kstride <= TOPK_MAX,
+             GH_ERR_ARG, "gh_select_topk: bad arguments");
+    for (int t = 0; t < T; t++)
+        GH_CHECK(K_h[t] >= 0 && K_h[t] <= kstride && klearn_h[t] >= 0, GH_ERR_ARG, "gh_select_topk: bad sizes");
+    hipStream_t s = nullptr;
+    GH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    DBuf y, ins, K, kl, st, Kd, it, cov, cnt, tg;
+    std::vector<int32_t> tv(T);
+    std::vector<CondItem> items(T);
+    for (int t = 0; t < T; t++) {
+        tv[t] = t;
+        std::memset(&items[t], 0, sizeof(CondItem));
+        items[t].K = K_h[t];
+        items[t].W = (K_h[t] + 63) / 64;
+    }
+    TopkArgs a;
+    a.targets = dalloc_copy(tg, tv.data(), (size_t)T, s);
+    a.y = dalloc_copy(y, y_h, (size_t)T * kstride, s);
+    a.ins = dalloc_copy(ins, ins_h, (size_t)T * kstride, s);
+    a.Kpool = dalloc_copy(K, K_h, (size_t)T, s);
+    a.Kdev = (int32_t *)a.Kpool;
+    a.klearn = dalloc_copy(kl, klearn_h, (size_t)T, s);
+    a.row0 = 0;
+    a.kstride = kstride;
+    a.rstride = kstride;
+    a.min_mass = min_mass;
+    a.states = dalloc_copy(st, states_h, (size_t)T * kstride, s);
+    a.items = dalloc_copy(it, items.data(), (size_t)T, s);
+    a.coverage = (float *)cov.reserve(sizeof(float) * T);
+    GH_HIP(hipMemsetAsync(cov.p, 0, sizeof(float) * T, s));
+    a.counters = (int32_t *)cnt.reserve(sizeof(int32_t) * 2);
+    GH_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * 2, s));
+    topk_launch(T, a, s);
+    GH_HIP(hipMemcpyAsync(states_h, st.p, sizeof(int32_t) * (size_t)T * kstride, hipMemcpyDeviceToHost, s));
+    GH_HIP(hipMemcpyAsync(Kout_h, K.p, sizeof(int32_t) * (size_t)T, hipMemcpyDeviceToHost, s));
+    if (cov_h) GH_HIP(hipMemcpyAsync(cov_h, cov.p, sizeof(float) * (size_t)T, hipMemcpyDeviceToHost, s));
+    GH_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace gh

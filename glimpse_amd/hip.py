@@ -40,7 +40,9 @@ class RunStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "fb_state_sites", "fb_poly_sites", "ph_state_sites", "ph_segments", "pbwt_hap_sites",
         "ms_select", "ms_condset", "ms_fb", "ms_phase", "ms_other", "ms_total",
-        "fb_kernel_ms", "fb_launches", "fb_bytes")]
+        "fb_kernel_ms", "fb_launches", "fb_bytes", "ms_learn", "learned_sel", "pool_kept", "collected_rows")]
+
+MLP_IN, MLP_HIDDEN, MLP_NPARAM = 32, 128, 20865
 
 
 def lib():
@@ -66,6 +68,19 @@ def lib():
         L.gh_run_execute.argtypes = [vp]
         L.gh_run_fetch.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(RunStats)]
         L.gh_run_destroy.argtypes = [vp]
+        L.gh_mlp_create.argtypes = [u32, ctypes.POINTER(vp)]
+        L.gh_mlp_destroy.argtypes = [vp]
+        L.gh_mlp_get_params.argtypes = [vp, vp]
+        L.gh_mlp_set_params.argtypes = [vp, vp]
+        L.gh_mlp_forward.argtypes = [vp, i64, vp, vp]
+        L.gh_mlp_backward.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.gh_mlp_step.argtypes = [vp, i32, vp, vp, vp, f32, vp]
+        L.gh_mlp_train_run.argtypes = [vp, vp, i32, i32, f32, u32, vp]
+        L.gh_run_collect.argtypes = [vp, i32]
+        L.gh_run_collected.argtypes = [vp, vp, vp, vp, vp]
+        L.gh_run_set_selector.argtypes = [vp, vp, vp, f32]
+        L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, i32, vp]
+        L.gh_select_topk.argtypes = [i32, vp, vp, vp, vp, i32, f32, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -205,6 +220,20 @@ class Panel:
                                       _p(b), ctypes.c_float(err_phase), ne, seed, it, _p(tg)))
         return a, b
 
+    def select_features(self, H, gl, flat, pool, K, depth=12):
+        """Learned-selector features (gh_select_features): H [2T][L], gl [T][L][3],
+        flat [T][L], pool [T][kstride] (insertion-order codes), K [T] ->
+        X [T][kstride][32]."""
+        H = np.ascontiguousarray(H, np.uint8)
+        gl = np.ascontiguousarray(gl, np.uint8)
+        flat = np.ascontiguousarray(flat, np.uint8)
+        pool = np.ascontiguousarray(pool, np.int32)
+        K = np.ascontiguousarray(K, np.int32)
+        T, ks = pool.shape
+        X = np.zeros((T, ks, MLP_IN), np.float32)
+        _check(lib().gh_select_features(self.h, T, _p(H), _p(gl), _p(flat), _p(pool), _p(K), ks, int(depth), _p(X)))
+        return X
+
     def pbwt_select(self, hap_alleles, stored, start_idx, depth=12, modulo=0.1, cap=None, kpbwt=0):
         """Returns the pbwt_states lists per target and, when kpbwt > 0, the
         compacted sorted conditioning set per target."""
@@ -242,6 +271,21 @@ class Panel:
         return out
 
 
+def select_topk(scores, pool, K, k_learn, states, min_mass=0.95):
+    """gh_select_topk: returns (states [T][kstride], K_out [T], coverage [T])."""
+    scores = np.ascontiguousarray(scores, np.float32)
+    pool = np.ascontiguousarray(pool, np.int32)
+    T, ks = pool.shape
+    K = np.ascontiguousarray(K, np.int32)
+    kl = np.ascontiguousarray(np.broadcast_to(np.asarray(k_learn, np.int32), (T,)), np.int32)
+    st = np.array(states, np.int32, copy=True, order="C")
+    Ko = np.zeros(T, np.int32)
+    cov = np.zeros(T, np.float32)
+    _check(lib().gh_select_topk(T, _p(scores), _p(pool), _p(K), _p(kl), ks, float(min_mass), _p(st), _p(Ko),
+                                _p(cov)))
+    return st, Ko, cov
+
+
 class Run:
     """A prepared Gibbs run (gh_run_*): inputs resident on the device.
     execute() runs the whole INIT + burn-in + main schedule on the GPU and
@@ -265,6 +309,31 @@ class Run:
     def execute(self):
         _check(lib().gh_run_execute(self.h))
 
+    def collect(self, from_iter=1):
+        """Record (features, label, weight) rows for every PBWT pool candidate at
+        iterations >= from_iter (a negative value switches recording off)."""
+        _check(lib().gh_run_collect(self.h, int(from_iter)))
+
+    def collected(self):
+        n = ctypes.c_int64(0)
+        _check(lib().gh_run_collected(self.h, ctypes.byref(n), None, None, None))
+        X = np.zeros((n.value, MLP_IN), np.float32)
+        y = np.zeros(n.value, np.float32)
+        w = np.zeros(n.value, np.float32)
+        _check(lib().gh_run_collected(self.h, ctypes.byref(n), _p(X), _p(y), _p(w)))
+        return X, y, w
+
+    def set_selector(self, mlp, k_learn, min_mass=0.95):
+        """Use the learned selector: target t keeps k_learn[t] states of its PBWT
+        pool, or the whole pool when their predicted mass share < min_mass."""
+        if mlp is None:
+            _check(lib().gh_run_set_selector(self.h, None, None, 0.0))
+            self._mlp = None
+            return
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(k_learn, np.int32), (self.T,)), np.int32)
+        _check(lib().gh_run_set_selector(self.h, mlp.h, _p(k), float(min_mass)))
+        self._mlp = mlp   # keep the network alive while the run may use it
+
     def fetch(self):
         T, L = self.T, self.L
         out = dict(gp0=np.zeros((T, L), np.float32), gp1=np.zeros((T, L), np.float32),
@@ -284,6 +353,68 @@ class Run:
     def close(self):
         if getattr(self, "h", None) and self.h.value:
             lib().gh_run_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mlp:
+    """The learned selector's network (gh_mlp_*): 32 -> 128 -> 128 -> 1, BF16
+    operands on MFMA with FP32 accumulation, FP32 master weights and Adam."""
+
+    def __init__(self, seed=15052011):
+        h = vp()
+        _check(lib().gh_mlp_create(int(seed) & 0xFFFFFFFF, ctypes.byref(h)))
+        self.h = h
+
+    def params(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        _check(lib().gh_mlp_get_params(self.h, _p(w)))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.shape == (MLP_NPARAM,)
+        _check(lib().gh_mlp_set_params(self.h, _p(w)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+        y = np.zeros(X.shape[0], np.float32)
+        _check(lib().gh_mlp_forward(self.h, X.shape[0], _p(X), _p(y)))
+        return y
+
+    def backward(self, X, t, w=None):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+        t = np.ascontiguousarray(t, np.float32)
+        w = None if w is None else np.ascontiguousarray(w, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = ctypes.c_float(0)
+        _check(lib().gh_mlp_backward(self.h, X.shape[0], _p(X), _p(t), _p(w), _p(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def step(self, X, t, w=None, lr=1e-3):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+        t = np.ascontiguousarray(t, np.float32)
+        w = None if w is None else np.ascontiguousarray(w, np.float32)
+        loss = ctypes.c_float(0)
+        _check(lib().gh_mlp_step(self.h, X.shape[0], _p(X), _p(t), _p(w), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def train_run(self, run, epochs=4, batch=16384, lr=2e-3, seed=1):
+        """Adam on the rows a run recorded (device resident); returns the mean
+        batch loss per epoch."""
+        le = np.zeros(epochs, np.float32)
+        _check(lib().gh_mlp_train_run(self.h, run.h, int(epochs), int(batch), float(lr), int(seed) & 0xFFFFFFFF,
+                                      _p(le)))
+        return le
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().gh_mlp_destroy(self.h)
             self.h = vp()
 
     def __del__(self):

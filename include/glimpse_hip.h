@@ -82,6 +82,10 @@ typedef struct {
     double fb_kernel_ms;       /* summed HIP-event duration of the forward-backward kernel launches */
     double fb_launches;        /* number of those launches */
     double fb_bytes;           /* their algorithmic HBM bytes: per haplotype Lp*(2*8*ceil(K/64) + 8*K + 16) + 8*L */
+    double ms_learn;           /* learned selection: features + network + top-k (included in ms_select) */
+    double learned_sel;        /* target-iterations that used the learned selection */
+    double pool_kept;          /* target-iterations that fell back to the whole PBWT pool */
+    double collected_rows;     /* training rows recorded by gh_run_collect */
 } gh_run_stats;
 
 const char *gh_last_error(void);
@@ -160,6 +164,71 @@ int gh_run_execute(gh_run *r);
 int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1, int32_t *nstates,
                  gh_run_stats *stats);
 int gh_run_destroy(gh_run *r);
+
+/* ---------------------------------------------------------------------------
+ * Learned state selector.  No counterpart in the reference: it is an addition
+ * beside conditioning_set::compactSelection's Kpbwt branch
+ * (phase/src/containers/conditioning_set.cpp:96-104), which stays the fallback.
+ *
+ * A 32 -> 128 -> 128 -> 1 ReLU network scores every candidate of a target's
+ * PBWT pool (the Kpbwt states the plain method would use) from 32 features of
+ * the candidate against the target's current haplotypes and reads, predicting
+ * y = ln(K * posterior mass + 1e-3).  The selector keeps the k highest-scoring
+ * candidates; when their predicted share of the pool's mass is below min_mass
+ * the whole pool is kept.  BF16 operands on the matrix cores, FP32
+ * accumulation, FP32 master weights, Adam.
+ */
+#define GH_MLP_IN 32
+#define GH_MLP_HIDDEN 128
+#define GH_MLP_NPARAM 20865   /* W1[32][128], b1[128], W2[128][128], b2[128], W3[128], b3 */
+typedef struct gh_mlp gh_mlp;
+/* He-initialised network on the current device. */
+int gh_mlp_create(uint32_t seed, gh_mlp **out);
+int gh_mlp_destroy(gh_mlp *m);
+/* Parameters as one FP32 block of GH_MLP_NPARAM (layout above); set also
+ * resets the optimiser. */
+int gh_mlp_get_params(gh_mlp *m, float *w);
+int gh_mlp_set_params(gh_mlp *m, const float *w);
+/* y[n] = net(X[n][32]) (host arrays). */
+int gh_mlp_forward(gh_mlp *m, int64_t n, const float *X, float *y);
+/* Gradient of sum_r w_r (net(X_r) - t_r)^2 / sum_r w_r (w may be NULL: all
+ * ones) into grad[GH_MLP_NPARAM]; the loss into *loss. */
+int gh_mlp_backward(gh_mlp *m, int32_t n, const float *X, const float *t, const float *w, float *grad, float *loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient. */
+int gh_mlp_step(gh_mlp *m, int32_t n, const float *X, const float *t, const float *w, float lr, float *loss);
+/* Trains on the rows a run recorded (gh_run_collect), on the device:
+ * shuffled batches, learning rate decaying linearly from lr to lr/10;
+ * loss_per_epoch optional [epochs]. */
+int gh_mlp_train_run(gh_mlp *m, gh_run *r, int32_t epochs, int32_t batch, float lr, uint32_t seed,
+                     float *loss_per_epoch);
+
+/* Before gh_run_execute: record a training row (features, label, weight) for
+ * every PBWT pool candidate of every target at iterations >= from_iter (>= 1);
+ * the label is ln(K * m + 1e-3) with m the candidate's posterior mass from both
+ * haplotypes' forward-backward, averaged over polymorphic sites. */
+int gh_run_collect(gh_run *r, int32_t from_iter);
+/* Copies the recorded rows (NULL buffers: only *n_rows). */
+int gh_run_collected(gh_run *r, int64_t *n_rows, float *X, float *y, float *w);
+/* Before gh_run_execute: use the learned selector from iteration 1 on.  Target
+ * t keeps k_learn[t] states of its pool, or the whole pool when the predicted
+ * mass share is below min_mass (0.95).  m = NULL switches it off.  The network
+ * must not be trained while the run executes. */
+int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass);
+
+/* The selector's two steps as batch calls (what a run does between the PBWT
+ * compaction and the conditioning sets).  pool [T][kstride]: candidates in PBWT
+ * insertion order, each id | rank << 20 | (listed by H1's walk) << 26 |
+ * (upper neighbour) << 27, K[t] of them.  H [2T][L], gl [T][L][3], flat [T][L].
+ * Features X [T][kstride][32] (rows >= K[t] are zero). */
+int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *gl, const uint8_t *flat,
+                       const int32_t *pool, const int32_t *K, int32_t kstride, int32_t depth, float *X);
+/* Top k_learn[t] of the pool by score (ties: earlier insertion first); states
+ * [T][kstride] holds the sorted pool on input and the sorted selection on
+ * output, K_out[t] its size; when k_learn[t] >= K[t] or the kept candidates'
+ * share of sum(max(exp(score) - 1e-3, 0)) is below min_mass, the pool is left
+ * as it is.  coverage [T] optional (0 where nothing was dropped). */
+int gh_select_topk(int32_t T, const float *scores, const int32_t *pool, const int32_t *K, const int32_t *k_learn,
+                   int32_t kstride, float min_mass, int32_t *states, int32_t *K_out, float *coverage);
 
 #ifdef __cplusplus
 }
