@@ -1,21 +1,33 @@
 #!/usr/bin/env python
 """bench.py -- GLIMPSE2_phase compute core on one MI355X.
 
-Workload (the batched plain-PBWT configuration): a synthetic panel of 10,000
-haplotypes x 60,000 biallelic SNPs (5,000 SNPs per Mb at 1 cM/Mb, 50-founder
-Li-Stephens mosaics, 1/x frequency spectrum, mutation 1e-3), 200 diploid
-targets with read coverage U(0.1, 2)x, cut into 5 chunks of U[20k, 60k] sites
-(windows of the panel, each with its own sparse PBWT), targets spread over the
-chunks, Kpbwt per target U[500, 2000], default Gibbs schedule (1 INIT + 5
-burn-in + 15 main).  One step = one complete phase run of every chunk (all
-targets, all iterations) with the inputs already resident in HBM; chunks run
-concurrently on their own HIP streams.
+Workload (the batched configuration): a synthetic panel of 10,000 haplotypes x
+60,000 biallelic SNPs (5,000 SNPs per Mb at 1 cM/Mb, 50-founder Li-Stephens
+mosaics, 1/x frequency spectrum, mutation 1e-3), 200 diploid targets with read
+coverage U(0.1, 2)x, cut into 5 chunks of U[20k, 60k] sites (windows of the
+panel, each with its own sparse PBWT), targets spread over the chunks, Kpbwt
+per target U[500, 2000], default Gibbs schedule (1 INIT + 5 burn-in + 15
+main).  One step = one complete phase run of every chunk (all targets, all
+iterations) with the inputs already resident in HBM; chunks run concurrently
+on their own HIP streams.
 
-metric value: target-site imputations/s = sum over chunks of targets x chunk
-sites / wall time of one step.  The roofline object reports the dominant
-kernel (haploid forward-backward) against HBM: algorithmic bytes per launch /
-average launch duration (HIP events on the launch stream).  cpu_baseline: the
-CPU oracle on a bounded sample of the same workload (one target of the
+Two paths over the same inputs:
+  plain PBWT  -- every target conditions on its Kpbwt PBWT states;
+  learned     -- the selector network keeps Kpbwt/5 states of each target's
+                 PBWT pool (from iteration 1 on; INIT is unchanged).
+The network is first trained on the GPU (outside the timed region) on rows a
+plain run records: a separate panel of 4,000 haplotypes x 10,000 SNPs (2 Mb),
+48 targets at 1x, Kpbwt 2,000, every PBWT iteration; labels are the HMM's own
+posterior masses.
+
+metric value: target-site imputations/s of the learned path = sum over chunks
+of targets x chunk sites / wall time of one step; the plain path's figure,
+both dosage r2 / NRC against the simulated truth, and the learned path's mean
+|DS - DS(plain)| (the plain run at Kpbwt judges its dosage error) are in the
+same line.  The roofline object reports the dominant kernel (haploid
+forward-backward) of the learned path against HBM: algorithmic bytes per launch
+/ average launch duration (HIP events on the launch stream).  cpu_baseline:
+the CPU oracle on a bounded sample of the same workload (one target of the
 smallest chunk, INIT + 1 burn-in + 1 main iteration, scaled to the 21
 iterations of a full run), one core.
 """
@@ -109,29 +121,30 @@ def cpu_baseline(chunks, seconds_hint=""):
                        "iterations in %.2f s, scaled x21/3 to a full run" % (L, int(c["K"][0]), dt))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--quick", action="store_true", help="small workload (profiling / smoke)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serial", action="store_true", help="run chunks one after another")
-    a = ap.parse_args()
-    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on exactly one GPU (the Gibbs chains are independent per target and "
-              "chunk; multi-GPU sharding is design only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    import torch
-    from glimpse_amd import hip
+def train_selector(seed=31337, epochs=8):
+    """Trains the selector network on rows a plain K=2000 run records."""
+    from glimpse_amd import hip, synth
+    t0 = time.perf_counter()
+    P = synth.make_panel(4000, 10000, 48, length_mb=2.0, seed=seed)
+    gl, flat = synth.make_reads(P["tgt_haps"], 1.0, seed=seed + 1)
+    panel = hip.Panel(P)
+    run = hip.Run(panel, gl, flat, hip.default_params(kpbwt=2000))
+    run.collect(1)
+    run.execute()
+    rows = int(run.stats()["collected_rows"])
+    t1 = time.perf_counter()
+    net = hip.Mlp(seed=seed)
+    losses = net.train_run(run, epochs=epochs, batch=16384, lr=2e-3, seed=seed)
+    t2 = time.perf_counter()
+    run.close()
+    info = dict(rows=rows, epochs=epochs, label_run_s=round(t1 - t0, 2), train_s=round(t2 - t1, 3),
+                loss_first=float(losses[0]), loss_last=float(losses[-1]),
+                data="4000 haps x 10000 SNPs (2 Mb), 48 targets at 1x, Kpbwt 2000, rows from iterations 1-20")
+    return net, info
 
-    t_setup = time.perf_counter()
-    chunks, desc, meta = build_workload(a.quick)
-    panels = [hip.Panel(c["panel"]) for c in chunks]
-    prm = hip.default_params()
-    runs = [hip.Run(p, c["gl"], c["flat"], prm, kpbwt_per_target=c["K"]) for p, c in zip(panels, chunks)]
-    setup_s = time.perf_counter() - t_setup
-    print("[bench] setup %.1f s (%s)" % (setup_s, desc), file=sys.stderr, flush=True)
+
+def timed(runs, a, label):
+    import torch
 
     def step():
         if a.serial:
@@ -155,48 +168,123 @@ def main():
 
     for i in range(a.warmup):
         step()
-        print("[bench] warmup step %d done" % i, file=sys.stderr, flush=True)
+        print("[bench] %s warmup step %d done" % (label, i), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / max(1, a.steps)
-    outs = [r.fetch() for r in runs]
-    tsites = sum(len(c["targets"]) * c["panel"]["L"] for c in chunks)
-    value = tsites / dt
+    print("[bench] %s: %.2f s per step" % (label, dt), file=sys.stderr, flush=True)
+    return dt, [r.fetch() for r in runs]
+
+
+def fb_roofline(outs):
     fb_bytes = sum(o["stats"]["fb_bytes"] for o in outs)
     fb_ms = sum(o["stats"]["fb_kernel_ms"] for o in outs)
     fb_n = sum(o["stats"]["fb_launches"] for o in outs)
-    per_launch_bytes = fb_bytes / fb_n
+    per_launch = fb_bytes / fb_n
     avg_ms = fb_ms / fb_n
-    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-    acc = accuracy(outs, chunks)
-    stage_ms = {k: round(sum(o["stats"][k] for o in outs), 1)
-                for k in ("ms_select", "ms_condset", "ms_fb", "ms_phase", "ms_other")}
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "fb_kernel (haploid forward-backward)",
+            "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches_per_step": fb_n}
+
+
+def stage_ms(outs):
+    return {k: round(sum(o["stats"][k] for o in outs), 1)
+            for k in ("ms_select", "ms_learn", "ms_condset", "ms_fb", "ms_phase", "ms_other")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--quick", action="store_true", help="small workload (profiling / smoke)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serial", action="store_true", help="run chunks one after another")
+    ap.add_argument("--path", choices=("both", "plain", "learned"), default="both")
+    ap.add_argument("--k-frac", type=float, default=0.2, help="learned path: kept fraction of the PBWT pool")
+    ap.add_argument("--min-mass", type=float, default=0.0,
+                    help="learned path: keep the whole pool below this predicted mass share")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on exactly one GPU (the Gibbs chains are independent per target and "
+              "chunk; multi-GPU sharding is design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.synchronize()   # bring torch's HIP runtime up before the library's first call
+    from glimpse_amd import hip
+
+    t_setup = time.perf_counter()
+    chunks, desc, meta = build_workload(a.quick)
+    panels = [hip.Panel(c["panel"]) for c in chunks]
+    prm = hip.default_params()
+    runs = [hip.Run(p, c["gl"], c["flat"], prm, kpbwt_per_target=c["K"]) for p, c in zip(panels, chunks)]
+    setup_s = time.perf_counter() - t_setup
+    print("[bench] setup %.1f s (%s)" % (setup_s, desc), file=sys.stderr, flush=True)
+    tsites = sum(len(c["targets"]) * c["panel"]["L"] for c in chunks)
+
+    res = {}
+    if a.path in ("both", "plain"):
+        dt, outs = timed(runs, a, "plain")
+        res["plain"] = dict(value=tsites / dt, ms_per_step=dt * 1e3, outs=outs)
+    train = None
+    if a.path in ("both", "learned"):
+        net, train = train_selector()
+        print("[bench] selector trained: %s" % json.dumps(train), file=sys.stderr, flush=True)
+        for r, c in zip(runs, chunks):
+            r.set_selector(net, np.maximum(1, np.round(c["K"] * a.k_frac)).astype(np.int32), a.min_mass)
+        dt, outs = timed(runs, a, "learned")
+        res["learned"] = dict(value=tsites / dt, ms_per_step=dt * 1e3, outs=outs)
+    for k, v in res.items():
+        v["accuracy"] = accuracy(v["outs"], chunks)
+        v["stage_ms_summed_over_chunks"] = stage_ms(v["outs"])
+    main_path = "learned" if "learned" in res else "plain"
+    M = res[main_path]
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": M["value"],
         "unit": "target-site imputations/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": dt * 1e3,
+        "ms_per_step": M["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded simulator: founder mosaics + Poisson reads)",
-        "config": dict(workload=desc, path="plain PBWT selection", concurrency="serial" if a.serial else "chunks on concurrent streams",
-                       **meta),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fb_kernel (haploid forward-backward)",
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms, "launches_per_step": fb_n},
-        "accuracy": acc,
-        "stage_ms_summed_over_chunks": stage_ms,
+        "config": dict(workload=desc, path=main_path,
+                       concurrency="serial" if a.serial else "chunks on concurrent streams", **meta),
+        "roofline": fb_roofline(M["outs"]),
+        "accuracy": M["accuracy"],
+        "stage_ms_summed_over_chunks": M["stage_ms_summed_over_chunks"],
         "setup_s": setup_s,
     }
+    if "learned" in res:
+        L_ = res["learned"]
+        st = [o["stats"] for o in L_["outs"]]
+        line["learned"] = dict(k_frac=a.k_frac, min_mass=a.min_mass, training=train,
+                               target_iterations_learned=sum(s["learned_sel"] for s in st),
+                               target_iterations_pool_kept=sum(s["pool_kept"] for s in st),
+                               mean_states=float(np.mean(np.concatenate(
+                                   [o["nstates"][:, 1:].ravel() for o in L_["outs"]]))))
+    if "plain" in res:
+        Pn = res["plain"]
+        line["plain_pbwt"] = dict(value=Pn["value"], ms_per_step=Pn["ms_per_step"], accuracy=Pn["accuracy"],
+                                  roofline=fb_roofline(Pn["outs"]),
+                                  stage_ms_summed_over_chunks=Pn["stage_ms_summed_over_chunks"],
+                                  mean_states=float(np.mean(np.concatenate(
+                                      [o["nstates"][:, 1:].ravel() for o in Pn["outs"]]))))
+        if "learned" in res:
+            dsl = np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in res["learned"]["outs"]])
+            dsp = np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in Pn["outs"]])
+            line["learned"]["mean_abs_ds_vs_plain"] = float(np.mean(np.abs(dsl - dsp)))
+            line["learned"]["r2_minus_plain"] = res["learned"]["accuracy"]["dosage_r2"] - Pn["accuracy"]["dosage_r2"]
+            line["learned"]["speedup_vs_plain"] = res["learned"]["value"] / Pn["value"]
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(chunks)
     print(json.dumps(line), flush=True)

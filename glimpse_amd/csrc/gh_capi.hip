@@ -27,7 +27,7 @@ void run_destroy(gh_run *G);
 void run_collect(gh_run *G, int from_iter);
 void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w);
 void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss);
-void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass);
+void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -428,8 +428,8 @@ int gh_run_collect(gh_run *r, int32_t from_iter) {
 int gh_run_collected(gh_run *r, int64_t *n_rows, float *X, float *y, float *w) {
     return guard([&] { run_collected(r, n_rows, X, y, w); });
 }
-int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass) {
-    return guard([&] { run_set_selector(r, m, k_learn, min_mass); });
+int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass, int32_t mode) {
+    return guard([&] { run_set_selector(r, m, k_learn, min_mass, mode); });
 }
 int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *gl, const uint8_t *flat,
                        const int32_t *pool, const int32_t *K, int32_t kstride, int32_t depth, float *X) {

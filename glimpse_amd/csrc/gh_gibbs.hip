@@ -255,10 +255,12 @@ struct gh_run {
     // learned selection / training-row collection
     gh_mlp *mlp = nullptr;
     float min_mass = 0.95f;
+    int sel_mode = 1;          // 0: global top-k, 1: top-k per PBWT group, groups served round robin
     int collect_from = -1;
     int64_t col_rows = 0;
     bool masks_ready = false;
-    gh::DBuf d_klearn, d_ins, d_fx, d_fy, d_cov, d_cnt, d_tmask, d_mass0, d_mass1, d_cx, d_cy, d_cw;
+    gh::DBuf d_klearn, d_ins, d_fx, d_fy, d_cov, d_cnt, d_tmask, d_mass0, d_mass1, d_cx, d_cy, d_cw, d_isite,
+        d_sgrp, d_tcls;
 };
 
 namespace gh {
@@ -445,6 +447,25 @@ void run_execute(gh_run *G) {
         }
         G->d_cnt.reserve(sizeof(int32_t) * 2);
         GH_HIP(hipMemsetAsync(G->d_cnt.p, 0, sizeof(int32_t) * 2, s));
+        if (!G->d_sgrp.p) {
+            // PBWT group of every HQ common site, per class; class of every target
+            const int nc = (int)G->cls_grp.size();
+            std::vector<int32_t> sg((size_t)std::max(nc, 1) * L, -1), tc(T, 0);
+            for (int c = 0; c < nc; c++) {
+                int g = 0;
+                for (int l_all = 0, l_hq = 0; l_all < P->Lc; l_all++) {
+                    const int k = P->common2tot[l_all];
+                    if (P->flags[k] & SF_LQ) continue;
+                    while (g < (int)G->cls_grp[c].size() && l_hq >= G->cls_grp[c][g]) g++;
+                    sg[(size_t)c * L + k] = g;
+                    l_hq++;
+                }
+            }
+            for (int t = 0; t < T; t++) tc[t] = G->hap_cls[2 * t];
+            dalloc_copy(G->d_sgrp, sg.data(), sg.size(), s);
+            dalloc_copy(G->d_tcls, tc.data(), tc.size(), s);
+            G->d_isite.reserve(sizeof(int32_t) * (size_t)T * kst);
+        }
     }
     if (learned) {
         G->d_fx.reserve(sizeof(float) * (size_t)npt * prs * MLP_FEAT);
@@ -458,6 +479,7 @@ void run_execute(gh_run *G) {
     G->col_rows = 0;
     PbwtRun R = G->R;
     R.ins = (learned || collecting) ? G->d_ins.as<int32_t>() : nullptr;
+    R.ins_site = (learned || collecting) ? G->d_isite.as<int32_t>() : nullptr;
     R.haps = H;
     R.status = status + 1;
     R.out_states = states;
@@ -565,6 +587,8 @@ void run_execute(gh_run *G) {
                     ta.min_mass = G->min_mass;
                     ta.ins = G->d_ins.as<int32_t>(); ta.states = states; ta.Kdev = Kdev; ta.items = items_d;
                     ta.coverage = G->d_cov.as<float>(); ta.counters = G->d_cnt.as<int32_t>();
+                    ta.ins_site = G->sel_mode == 1 ? G->d_isite.as<int32_t>() : nullptr;
+                    ta.site_grp = G->d_sgrp.as<int32_t>(); ta.tcls = G->d_tcls.as<int32_t>(); ta.L = L;
                     topk_launch(npt, ta, s);
                 }
                 GH_HIP(hipEventRecord(e_learn.b, s));
@@ -715,7 +739,9 @@ void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t s
                      lr, seed, G->P->stream, loss);
 }
 
-void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass) {
+void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode) {
+    GH_CHECK(mode == 0 || mode == 1, GH_ERR_ARG, "gh_run_set_selector: mode must be 0 or 1");
+    if (G) G->sel_mode = mode;
     GH_CHECK(G, GH_ERR_ARG, "gh_run_set_selector: null run");
     if (!M) { G->mlp = nullptr; return; }
     GH_CHECK(klearn, GH_ERR_ARG, "gh_run_set_selector: k_learn is required");

@@ -33,6 +33,13 @@
 
 namespace gh {
 
+// LDS tiles of the walk: the rank data (column words + popcount prefixes) of
+// consecutive site ops is contiguous, so a tile is one copy of a word range.
+constexpr int WALK_NT = 256;
+constexpr int TILE_WORDS = 8192;
+constexpr int TILE_OPS = 64;
+constexpr size_t WALK_LDS = (size_t)TILE_WORDS * 12 + sizeof(SiteOp) * TILE_OPS + (size_t)TILE_OPS * WALK_NT;
+
 /* ------------------------------------------------------------------ host: pack3 */
 // pack3 code (ref_haplotype_set.h:45-85): bit 7 = symbol; 0x40 clear -> count in
 // low 6 bits; 0x40 set and 0x20 clear -> (low 5 bits) << 6; both set -> << 11.
@@ -277,6 +284,29 @@ void pbwt_decode(gh_panel *P) {
     GH_CHECK(y == yend, GH_ERR_ARG, "sparse-PBWT stream longer than the panel");
     D->n_ops = (int)ops.size();
     D->n_runs = n_runs;
+    // LDS tiles for the walk: greedy runs of ops whose rank data fits TILE_WORDS
+    D->total_words = (int64_t)bits.size();
+    {
+        std::vector<int32_t> tiles;
+        const int n = (int)ops.size();
+        auto end_of = [&](int o) { return o < n ? ops[o].roff : (int64_t)bits.size(); };
+        bool ok = true;
+        for (int o = 0; o < n && ok;) {
+            const int o0 = o;
+            if (end_of(o0 + 1) - ops[o0].roff > TILE_WORDS) { ok = false; break; }
+            int o1 = o0 + 1;
+            while (o1 < n && o1 - o0 < TILE_OPS && end_of(o1 + 1) - ops[o0].roff <= TILE_WORDS) o1++;
+            tiles.push_back(o0);
+            o = o1;
+        }
+        if (ok && n > 0) {
+            tiles.push_back(n);
+            D->n_tiles = (int)tiles.size() - 1;
+            dalloc_copy(D->tiles, tiles.data(), tiles.size(), P->stream);
+        } else {
+            D->n_tiles = 0;   // a single site's rank data exceeds a tile: global lookups
+        }
+    }
     D->n_arrays = Lhq + n_extra;
     hipStream_t s = P->stream;
     dalloc_copy(D->ops, ops.data(), ops.size(), s);
@@ -315,6 +345,9 @@ struct WalkArgs {
     int cap;
     int32_t *ev_cnt;       // [n_haps]
     int32_t *status;
+    const int32_t *tile_op;  // [n_tiles + 1] first op of each LDS tile
+    int n_tiles;
+    int64_t total_words;     // length of bits / cnt
 };
 
 __device__ __forceinline__ int rank1(const uint64_t *bits, const uint32_t *cnt, int64_t off, int x) {
@@ -332,9 +365,22 @@ __device__ __forceinline__ int lower_count(const int32_t *a, int n, int v) {
     return lo;
 }
 
-__global__ __launch_bounds__(64) void pbwt_walk_kernel(WalkArgs w) {
-    const int h = blockIdx.x * 64 + threadIdx.x;
-    if (h >= w.n_haps) return;
+// STAGED: the rank data, site ops and this tile's target alleles / stored
+// flags are copied to LDS tile by tile, so the per-site dependency chain (rank
+// lookup -> new insertion index -> next lookup) runs at LDS latency.  Without
+// STAGED every lookup goes to global memory (panels whose single site exceeds
+// a tile).
+template <bool STAGED>
+__global__ __launch_bounds__(WALK_NT) void pbwt_walk_kernel(WalkArgs w) {
+    extern __shared__ uint64_t wsm[];
+    uint64_t *s_bits = wsm;
+    uint32_t *s_cnt = (uint32_t *)(wsm + TILE_WORDS);
+    SiteOp *s_ops = (SiteOp *)(s_cnt + TILE_WORDS);
+    uint8_t *s_a = (uint8_t *)(s_ops + TILE_OPS);
+    const int h0 = blockIdx.x * WALK_NT + threadIdx.x;
+    const bool act = h0 < w.n_haps;
+    if (!STAGED && !act) return;
+    const int h = act ? h0 : 0;   // idle threads only help to fill the tiles
     const int N = w.N, depth = w.depth, rec = 2 + 2 * depth;
     const uint8_t *hap = w.haps + (size_t)(w.hap_row ? w.hap_row[h] : h) * w.L;
     const uint8_t *stored = w.stored + (size_t)w.hap_class[h] * w.Lhq;
@@ -371,9 +417,60 @@ __global__ __launch_bounds__(64) void pbwt_walk_kernel(WalkArgs w) {
         nev++;
     };
 
-    for (int o = 0; o < w.n_ops; o++) {
-        const SiteOp op = w.ops[o];
+    const int n_tiles = STAGED ? w.n_tiles : 1;
+    for (int ti = 0; ti < n_tiles; ti++) {
+    int o0 = 0, o1 = w.n_ops;
+    int64_t r0 = 0;
+    if (STAGED) {
+        o0 = w.tile_op[ti];
+        o1 = w.tile_op[ti + 1];
+        r0 = w.ops[o0].roff;
+        const int64_t r1 = o1 < w.n_ops ? w.ops[o1].roff : w.total_words;
+        __syncthreads();   // the previous tile is no longer read
+        // batches of independent loads, so a tile costs a few memory latencies
+        const int nwd = (int)(r1 - r0);
+        for (int i0 = 0; i0 < nwd; i0 += WALK_NT * 8) {
+            uint64_t vb[8];
+            uint32_t vc[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + u * WALK_NT + threadIdx.x;
+                if (i < nwd) { vb[u] = w.bits[r0 + i]; vc[u] = w.cnt[r0 + i]; }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + u * WALK_NT + threadIdx.x;
+                if (i < nwd) { s_bits[i] = vb[u]; s_cnt[i] = vc[u]; }
+            }
+        }
+        for (int i = threadIdx.x; i < o1 - o0; i += WALK_NT) s_ops[i] = w.ops[o0 + i];
+        __syncthreads();
+        const int no = o1 - o0;
+        for (int j0 = 0; j0 < no; j0 += 16) {
+            uint8_t va[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int j = j0 + u;
+                if (j < no) {
+                    const SiteOp &oj = s_ops[j];
+                    va[u] = (uint8_t)(hap[oj.k] | (oj.kind == 0 ? (stored[oj.l] != 0) << 1 : 0));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                if (j0 + u < no) s_a[(j0 + u) * WALK_NT + threadIdx.x] = va[u];
+        }
+        __syncthreads();
+    }
+    const uint64_t *Rb = STAGED ? s_bits : w.bits;
+    const uint32_t *Rc = STAGED ? s_cnt : w.cnt;
+    if (act)
+    for (int o = o0; o < o1; o++) {
+        const SiteOp op = STAGED ? s_ops[o - o0] : w.ops[o];
         const int k = op.k;
+        const int64_t roff = op.roff - r0;
+        const int sa = STAGED ? s_a[(o - o0) * WALK_NT + threadIdx.x] : 0;
+        const int a_here = STAGED ? (sa & 1) : hap[k];
         if (op.kind == 0) {
             if (op.init) {
                 // haplotype_set::init_common (haplotype_set.cpp:562-630)
@@ -404,18 +501,18 @@ __global__ __launch_bounds__(64) void pbwt_walk_kernel(WalkArgs w) {
                 in_rare = false;
             }
             // haplotype_set::select_common_pd_fg (haplotype_set.cpp:445-490)
-            const int a = hap[k];
+            const int a = a_here;
             const int prev = tar;
             tar = a;
             const int ref = op.ref;
-            const int vi = rank1(w.bits, w.cnt, op.roff, idx);
-            const int vf = rank1(w.bits, w.cnt, op.roff, f);
-            const int vg = rank1(w.bits, w.cnt, op.roff, g);
+            const int vi = rank1(Rb, Rc, roff, idx);
+            const int vf = rank1(Rb, Rc, roff, f);
+            const int vg = rank1(Rb, Rc, roff, g);
             const int id = a ? ref + vi : idx - vi;
             int fd = a ? ref + vf : f - vf;
             int gd = a ? ref + vg : g - vg;
             bool reset = false;
-            const bool st = stored[op.l] != 0;
+            const bool st = STAGED ? ((sa >> 1) & 1) != 0 : stored[op.l] != 0;
             if (gd <= fd) {
                 if (st || w.cm_pos[k > 0 ? k - 1 : 0] - w.cm_pos[last_reset] > modulo / 2.0f)
                     selectK(2 * k + 1, op.prev_ref, w.A + (size_t)op.a_in * N, prev);
@@ -454,11 +551,11 @@ __global__ __launch_bounds__(64) void pbwt_walk_kernel(WalkArgs w) {
             if (in_rare) {
                 // haplotype_set::select_rare_pd_fg (haplotype_set.cpp:492-560)
                 const int ref = op.ref;
-                const int vi = rank1(w.bits, w.cnt, op.roff, sidx);
-                const int vf = rank1(w.bits, w.cnt, op.roff, sf);
-                const int vg = rank1(w.bits, w.cnt, op.roff, sg);
+                const int vi = rank1(Rb, Rc, roff, sidx);
+                const int vf = rank1(Rb, Rc, roff, sf);
+                const int vg = rank1(Rb, Rc, roff, sg);
                 int si, fd, gd;
-                if (hap[k] != op.major) {
+                if (a_here != op.major) {
                     last_rare = k;
                     si = ref + vi; fd = ref + vf; gd = ref + vg;
                     if (gd <= fd) { fd = ref; gd = ns; last_reset = k; }
@@ -470,6 +567,8 @@ __global__ __launch_bounds__(64) void pbwt_walk_kernel(WalkArgs w) {
             }
         }
     }
+    }   // tiles
+    if (!act) return;
     w.ev_cnt[h] = nev;
     if (overflow) atomicExch(w.status, 2);
 }
@@ -486,6 +585,7 @@ struct CompactArgs {
     int out_stride;
     int32_t *out_K;             // [n_samples]
     int32_t *ins;               // [n_samples][out_stride] or NULL
+    int32_t *ins_site;          // [n_samples][out_stride] or NULL
 };
 
 __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
@@ -501,6 +601,7 @@ __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
     const int kp = c.kpbwt[t];
     const int orow = c.sample_row ? c.sample_row[t] : t;
     int32_t *ins = c.ins ? c.ins + (size_t)orow * c.out_stride : nullptr;
+    int32_t *ins_site = c.ins_site ? c.ins_site + (size_t)orow * c.out_stride : nullptr;
     if (lane == 0) {
         int count = 0;
         for (int o = 0; o < c.depth && count < kp; o++) {
@@ -523,6 +624,7 @@ __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
                     if (!(bm[x >> 6] & m)) {
                         bm[x >> 6] |= m;
                         if (ins) ins[count] = x | (o << 20) | (fromB << 26) | (1 << 27);
+                        if (ins_site) ins_site[count] = r[0] >> 1;
                         count++;
                     }
                 }
@@ -532,6 +634,7 @@ __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
                     if (!(bm[x >> 6] & m)) {
                         bm[x >> 6] |= m;
                         if (ins) ins[count] = x | (o << 20) | (fromB << 26);
+                        if (ins_site) ins_site[count] = r[0] >> 1;
                         count++;
                     }
                 }
@@ -577,7 +680,19 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     w.haps = R.haps; w.hap_row = R.hap_row; w.L = P->L; w.n_haps = R.n_haps; w.start = R.start;
     w.stored = R.stored; w.hap_class = R.hap_class; w.modulo = R.modulo; w.Lhq = P->Lhq;
     w.depth = R.depth; w.ev = R.ev; w.cap = R.cap; w.ev_cnt = R.ev_cnt; w.status = R.status;
-    hipLaunchKernelGGL(pbwt_walk_kernel, dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
+    w.tile_op = D->tiles.as<int32_t>(); w.n_tiles = D->n_tiles; w.total_words = D->total_words;
+    if (D->n_tiles > 0) {
+        static const bool attr = [] {
+            GH_HIP(hipFuncSetAttribute((const void *)pbwt_walk_kernel<true>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)WALK_LDS));
+            return true;
+        }();
+        (void)attr;
+        hipLaunchKernelGGL(pbwt_walk_kernel<true>, dim3((R.n_haps + WALK_NT - 1) / WALK_NT), dim3(WALK_NT), WALK_LDS,
+                           s, w);
+    } else {
+        hipLaunchKernelGGL(pbwt_walk_kernel<false>, dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
+    }
     GH_HIP(hipGetLastError());
     if (R.out_states) {
         CompactArgs c;
@@ -585,6 +700,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
         c.kpbwt = R.kpbwt; c.sample_hap = R.sample_hap; c.sample_row = R.sample_row; c.out_states = R.out_states; c.out_stride = R.out_stride;
         c.out_K = R.out_K;
         c.ins = R.ins;
+        c.ins_site = R.ins_site;
         const size_t shm = sizeof(uint64_t) * ((P->N + 63) / 64);
         hipLaunchKernelGGL(pbwt_compact_kernel, dim3(R.n_samples), dim3(64), shm, s, c);
         GH_HIP(hipGetLastError());

@@ -47,6 +47,7 @@ struct PbwtRun {
     // optional: the same states in insertion order, id | rank << 20 | second
     // haplotype << 26 | from the upper neighbours << 27  ([n_samples][out_stride])
     int32_t *ins;
+    int32_t *ins_site;          // optional: site of the record that listed each of them
 };
 
 void build_sparse_pbwt(const gh_panel_desc *d, std::vector<uint8_t> &Y, std::vector<int32_t> &aptr,
@@ -59,6 +60,7 @@ float pbwt_groups(gh_panel *P, int depth, float modulo_in, int kpbwt, std::vecto
 }  // namespace gh
 
 struct gh_panel::PbwtDev {
-    gh::DBuf ops, bits, cnt, A, aptr, aidx;
-    int n_ops = 0, n_runs = 0, n_arrays = 0;
+    gh::DBuf ops, bits, cnt, A, aptr, aidx, tiles;
+    int n_ops = 0, n_runs = 0, n_arrays = 0, n_tiles = 0;
+    int64_t total_words = 0;
 };

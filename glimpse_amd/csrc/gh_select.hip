@@ -276,15 +276,63 @@ __global__ __launch_bounds__(TOPK_NT) void topk_kernel(TopkArgs a) {
     const float *y = a.y + (size_t)a.row0 + (size_t)i * a.rstride;
     int n2 = 1;
     while (n2 < K) n2 <<= 1;
-    for (int j = threadIdx.x; j < n2; j += TOPK_NT)
-        s_key[j] = j < K ? (((uint64_t)ord_key(y[j]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)j)) : 0ull;
-    __syncthreads();
-    bitonic<true>(s_key, n2);
+    const bool quota = a.ins_site != nullptr;
+    if (!quota) {
+        // global ranking: (score desc, insertion order asc)
+        for (int j = threadIdx.x; j < n2; j += TOPK_NT)
+            s_key[j] = j < K ? (((uint64_t)ord_key(y[j]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)j)) : 0ull;
+        __syncthreads();
+        bitonic<true>(s_key, n2);
+        for (int p = threadIdx.x; p < n2; p += TOPK_NT)
+            s_key[p] = p < K ? (uint64_t)(0xFFFFFFFFu - (uint32_t)(s_key[p] & 0xFFFFFFFFull)) : ~0ull;
+        __syncthreads();
+    } else {
+        // rank within the listing group by (score desc, insertion asc) ...
+        const int32_t *sg = a.site_grp + (size_t)a.tcls[t] * a.L;
+        const int32_t *isite = a.ins_site + (size_t)t * a.kstride;
+        for (int j = threadIdx.x; j < n2; j += TOPK_NT) {
+            uint64_t key = ~0ull;
+            if (j < K) {
+                const uint64_t g = (uint64_t)max(sg[isite[j]], 0);
+                key = (g << 43) | ((uint64_t)(~ord_key(y[j])) << 11) | (uint64_t)j;
+            }
+            s_key[j] = key;
+        }
+        __syncthreads();
+        bitonic<false>(s_key, n2);
+        // ... then serve the groups round robin: (rank in group, group, insertion)
+        uint64_t nk[TOPK_MAX / TOPK_NT];
+#pragma unroll
+        for (int q = 0; q < TOPK_MAX / TOPK_NT; q++) {
+            const int p = threadIdx.x + q * TOPK_NT;
+            nk[q] = ~0ull;
+            if (p < K) {
+                const uint64_t x = s_key[p], g = x >> 43;
+                int lo = 0, hi = p;   // first position of group g
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((s_key[mid] >> 43) < g) lo = mid + 1; else hi = mid;
+                }
+                nk[q] = ((uint64_t)(p - lo) << 43) | (g << 11) | (x & 0x7FFull);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TOPK_MAX / TOPK_NT; q++) {
+            const int p = threadIdx.x + q * TOPK_NT;
+            if (p < n2) s_key[p] = nk[q];
+        }
+        __syncthreads();
+        bitonic<false>(s_key, n2);
+        for (int p = threadIdx.x; p < n2; p += TOPK_NT) s_key[p] = p < K ? (s_key[p] & 0x7FFull) : ~0ull;
+        __syncthreads();
+    }
+    // s_key[p] = insertion index of the p-th kept candidate (p < K)
     float top = 0.0f, all = 0.0f;
-    for (int j = threadIdx.x; j < K; j += TOPK_NT) {
-        const float m = fmaxf(expf(key_score((uint32_t)(s_key[j] >> 32))) - LABEL_EPS, 0.0f);
+    for (int p = threadIdx.x; p < K; p += TOPK_NT) {
+        const float m = fmaxf(expf(y[(int)s_key[p]]) - LABEL_EPS, 0.0f);
         all += m;
-        top += j < k ? m : 0.0f;
+        top += p < k ? m : 0.0f;
     }
     top = block_sum(top, red);
     all = block_sum(all, red);
@@ -296,8 +344,8 @@ __global__ __launch_bounds__(TOPK_NT) void topk_kernel(TopkArgs a) {
     if (keep_pool) return;   // plain PBWT selection stays
     const int32_t *ins = a.ins + (size_t)t * a.kstride;
     __syncthreads();
-    for (int j = threadIdx.x; j < n2; j += TOPK_NT)
-        s_key[j] = j < k ? (uint64_t)(ins[0xFFFFFFFFu - (uint32_t)(s_key[j] & 0xFFFFFFFFull)] & 0xFFFFF) : ~0ull;
+    for (int p = threadIdx.x; p < n2; p += TOPK_NT)
+        s_key[p] = p < k ? (uint64_t)(ins[(int)s_key[p]] & 0xFFFFF) : ~0ull;
     __syncthreads();
     bitonic<false>(s_key, n2);
     int32_t *st = a.states + (size_t)t * a.kstride;
@@ -456,6 +504,7 @@ void select_topk_batch(int T, const float *y_h, const int32_t *ins_h, const int3
     GH_HIP(hipMemsetAsync(cov.p, 0, sizeof(float) * T, s));
     a.counters = (int32_t *)cnt.reserve(sizeof(int32_t) * 2);
     GH_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * 2, s));
+    a.ins_site = nullptr; a.site_grp = nullptr; a.tcls = nullptr; a.L = 0;
     topk_launch(T, a, s);
     GH_HIP(hipMemcpyAsync(states_h, st.p, sizeof(int32_t) * (size_t)T * kstride, hipMemcpyDeviceToHost, s));
     GH_HIP(hipMemcpyAsync(Kout_h, K.p, sizeof(int32_t) * (size_t)T, hipMemcpyDeviceToHost, s));

@@ -57,6 +57,12 @@ struct TopkArgs {
     CondItem *items;           // [T]
     float *coverage;           // [T] predicted mass share of the kept states
     int32_t *counters;         // [0] learned selections, [1] kept pools
+    // group quota (when ins_site != NULL): candidates compete within the PBWT
+    // group of the site that listed them; groups are served round robin
+    const int32_t *ins_site;   // [T][kstride]
+    const int32_t *site_grp;   // [n_classes][L] PBWT group of a site (-1 if none)
+    const int32_t *tcls;       // [T] class of each target
+    int L;
 };
 
 struct LabelArgs {

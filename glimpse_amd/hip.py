@@ -78,7 +78,7 @@ def lib():
         L.gh_mlp_train_run.argtypes = [vp, vp, i32, i32, f32, u32, vp]
         L.gh_run_collect.argtypes = [vp, i32]
         L.gh_run_collected.argtypes = [vp, vp, vp, vp, vp]
-        L.gh_run_set_selector.argtypes = [vp, vp, vp, f32]
+        L.gh_run_set_selector.argtypes = [vp, vp, vp, f32, i32]
         L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, i32, vp]
         L.gh_select_topk.argtypes = [i32, vp, vp, vp, vp, i32, f32, vp, vp, vp]
         _lib = L
@@ -323,15 +323,17 @@ class Run:
         _check(lib().gh_run_collected(self.h, ctypes.byref(n), _p(X), _p(y), _p(w)))
         return X, y, w
 
-    def set_selector(self, mlp, k_learn, min_mass=0.95):
+    def set_selector(self, mlp, k_learn, min_mass=0.95, mode=1):
         """Use the learned selector: target t keeps k_learn[t] states of its PBWT
-        pool, or the whole pool when their predicted mass share < min_mass."""
+        pool, or the whole pool when their predicted mass share < min_mass.
+        mode 0: best scores of the pool; mode 1: best scores per PBWT group,
+        groups served round robin."""
         if mlp is None:
-            _check(lib().gh_run_set_selector(self.h, None, None, 0.0))
+            _check(lib().gh_run_set_selector(self.h, None, None, 0.0, 0))
             self._mlp = None
             return
         k = np.ascontiguousarray(np.broadcast_to(np.asarray(k_learn, np.int32), (self.T,)), np.int32)
-        _check(lib().gh_run_set_selector(self.h, mlp.h, _p(k), float(min_mass)))
+        _check(lib().gh_run_set_selector(self.h, mlp.h, _p(k), float(min_mass), int(mode)))
         self._mlp = mlp   # keep the network alive while the run may use it
 
     def fetch(self):

@@ -211,9 +211,13 @@ int gh_run_collect(gh_run *r, int32_t from_iter);
 int gh_run_collected(gh_run *r, int64_t *n_rows, float *X, float *y, float *w);
 /* Before gh_run_execute: use the learned selector from iteration 1 on.  Target
  * t keeps k_learn[t] states of its pool, or the whole pool when the predicted
- * mass share is below min_mass (0.95).  m = NULL switches it off.  The network
- * must not be trained while the run executes. */
-int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass);
+ * mass share is below min_mass (0.95).  mode 0 keeps the k best scores of the
+ * pool; mode 1 ranks candidates within the PBWT group of the stored site that
+ * listed them and serves the groups round robin (rank 1 of every group, then
+ * rank 2, ...), which keeps the plain method's coverage along the chunk.
+ * m = NULL switches it off.  The network must not be trained while the run
+ * executes. */
+int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass, int32_t mode);
 
 /* The selector's two steps as batch calls (what a run does between the PBWT
  * compaction and the conditioning sets).  pool [T][kstride]: candidates in PBWT
