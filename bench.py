@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--k-frac", type=float, default=0.2, help="learned path: kept fraction of the PBWT pool")
     ap.add_argument("--min-mass", type=float, default=0.0,
                     help="learned path: keep the whole pool below this predicted mass share")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of 1 GiB three times (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on exactly one GPU (the Gibbs chains are independent per target and "
@@ -216,6 +218,16 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.synchronize()   # bring torch's HIP runtime up before the library's first call
+    if a.copy_calibration:
+        n = 1 << 28   # 2^28 floats = 1 GiB read + 1 GiB written per copy
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
+        return
     from glimpse_amd import hip
 
     t_setup = time.perf_counter()

@@ -28,6 +28,8 @@ void run_collect(gh_run *G, int from_iter);
 void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w);
 void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss);
 void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode);
+void output_fields(int T, int L, const float *gp0, const float *gp1, const uint8_t *hds, const uint32_t *cref,
+                   const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info, float *af, float *raf);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -431,9 +433,15 @@ int gh_run_collected(gh_run *r, int64_t *n_rows, float *X, float *y, float *w) {
 int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass, int32_t mode) {
     return guard([&] { run_set_selector(r, m, k_learn, min_mass, mode); });
 }
+int gh_output_fields(int32_t T, int32_t L, const float *gp0, const float *gp1, const uint8_t *hds,
+                     const uint32_t *cref, const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info,
+                     float *af, float *raf) {
+    return guard([&] { output_fields(T, L, gp0, gp1, hds, cref, calt, ds, gp, gt, info, af, raf); });
+}
 int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *gl, const uint8_t *flat,
-                       const int32_t *pool, const int32_t *K, int32_t kstride, int32_t depth, float *X) {
-    return guard([&] { select_features_batch(p, T, H, gl, flat, pool, K, kstride, depth, X); });
+                       const int32_t *pool, const int32_t *pool_site, const int32_t *K, int32_t kstride, int32_t depth,
+                       float *X) {
+    return guard([&] { select_features_batch(p, T, H, gl, flat, pool, pool_site, K, kstride, depth, X); });
 }
 int gh_select_topk(int32_t T, const float *scores, const int32_t *pool, const int32_t *K, const int32_t *k_learn,
                    int32_t kstride, float min_mass, int32_t *states, int32_t *K_out, float *coverage) {

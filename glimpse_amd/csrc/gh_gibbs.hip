@@ -260,7 +260,7 @@ struct gh_run {
     int64_t col_rows = 0;
     bool masks_ready = false;
     gh::DBuf d_klearn, d_ins, d_fx, d_fy, d_cov, d_cnt, d_tmask, d_mass0, d_mass1, d_cx, d_cy, d_cw, d_isite,
-        d_sgrp, d_tcls;
+        d_sgrp, d_tcls, d_wlo, d_whi;
 };
 
 namespace gh {
@@ -475,6 +475,8 @@ void run_execute(gh_run *G) {
     if (collecting) {
         G->d_mass0.reserve(sizeof(float) * (size_t)T * kst);
         G->d_mass1.reserve(sizeof(float) * (size_t)T * kst);
+        G->d_wlo.reserve(sizeof(int32_t) * (size_t)T * kst);
+        G->d_whi.reserve(sizeof(int32_t) * (size_t)T * kst);
     }
     G->col_rows = 0;
     PbwtRun R = G->R;
@@ -557,7 +559,8 @@ void run_execute(gh_run *G) {
                 FeatArgs fa;
                 fa.hm = P->d_hm.as<uint64_t>(); fa.Wc = P->Wc; fa.Lc = P->Lc; fa.N = N; fa.L = L;
                 fa.depth = prm->pbwt_depth; fa.kstride = kst; fa.rstride = prs;
-                fa.c2t = P->d_c2t.as<int32_t>(); fa.flags = P->d_flags.as<uint8_t>();
+                fa.c2t = P->d_c2t.as<int32_t>(); fa.crow = P->d_crow.as<int32_t>(); fa.flags = P->d_flags.as<uint8_t>();
+                fa.ins_site = G->d_isite.as<int32_t>();
                 fa.rare_ptr = P->d_rare_ptr.as<int32_t>(); fa.rare_idx = P->d_rare_idx.as<int32_t>();
                 fa.H = H; fa.tmask = G->d_tmask.as<uint64_t>(); fa.targets = ptg; fa.Kpool = Kdev;
                 fa.ins = G->d_ins.as<int32_t>();
@@ -578,6 +581,14 @@ void run_execute(gh_run *G) {
                     fa.row0 = G->col_rows;
                 }
                 feature_launch(P, npt, fa, s);
+                if (col_it) {
+                    WinArgs wa;
+                    wa.targets = ptg; wa.Kpool = Kdev; wa.states = states; wa.ins = G->d_ins.as<int32_t>();
+                    wa.ins_site = G->d_isite.as<int32_t>(); wa.c2t = P->d_c2t.as<int32_t>();
+                    wa.crow = P->d_crow.as<int32_t>(); wa.Lc = P->Lc; wa.Wc = P->Wc; wa.kstride = kst;
+                    wa.win_lo = G->d_wlo.as<int32_t>(); wa.win_hi = G->d_whi.as<int32_t>();
+                    win_launch(npt, wa, s);
+                }
                 if (learned) {
                     mlp_forward_launch(G->mlp->w.as<float>(), G->mlp->wb.as<uint16_t>(), G->d_fx.as<float>(),
                                        (int64_t)npt * prs, G->d_fy.as<float>(), s);
@@ -608,19 +619,22 @@ void run_execute(gh_run *G) {
         condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, maxW_it, s);
         const bool label_it = collecting && stage > 0 && npt > 0 && it >= G->collect_from;
         float *mass0 = label_it ? G->d_mass0.as<float>() : nullptr, *mass1 = label_it ? G->d_mass1.as<float>() : nullptr;
+        const int32_t *wlo = label_it ? G->d_wlo.as<int32_t>() : nullptr, *whi = label_it ? G->d_whi.as<int32_t>() : nullptr;
         GH_HIP(hipEventRecord(e_cond.b, s));
         GH_HIP(hipEventRecord(e_fb.a, s));
         // H0 (caller_algorithm.cpp:58-65)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k0.a, s));
-        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP0, G->ee_i, G->ed_i, s, mass0);
+        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP0, G->ee_i, G->ed_i, s, mass0, wlo,
+                  whi);
         GH_HIP(hipEventRecord(e_k0.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
         // H1 (caller_algorithm.cpp:66-71)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k1.a, s));
-        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i, G->ed_i, s, mass1);
+        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i, G->ed_i, s, mass1, wlo,
+                  whi);
         GH_HIP(hipEventRecord(e_k1.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
@@ -629,6 +643,7 @@ void run_execute(gh_run *G) {
         if (label_it) {
             LabelArgs la;
             la.targets = ptg; la.items = items_d; la.states = states; la.ins = G->d_ins.as<int32_t>();
+            la.poly = poly; la.win_lo = wlo; la.win_hi = whi;
             la.mass0 = mass0; la.mass1 = mass1; la.kstride = kst; la.rstride = prs;
             la.X = G->d_cx.as<float>(); la.Y = G->d_cy.as<float>(); la.W = G->d_cw.as<float>();
             la.row0 = G->col_rows;

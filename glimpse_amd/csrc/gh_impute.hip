@@ -45,6 +45,7 @@ struct FbArgs {
     float *ckpt_all;
     float *HP_all;
     float *mass_all;   // optional: per-state posterior mass (summed over sites)
+    const int32_t *win_lo, *win_hi;   // optional with mass_all: per-state site window [lo, hi) of the sum
     float ee, ed;
     int C;             // sites per checkpoint block (divides FB_TILE)
     int maxW;          // LDS row stride bound
@@ -199,6 +200,16 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
 
     // ---------------- backward over checkpoint blocks, alpha recomputed into LDS
     float b[MWL], ms[MWL];
+    int wlo[MWL], whi[MWL];
+#pragma unroll
+    for (int i = 0; i < MWL; i++) {
+        wlo[i] = 0;
+        whi[i] = 0x7FFFFFFF;
+        if (A.win_lo && valid[i]) {
+            wlo[i] = A.win_lo[c.state_off + 64 * wid[i] + lane];
+            whi[i] = A.win_hi[c.state_off + 64 * wid[i] + lane];
+        }
+    }
 #pragma unroll
     for (int i = 0; i < MWL; i++) { b[i] = 0.0f; ms[i] = 0.0f; }
     float bsum_next = 1.0f;
@@ -267,7 +278,11 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                 if (A.mass_all) {
                     const float inv = 1.0f / (p0 + p1);
 #pragma unroll
-                    for (int i = 0; i < MWL; i++) ms[i] = fmaf(dpv[i], inv, ms[i]);
+                    for (int i = 0; i < MWL; i++) {
+                        const int site = rli(m_site, j);
+                        const bool in = site >= wlo[i] && site < whi[i];
+                        ms[i] = in ? fmaf(dpv[i], inv, ms[i]) : ms[i];
+                    }
                 }
                 if (!fl) { p0 /= e0; p1 /= e1; }
                 float o0 = p0 * ee + p1 * ed, o1 = p0 * ed + p1 * ee;
@@ -346,14 +361,16 @@ static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {
 
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
-               float *d_HP, float ee, float ed, hipStream_t s, float *d_mass) {
+               float *d_HP, float ee, float ed, hipStream_t s, float *d_mass, const int32_t *d_win_lo,
+               const int32_t *d_win_hi) {
     if (n <= 0) return;
     GH_CHECK(maxW <= 32, GH_ERR_ARG, "fb: more than 2048 conditioning states per haplotype is not supported");
     const FbShape sh = fb_shape(maxW);
     FbArgs A;
     A.fb = d_fb; A.cond = d_cond; A.poly_all = d_poly; A.t_all = d_t; A.hvar_all = d_hvar; A.HL_all = d_HL;
     A.flat_all = d_flat; A.flags = P->d_flags.as<uint8_t>(); A.ckpt_all = d_ckpt; A.HP_all = d_HP;
-    A.mass_all = d_mass; A.ee = ee; A.ed = ed; A.C = sh.C; A.maxW = maxW;
+    A.mass_all = d_mass; A.win_lo = d_win_lo; A.win_hi = d_win_hi;
+    A.ee = ee; A.ed = ed; A.C = sh.C; A.maxW = maxW;
     if (sh.NW == 1) fb_go<1, 4>(A, n, sh.smem, s);
     else if (sh.NW == 2) fb_go<2, 4>(A, n, sh.smem, s);
     else if (sh.MWL == 4) fb_go<4, 4>(A, n, sh.smem, s);

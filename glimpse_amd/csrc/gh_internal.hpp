@@ -110,7 +110,9 @@ void condset_build(gh_panel *P, int n, CondItem *d_items, const CondItem *h_item
 
 // FB over n haploid items.  HL [item][2L], flat [item][L] (device); alpha
 // checkpoints per item: ceil(Lp/C)*64W floats at alpha_off[i], C = fb_block_sites(maxW).
-// HP out [item][2L]; optional per-state posterior mass at mass + state_off.
+// HP out [item][2L]; optional per-state posterior mass at mass + state_off,
+// summed over the polymorphic sites inside the state's [win_lo, win_hi) when
+// those per-state windows are given (at win + state_off), else over all sites.
 struct FbItem {
     int32_t K, W, Lp, cond;   // cond = index of the CondItem
     int64_t alpha_off;        // floats
@@ -119,7 +121,8 @@ struct FbItem {
 };
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
-               float *d_HP, float ee, float ed, hipStream_t s, float *d_mass = nullptr);
+               float *d_HP, float ee, float ed, hipStream_t s, float *d_mass = nullptr,
+               const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr);
 int fb_block_sites(int maxW);
 // HP at monomorphic sites (closed form) for the same items.
 void fb_mono_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_pos,

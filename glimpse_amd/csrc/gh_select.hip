@@ -159,69 +159,135 @@ __global__ __launch_bounds__(FEAT_NT) void feature_kernel(FeatArgs a) {
     const int tail = a.Lc - 64 * (Wc - 1);   // valid bits of the last word
     const uint64_t lastm = tail >= 64 ? ~0ull : ((1ull << tail) - 1ull);
     const int32_t *ins = a.ins + (size_t)t * a.kstride;
+    const int32_t *isite = a.ins_site + (size_t)t * a.kstride;
     float *Xo = a.X + ((size_t)a.row0 + (size_t)i * a.rstride) * MLP_FEAT;
     const float inv_k = 1.0f / (float)max(K, 1);
+    // sites of common words [lo, hi) clipped to the chunk
+    auto wsites = [&](int lo, int hi) -> float {
+        lo = max(lo, 0);
+        hi = min(hi, Wc);
+        return (float)max(min(64 * hi, a.Lc) - 64 * lo, 0);
+    };
     for (int j = wv; j < K; j += nw) {
         const int v = ins[j];
         const int id = v & 0xFFFFF;
+        const int cw = max(a.crow[isite[j]], 0) >> 6;
         const uint64_t *row = a.hm + (size_t)id * Wc;
         float f[MLP_FEAT];
         f[0] = (float)((v >> 20) & 63) / (float)a.depth;
         f[1] = (float)j * inv_k;
         f[2] = (float)((v >> 26) & 1);
         f[3] = (float)((v >> 27) & 1);
-        float mD0 = 0, mD1 = 0, cRR = 0, cAA = 0, cRA = 0, alt = 0;
-        float all0 = 0, all1 = 0;
+        float g0 = 0, g1 = 0, gD0 = 0, gD1 = 0;
+        float nb0 = 0, nb1 = 0, nf0 = 0, nf1 = 0, fb0 = 0, fb1 = 0, ff0 = 0, ff1 = 0;
+        float lD0 = 0, lD1 = 0, lRR = 0, lAA = 0, lRA = 0, lalt = 0, nD = 0, nRR = 0, nAA = 0, nRA = 0;
         RunState run0, run1;
-#pragma unroll
-        for (int win = 0; win < FEAT_WIN; win++) {
-            const int ws = (win * Wc) / FEAT_WIN, we = ((win + 1) * Wc) / FEAT_WIN;
-            float m0 = 0, m1 = 0;
-            for (int cw = ws; cw < we; cw += 64) {
-                const int w = cw + lane;
-                const bool valid = w < we;
-                uint64_t x0 = 1, x1 = 1;
-                if (valid) {
-                    const uint64_t vm = w == Wc - 1 ? lastm : ~0ull;
-                    const uint64_t R = row[w] & vm;
-                    x0 = (R ^ s_h0[w]) & vm;
-                    x1 = (R ^ s_h1[w]) & vm;
-                    const uint64_t D = s_d[w];
-                    m0 += __popcll(x0);
-                    m1 += __popcll(x1);
-                    mD0 += __popcll(x0 & D);
-                    mD1 += __popcll(x1 & D);
-                    cRR += __popcll(R & s_rr[w]);
-                    cAA += __popcll(~R & s_aa[w] & vm);
-                    cRA += __popcll(R & s_ra[w]);
-                    alt += __popcll(R);
+        for (int c0 = 0; c0 < Wc; c0 += 64) {
+            const int w = c0 + lane;
+            const bool valid = w < Wc;
+            uint64_t x0 = 1, x1 = 1;
+            if (valid) {
+                const uint64_t vm = w == Wc - 1 ? lastm : ~0ull;
+                const uint64_t R = row[w] & vm;
+                x0 = (R ^ s_h0[w]) & vm;
+                x1 = (R ^ s_h1[w]) & vm;
+                const uint64_t D = s_d[w];
+                const float p0 = (float)__popcll(x0), p1 = (float)__popcll(x1);
+                g0 += p0;
+                g1 += p1;
+                gD0 += __popcll(x0 & D);
+                gD1 += __popcll(x1 & D);
+                const int d = w - cw;
+                if (d >= -FEAT_FAR && d < FEAT_FAR) {
+                    lD0 += __popcll(x0 & D);
+                    lD1 += __popcll(x1 & D);
+                    lRR += __popcll(R & s_rr[w]);
+                    lAA += __popcll(~R & s_aa[w] & vm);
+                    lRA += __popcll(R & s_ra[w]);
+                    lalt += __popcll(R);
+                    nD += __popcll(D);
+                    nRR += __popcll(s_rr[w]);
+                    nAA += __popcll(s_aa[w]);
+                    nRA += __popcll(s_ra[w]);
+                    if (d < 0) {
+                        fb0 += p0; fb1 += p1;
+                        if (d >= -FEAT_NEAR) { nb0 += p0; nb1 += p1; }
+                    } else {
+                        ff0 += p0; ff1 += p1;
+                        if (d < FEAT_NEAR) { nf0 += p0; nf1 += p1; }
+                    }
                 }
-                const int n = min(64, we - cw);
-                run0.push(__ballot(valid && x0 == 0ull), n);
-                run1.push(__ballot(valid && x1 == 0ull), n);
             }
-            const float nsite = (float)max(min(64 * we, a.Lc) - 64 * ws, 1);
-            m0 = wave_sum(m0);
-            m1 = wave_sum(m1);
-            all0 += m0;
-            all1 += m1;
-            f[16 + win] = m0 / nsite;
-            f[24 + win] = m1 / nsite;
+            const int n = min(64, Wc - c0);
+            run0.push(__ballot(valid && x0 == 0ull), n);
+            run1.push(__ballot(valid && x1 == 0ull), n);
         }
-        wave_sum3(mD0, mD1, cRR);
-        wave_sum3(cAA, cRA, alt);
-        f[4] = mD0 * inv_d;
-        f[5] = mD1 * inv_d;
-        f[6] = all0 * inv_lc;
-        f[7] = all1 * inv_lc;
+        // identical words running back from cw-1 and forward from cw (up to FEAT_RUN)
+        int back0, back1, fwd0, fwd1;
+        {
+            int w = cw - FEAT_RUN + lane;
+            bool c0 = false, c1 = false;
+            if (w >= 0 && w < Wc) {
+                const uint64_t vm = w == Wc - 1 ? lastm : ~0ull;
+                const uint64_t R = row[w] & vm;
+                c0 = ((R ^ s_h0[w]) & vm) == 0ull;
+                c1 = ((R ^ s_h1[w]) & vm) == 0ull;
+            }
+            uint64_t z0 = __ballot(c0), z1 = __ballot(c1);
+            back0 = z0 == ~0ull ? 64 : __builtin_clzll(~z0);
+            back1 = z1 == ~0ull ? 64 : __builtin_clzll(~z1);
+            w = cw + lane;
+            c0 = c1 = false;
+            if (w < Wc) {
+                const uint64_t vm = w == Wc - 1 ? lastm : ~0ull;
+                const uint64_t R = row[w] & vm;
+                c0 = ((R ^ s_h0[w]) & vm) == 0ull;
+                c1 = ((R ^ s_h1[w]) & vm) == 0ull;
+            }
+            z0 = __ballot(c0);
+            z1 = __ballot(c1);
+            fwd0 = z0 == ~0ull ? 64 : __builtin_ctzll(~z0);
+            fwd1 = z1 == ~0ull ? 64 : __builtin_ctzll(~z1);
+        }
+        wave_sum3(g0, g1, gD0);
+        wave_sum3(gD1, nb0, nb1);
+        wave_sum3(nf0, nf1, fb0);
+        wave_sum3(fb1, ff0, ff1);
+        wave_sum3(lD0, lD1, lRR);
+        wave_sum3(lAA, lRA, lalt);
+        wave_sum3(nD, nRR, nAA);
+        nRA = wave_sum(nRA);
+        const float s_nb = wsites(cw - FEAT_NEAR, cw), s_nf = wsites(cw, cw + FEAT_NEAR);
+        const float s_fb = wsites(cw - FEAT_FAR, cw), s_ff = wsites(cw, cw + FEAT_FAR);
+        const float s_seg = wsites(cw - FEAT_FAR, cw + FEAT_FAR);
+        f[4] = g0 * inv_lc;
+        f[5] = g1 * inv_lc;
+        f[6] = gD0 * inv_d;
+        f[7] = gD1 * inv_d;
         f[8] = (float)run0.best * inv_wc;
         f[9] = (float)run1.best * inv_wc;
-        f[10] = cRR * inv_rr;
-        f[11] = cAA * inv_aa;
-        f[12] = cRA * inv_ra;
-        f[13] = alt * inv_lc;
-        f[14] = (float)((bm0[id >> 6] >> (id & 63)) & 1ull);
-        f[15] = (float)((bm1[id >> 6] >> (id & 63)) & 1ull);
+        f[10] = nb0 / fmaxf(s_nb, 1.0f);
+        f[11] = nb1 / fmaxf(s_nb, 1.0f);
+        f[12] = nf0 / fmaxf(s_nf, 1.0f);
+        f[13] = nf1 / fmaxf(s_nf, 1.0f);
+        f[14] = fb0 / fmaxf(s_fb, 1.0f);
+        f[15] = fb1 / fmaxf(s_fb, 1.0f);
+        f[16] = ff0 / fmaxf(s_ff, 1.0f);
+        f[17] = ff1 / fmaxf(s_ff, 1.0f);
+        f[18] = lD0 / fmaxf(nD, 1.0f);
+        f[19] = lD1 / fmaxf(nD, 1.0f);
+        f[20] = (float)back0 * (1.0f / FEAT_RUN);
+        f[21] = (float)back1 * (1.0f / FEAT_RUN);
+        f[22] = (float)fwd0 * (1.0f / FEAT_RUN);
+        f[23] = (float)fwd1 * (1.0f / FEAT_RUN);
+        f[24] = lRR / fmaxf(nRR, 1.0f);
+        f[25] = lAA / fmaxf(nAA, 1.0f);
+        f[26] = lRA / fmaxf(nRA, 1.0f);
+        f[27] = lalt / fmaxf(s_seg, 1.0f);
+        f[28] = (float)((bm0[id >> 6] >> (id & 63)) & 1ull);
+        f[29] = (float)((bm1[id >> 6] >> (id & 63)) & 1ull);
+        f[30] = (float)cw * inv_wc;
+        f[31] = s_seg * (1.0f / (64.0f * 2 * FEAT_FAR));
         const float out = pick32(lane, f);
         if (lane < MLP_FEAT) Xo[(size_t)j * MLP_FEAT + lane] = out;
     }
@@ -358,15 +424,51 @@ __global__ __launch_bounds__(TOPK_NT) void topk_kernel(TopkArgs a) {
 }
 
 /* ------------------------------------------------------------------ labels */
+// Segment [lo, hi) in sites of a candidate listed at site s.
+__device__ __forceinline__ void seg_sites(int s, const int32_t *crow, const int32_t *c2t, int Lc, int Wc, int &lo,
+                                          int &hi) {
+    const int cw = max(crow[s], 0) >> 6;
+    const int wlo = max(cw - FEAT_FAR, 0), whi = min(cw + FEAT_FAR, Wc);
+    lo = c2t[64 * wlo];
+    hi = c2t[min(64 * whi, Lc) - 1] + 1;
+}
+
+__global__ __launch_bounds__(256) void win_kernel(WinArgs a) {
+    const int t = a.targets[blockIdx.x];
+    const int K = a.Kpool[t];
+    const int32_t *st = a.states + (size_t)t * a.kstride;
+    const int32_t *ins = a.ins + (size_t)t * a.kstride, *isite = a.ins_site + (size_t)t * a.kstride;
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        const int id = ins[j] & 0xFFFFF;
+        int lo = 0, hi = K;   // position of id in the sorted pool
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (st[mid] < id) lo = mid + 1; else hi = mid;
+        }
+        int s0, s1;
+        seg_sites(isite[j], a.crow, a.c2t, a.Lc, a.Wc, s0, s1);
+        a.win_lo[(size_t)t * a.kstride + lo] = s0;
+        a.win_hi[(size_t)t * a.kstride + lo] = s1;
+    }
+}
+
 __global__ __launch_bounds__(256) void label_kernel(LabelArgs a) {
     const int i = blockIdx.x, t = a.targets[i];
     const CondItem c = a.items[t];
     const int K = c.K;
     const int32_t *st = a.states + c.state_off;
+    const int32_t *poly = a.poly + c.site_off;
     const int32_t *ins = a.ins + (size_t)t * a.kstride;
     const float *m0 = a.mass0 + c.state_off, *m1 = a.mass1 + c.state_off;
-    const float inv = 1.0f / (2.0f * (float)max(c.Lp, 1));
     const size_t r0 = (size_t)a.row0 + (size_t)i * a.rstride;
+    auto count_below = [&](int v) {   // polymorphic sites < v
+        int lo = 0, hi = c.Lp;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (poly[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
     for (int j = threadIdx.x; j < a.rstride; j += blockDim.x) {
         float lab = 0.0f, wt = 0.0f;
         if (j < K) {
@@ -376,12 +478,15 @@ __global__ __launch_bounds__(256) void label_kernel(LabelArgs a) {
                 const int mid = (lo + hi) >> 1;
                 if (st[mid] < id) lo = mid + 1; else hi = mid;
             }
-            const float m = (m0[lo] + m1[lo]) * inv;
-            lab = __logf((float)K * m + LABEL_EPS);
-            wt = 1.0f;
-        } else {
-            for (int f = 0; f < MLP_FEAT; f++) a.X[(r0 + j) * MLP_FEAT + f] = 0.0f;
+            const int np = count_below(a.win_hi[c.state_off + lo]) - count_below(a.win_lo[c.state_off + lo]);
+            if (np > 0) {
+                const float m = (m0[lo] + m1[lo]) / (2.0f * (float)np);
+                lab = __logf((float)K * m + LABEL_EPS);
+                wt = 1.0f;
+            }
         }
+        if (wt == 0.0f)
+            for (int f = 0; f < MLP_FEAT; f++) a.X[(r0 + j) * MLP_FEAT + f] = 0.0f;
         a.Y[r0 + j] = lab;
         a.W[r0 + j] = wt;
     }
@@ -428,20 +533,31 @@ void label_launch(int n, const LabelArgs &a, hipStream_t s) {
     GH_HIP(hipGetLastError());
 }
 
+void win_launch(int n, const WinArgs &a, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(win_kernel, dim3(n), dim3(256), 0, s, a);
+    GH_HIP(hipGetLastError());
+}
+
 /* ------------------------------------------------------------------ batch entry points */
 void select_features_batch(gh_panel *P, int T, const uint8_t *H_h, const uint8_t *gl_h, const uint8_t *flat_h,
-                           const int32_t *ins_h, const int32_t *K_h, int kstride, int depth, float *X_h) {
+                           const int32_t *ins_h, const int32_t *site_h, const int32_t *K_h, int kstride, int depth,
+                           float *X_h) {
     GH_CHECK(P && T > 0 && H_h && gl_h && flat_h && ins_h && K_h && X_h && kstride > 0 && depth > 0, GH_ERR_ARG,
              "gh_select_features: bad arguments");
     GH_CHECK(P->N < (1 << 20), GH_ERR_ARG, "gh_select_features: at most 2^20 panel haplotypes");
     for (int t = 0; t < T; t++) {
         GH_CHECK(K_h[t] >= 0 && K_h[t] <= kstride, GH_ERR_ARG, "gh_select_features: K out of range");
-        for (int j = 0; j < K_h[t]; j++)
+        for (int j = 0; j < K_h[t]; j++) {
             GH_CHECK((ins_h[(size_t)t * kstride + j] & 0xFFFFF) < P->N, GH_ERR_ARG, "gh_select_features: bad id");
+            const int s = site_h[(size_t)t * kstride + j];
+            GH_CHECK(s >= 0 && s < P->L && P->flags[s] & SF_COMMON, GH_ERR_ARG,
+                     "gh_select_features: listing sites must be common sites");
+        }
     }
     hipStream_t s = P->stream;
     const int L = P->L;
-    DBuf H, gl, flat, ins, K, X, tm, tg;
+    DBuf H, gl, flat, ins, isite, K, X, tm, tg;
     std::vector<int32_t> tv(T);
     for (int t = 0; t < T; t++) tv[t] = t;
     FeatArgs a;
@@ -457,6 +573,8 @@ void select_features_batch(gh_panel *P, int T, const uint8_t *H_h, const uint8_t
     a.targets = dalloc_copy(tg, tv.data(), (size_t)T, s);
     a.Kpool = dalloc_copy(K, K_h, (size_t)T, s);
     a.ins = dalloc_copy(ins, ins_h, (size_t)T * kstride, s);
+    a.ins_site = dalloc_copy(isite, site_h, (size_t)T * kstride, s);
+    a.crow = P->d_crow.as<int32_t>();
     X.reserve(sizeof(float) * (size_t)T * kstride * MLP_FEAT);
     GH_HIP(hipMemsetAsync(X.p, 0, sizeof(float) * (size_t)T * kstride * MLP_FEAT, s));
     a.X = X.as<float>();

@@ -79,8 +79,9 @@ def lib():
         L.gh_run_collect.argtypes = [vp, i32]
         L.gh_run_collected.argtypes = [vp, vp, vp, vp, vp]
         L.gh_run_set_selector.argtypes = [vp, vp, vp, f32, i32]
-        L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, i32, vp]
+        L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
         L.gh_select_topk.argtypes = [i32, vp, vp, vp, vp, i32, f32, vp, vp, vp]
+        L.gh_output_fields.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -220,10 +221,11 @@ class Panel:
                                       _p(b), ctypes.c_float(err_phase), ne, seed, it, _p(tg)))
         return a, b
 
-    def select_features(self, H, gl, flat, pool, K, depth=12):
+    def select_features(self, H, gl, flat, pool, pool_site, K, depth=12):
         """Learned-selector features (gh_select_features): H [2T][L], gl [T][L][3],
-        flat [T][L], pool [T][kstride] (insertion-order codes), K [T] ->
-        X [T][kstride][32]."""
+        flat [T][L], pool [T][kstride] (insertion-order codes), pool_site
+        [T][kstride] (listing sites), K [T] -> X [T][kstride][32]."""
+        pool_site = np.ascontiguousarray(pool_site, np.int32)
         H = np.ascontiguousarray(H, np.uint8)
         gl = np.ascontiguousarray(gl, np.uint8)
         flat = np.ascontiguousarray(flat, np.uint8)
@@ -231,7 +233,8 @@ class Panel:
         K = np.ascontiguousarray(K, np.int32)
         T, ks = pool.shape
         X = np.zeros((T, ks, MLP_IN), np.float32)
-        _check(lib().gh_select_features(self.h, T, _p(H), _p(gl), _p(flat), _p(pool), _p(K), ks, int(depth), _p(X)))
+        _check(lib().gh_select_features(self.h, T, _p(H), _p(gl), _p(flat), _p(pool), _p(pool_site), _p(K), ks,
+                                        int(depth), _p(X)))
         return X
 
     def pbwt_select(self, hap_alleles, stored, start_idx, depth=12, modulo=0.1, cap=None, kpbwt=0):
@@ -269,6 +272,21 @@ class Panel:
                                   _p(out["nstates"]), ctypes.byref(st)))
         out["stats"] = {n: getattr(st, n) for n, _ in RunStats._fields_}
         return out
+
+
+def output_fields(gp0, gp1, hds, cref, calt):
+    """gh_output_fields: the writer's DS, GP, GT and per-site INFO / AF / RAF."""
+    gp0 = np.ascontiguousarray(gp0, np.float32)
+    gp1 = np.ascontiguousarray(gp1, np.float32)
+    hds = np.ascontiguousarray(hds, np.uint8)
+    cref = np.ascontiguousarray(cref, np.uint32)
+    calt = np.ascontiguousarray(calt, np.uint32)
+    T, L = gp0.shape
+    out = dict(ds=np.zeros((T, L), np.float32), gp=np.zeros((T, L, 3), np.float32), gt=np.zeros((T, L, 2), np.uint8),
+               info=np.zeros(L, np.float32), af=np.zeros(L, np.float32), raf=np.zeros(L, np.float32))
+    _check(lib().gh_output_fields(T, L, _p(gp0), _p(gp1), _p(hds), _p(cref), _p(calt), _p(out["ds"]), _p(out["gp"]),
+                                  _p(out["gt"]), _p(out["info"]), _p(out["af"]), _p(out["raf"])))
+    return out
 
 
 def select_topk(scores, pool, K, k_learn, states, min_mass=0.95):

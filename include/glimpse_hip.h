@@ -165,6 +165,20 @@ int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, u
                  gh_run_stats *stats);
 int gh_run_destroy(gh_run *r);
 
+/* Output fields as genotype_writer::writeGenotypes computes them
+ * (phase/src/io/genotype_writer.cpp:109-190), bit for bit, for T diploid
+ * targets at L sites from the finalized run outputs gp0, gp1, hds [T][L] and
+ * the panel's allele counts cref, calt [L]:
+ *   ds [T][L]   = round(gp1 + 2 gp2, 1e-3)
+ *   gp [T][L][3] floored to 1e-3, then the largest remainders raised by 1e-3
+ *                 until the three sum to >= 0.9999
+ *   gt [T][L][2] phased alleles (het: the alt allele on haplotype hds)
+ *   info, af, raf [L]: INFO score (rounded to 1e-3), alt frequency of the
+ *                 targets' dosages, alt frequency of the panel. */
+int gh_output_fields(int32_t T, int32_t L, const float *gp0, const float *gp1, const uint8_t *hds,
+                     const uint32_t *cref, const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info,
+                     float *af, float *raf);
+
 /* ---------------------------------------------------------------------------
  * Learned state selector.  No counterpart in the reference: it is an addition
  * beside conditioning_set::compactSelection's Kpbwt branch
@@ -205,7 +219,8 @@ int gh_mlp_train_run(gh_mlp *m, gh_run *r, int32_t epochs, int32_t batch, float 
 /* Before gh_run_execute: record a training row (features, label, weight) for
  * every PBWT pool candidate of every target at iterations >= from_iter (>= 1);
  * the label is ln(K * m + 1e-3) with m the candidate's posterior mass from both
- * haplotypes' forward-backward, averaged over polymorphic sites. */
+ * haplotypes' forward-backward, averaged over the polymorphic sites of its
+ * segment (the 2048 common sites around the stored site that listed it). */
 int gh_run_collect(gh_run *r, int32_t from_iter);
 /* Copies the recorded rows (NULL buffers: only *n_rows). */
 int gh_run_collected(gh_run *r, int64_t *n_rows, float *X, float *y, float *w);
@@ -222,10 +237,12 @@ int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_
 /* The selector's two steps as batch calls (what a run does between the PBWT
  * compaction and the conditioning sets).  pool [T][kstride]: candidates in PBWT
  * insertion order, each id | rank << 20 | (listed by H1's walk) << 26 |
- * (upper neighbour) << 27, K[t] of them.  H [2T][L], gl [T][L][3], flat [T][L].
- * Features X [T][kstride][32] (rows >= K[t] are zero). */
+ * (upper neighbour) << 27, K[t] of them; pool_site [T][kstride]: the common
+ * site whose PBWT record listed each candidate.  H [2T][L], gl [T][L][3],
+ * flat [T][L].  Features X [T][kstride][32] (rows >= K[t] are zero). */
 int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *gl, const uint8_t *flat,
-                       const int32_t *pool, const int32_t *K, int32_t kstride, int32_t depth, float *X);
+                       const int32_t *pool, const int32_t *pool_site, const int32_t *K, int32_t kstride,
+                       int32_t depth, float *X);
 /* Top k_learn[t] of the pool by score (ties: earlier insertion first); states
  * [T][kstride] holds the sorted pool on input and the sorted selection on
  * output, K_out[t] its size; when k_learn[t] >= K[t] or the kept candidates'

@@ -1807,4 +1807,58 @@ uint32_t orc_mt19937_nth(uint32_t seed, int64_t n) {
 
 float orc_unphred(int32_t i) { return UNPHRED[i & 255]; }
 
+/* genotype_writer::writeGenotypes per-site fields for diploid targets
+ * (phase/src/io/genotype_writer.cpp:109-190): DS, GP with the writer's
+ * floor-then-raise rounding, phased GT, INFO / AF / RAF.  A target whose
+ * genotype is not stored at a site enters as gp = (1, 0). */
+int orc_output_fields(int32_t T, int32_t L, const float *gp0s, const float *gp1s, const uint8_t *hdss,
+                      const uint32_t *cref, const uint32_t *calt, float *ds_out, float *gp_out, uint8_t *gt_out,
+                      float *info_out, float *af_out, float *raf_out) {
+    ORC_TRY(
+        for (int l = 0; l < L; l++) {
+            float ds_sum = 0.0, ds2_sum = 0.0, ds4_sum = 0.0;
+            for (int i = 0; i < T; i++) {
+                const size_t o = (size_t)i * L + l;
+                const float gp0 = gp0s[o], gp1 = gp1s[o];
+                const float gp2 = std::clamp(1.0f - gp1 - gp0, 0.0f, 1.0f);
+                const float ds = gp1 + 2.0f * gp2;
+                uint8_t g[2];
+                if (gp1 > gp0 && gp1 > gp2) {
+                    const bool hds = hdss[o] != 0;
+                    g[hds] = 1;
+                    g[1 - hds] = 0;
+                } else {
+                    g[0] = g[1] = gp0 < gp2;
+                }
+                float P[3];
+                ds_out[o] = std::roundf(ds * 1000.0) / 1000.0;
+                P[0] = floorf(gp0 * 1000.0) / 1000.0;
+                P[1] = floorf(gp1 * 1000.0) / 1000.0;
+                std::map<float, float *> mp;
+                mp.insert(std::make_pair(1.0f - (gp0 - P[0]), &P[0]));
+                mp.insert(std::make_pair(1.0f - (gp1 - P[1]), &P[1]));
+                P[2] = floorf(std::max(1.0f - (P[0] + P[1]), 0.0f) * 1000.0) / 1000.0;
+                mp.insert(std::make_pair(1.0f - (gp2 - P[2]), &P[2]));
+                for (auto it = mp.begin(); it != mp.end() && std::accumulate(P, P + 3, 0.0f) < 0.9999f; ++it)
+                    *it->second += 0.001f;
+                std::memcpy(gp_out + 3 * o, P, sizeof(P));
+                gt_out[2 * o] = g[0];
+                gt_out[2 * o + 1] = g[1];
+                ds_sum += ds;
+                ds2_sum += ds * ds;
+                ds4_sum += gp1 + 4.0 * gp2;
+            }
+            const int n_tar_haps = 2 * T;
+            const float raf = calt[l] * 1.0f / (calt[l] + cref[l]);
+            const float af = ds_sum / n_tar_haps;
+            float info = (af > 0.0 && af < 1.0) ? (float)(1.0 - (ds4_sum - ds2_sum) / (n_tar_haps * af * (1.0 - af)))
+                                                : 1.0f;
+            info = (info < 0.0f) ? 0.0f : info;
+            info = roundf(info * 1000.0) / 1000.0;
+            info_out[l] = info;
+            af_out[l] = af;
+            raf_out[l] = raf;
+        })
+}
+
 } // extern "C"

@@ -130,6 +130,9 @@ void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4
 /* The n-th output (1-based) of a default-seeded std::mt19937. */
 uint32_t orc_mt19937_nth(uint32_t seed, int64_t n);
 float orc_unphred(int32_t i);
+int orc_output_fields(int32_t T, int32_t L, const float *gp0, const float *gp1, const uint8_t *hds,
+                      const uint32_t *cref, const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info,
+                      float *af, float *raf);
 
 #ifdef __cplusplus
 }

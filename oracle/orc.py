@@ -71,6 +71,7 @@ def lib():
         L.orc_mt19937_nth.argtypes = [ctypes.c_uint32, ctypes.c_int64]
         L.orc_unphred.restype = ctypes.c_float
         L.orc_unphred.argtypes = [ctypes.c_int32]
+        L.orc_output_fields.argtypes = [ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 11
         _lib = L
     return _lib
 
@@ -260,3 +261,21 @@ def mt19937_nth(seed, n):
 
 def unphred(i):
     return lib().orc_unphred(i)
+
+
+def output_fields(gp0, gp1, hds, cref, calt):
+    """genotype_writer per-site fields (DS, GP, GT, INFO, AF, RAF) for diploid targets."""
+    gp0 = np.ascontiguousarray(gp0, np.float32)
+    gp1 = np.ascontiguousarray(gp1, np.float32)
+    hds = np.ascontiguousarray(hds, np.uint8)
+    cref = np.ascontiguousarray(cref, np.uint32)
+    calt = np.ascontiguousarray(calt, np.uint32)
+    T, L = gp0.shape
+    out = dict(ds=np.zeros((T, L), np.float32), gp=np.zeros((T, L, 3), np.float32), gt=np.zeros((T, L, 2), np.uint8),
+               info=np.zeros(L, np.float32), af=np.zeros(L, np.float32), raf=np.zeros(L, np.float32))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = lib().orc_output_fields(T, L, p(gp0), p(gp1), p(hds), p(cref), p(calt), p(out["ds"]), p(out["gp"]),
+                                 p(out["gt"]), p(out["info"]), p(out["af"]), p(out["raf"]))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out

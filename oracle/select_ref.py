@@ -38,19 +38,37 @@ def _longest_true_run(b):
     return best
 
 
-def features(P, H, gl, flat, pool, K, depth):
+NEAR, FAR, RUN = 4, 16, 64    # words of 64 common sites
+
+
+def _word_counts(bits, Wc):
+    """Per-word count of set bits (the last word may be partial)."""
+    return np.bincount(np.arange(len(bits)) // 64, weights=bits, minlength=Wc)
+
+
+def features(P, H, gl, flat, pool, pool_site, K, depth):
     """X [T][kstride][32] for the pools (rows >= K[t] stay zero)."""
     T, ks = pool.shape
     L = P["L"]
     common = P["common"].astype(bool)
     c2t = np.nonzero(common)[0]
+    crow = np.full(L, -1)
+    crow[c2t] = np.arange(len(c2t))
     Lc = len(c2t)
     Wc = (Lc + 63) // 64
     A = _common_alleles(P)
     major = P["major"].astype(np.uint8)
     rp, ri = P["rare_ptr"], P["rare_idx"]
     X = np.zeros((T, ks, FEAT), np.float32)
-    word_of = np.arange(Lc) // 64
+
+    def wsites(lo, hi):
+        lo, hi = max(lo, 0), min(hi, Wc)
+        return f32(max(min(64 * hi, Lc) - 64 * lo, 0))
+
+    def rng(cnt, lo, hi):
+        lo, hi = max(lo, 0), min(hi, Wc)
+        return f32(cnt[lo:hi].sum()) if hi > lo else f32(0)
+
     for t in range(T):
         h0 = H[2 * t][c2t].astype(bool)
         h1 = H[2 * t + 1][c2t].astype(bool)
@@ -61,12 +79,9 @@ def features(P, H, gl, flat, pool, K, depth):
         RA = D & (g[:, 1] == 0) & (g[:, 0] >= CONF) & (g[:, 2] >= CONF)
         AA = D & (g[:, 2] == 0) & (g[:, 0] >= CONF) & (g[:, 1] >= CONF)
         inv_d = f32(1.0) / f32(max(D.sum(), 1))
-        inv_rr = f32(1.0) / f32(max(RR.sum(), 1))
-        inv_aa = f32(1.0) / f32(max(AA.sum(), 1))
-        inv_ra = f32(1.0) / f32(max(RA.sum(), 1))
         inv_lc = f32(1.0) / f32(max(Lc, 1))
         inv_wc = f32(1.0) / f32(max(Wc, 1))
-        # panel haplotypes carrying a rare allele that H0 / H1 carries
+        wD, wRR, wAA, wRA = (_word_counts(m, Wc) for m in (D, RR, AA, RA))
         share0, share1 = set(), set()
         for l in range(L):
             if common[l]:
@@ -81,33 +96,49 @@ def features(P, H, gl, flat, pool, K, depth):
         for j in range(k):
             v = int(pool[t, j])
             cid = v & 0xFFFFF
+            cw = max(int(crow[pool_site[t, j]]), 0) >> 6
             R = A[:, cid]
             x0, x1 = R ^ h0, R ^ h1
+            w0, w1 = _word_counts(x0, Wc), _word_counts(x1, Wc)
+            wD0, wD1 = _word_counts(x0 & D, Wc), _word_counts(x1 & D, Wc)
             f = np.zeros(FEAT, np.float32)
             f[0] = f32((v >> 20) & 63) / f32(depth)
             f[1] = f32(j) * inv_k
             f[2] = (v >> 26) & 1
             f[3] = (v >> 27) & 1
-            f[4] = f32((x0 & D).sum()) * inv_d
-            f[5] = f32((x1 & D).sum()) * inv_d
-            f[6] = f32(x0.sum()) * inv_lc
-            f[7] = f32(x1.sum()) * inv_lc
-            clean0 = np.bincount(word_of, weights=x0, minlength=Wc) == 0
-            clean1 = np.bincount(word_of, weights=x1, minlength=Wc) == 0
-            f[8] = f32(_longest_true_run(clean0)) * inv_wc
-            f[9] = f32(_longest_true_run(clean1)) * inv_wc
-            f[10] = f32((R & RR).sum()) * inv_rr
-            f[11] = f32((~R & AA).sum()) * inv_aa
-            f[12] = f32((R & RA).sum()) * inv_ra
-            f[13] = f32(R.sum()) * inv_lc
-            f[14] = cid in share0
-            f[15] = cid in share1
-            for w in range(WIN):
-                ws, we = (w * Wc) // WIN, ((w + 1) * Wc) // WIN
-                s0, s1 = 64 * ws, min(64 * we, Lc)
-                nsite = f32(max(s1 - 64 * ws, 1))
-                f[16 + w] = f32(x0[s0:s1].sum()) / nsite
-                f[24 + w] = f32(x1[s0:s1].sum()) / nsite
+            f[4] = f32(x0.sum()) * inv_lc
+            f[5] = f32(x1.sum()) * inv_lc
+            f[6] = f32(wD0.sum()) * inv_d
+            f[7] = f32(wD1.sum()) * inv_d
+            f[8] = f32(_longest_true_run(w0 == 0)) * inv_wc
+            f[9] = f32(_longest_true_run(w1 == 0)) * inv_wc
+            for o, (a, b) in enumerate(((cw - NEAR, cw), (cw, cw + NEAR), (cw - FAR, cw), (cw, cw + FAR))):
+                den = max(wsites(a, b), f32(1))
+                f[10 + 2 * o] = rng(w0, a, b) / den
+                f[11 + 2 * o] = rng(w1, a, b) / den
+            a, b = cw - FAR, cw + FAR
+            f[18] = rng(wD0, a, b) / max(rng(wD, a, b), f32(1))
+            f[19] = rng(wD1, a, b) / max(rng(wD, a, b), f32(1))
+            for o, wv in enumerate((w0, w1)):
+                back = 0
+                while back < RUN and cw - 1 - back >= 0 and wv[cw - 1 - back] == 0:
+                    back += 1
+                fwd = 0
+                while fwd < RUN and cw + fwd < Wc and wv[cw + fwd] == 0:
+                    fwd += 1
+                f[20 + o] = f32(back) * f32(1.0 / RUN)
+                f[22 + o] = f32(fwd) * f32(1.0 / RUN)
+            lo, hi = max(a, 0), min(b, Wc)
+            s0, s1 = 64 * lo, min(64 * hi, Lc)
+            Rs = R[s0:s1]
+            f[24] = f32((Rs & RR[s0:s1]).sum()) / max(rng(wRR, a, b), f32(1))
+            f[25] = f32((~Rs & AA[s0:s1]).sum()) / max(rng(wAA, a, b), f32(1))
+            f[26] = f32((Rs & RA[s0:s1]).sum()) / max(rng(wRA, a, b), f32(1))
+            f[27] = f32(Rs.sum()) / max(wsites(a, b), f32(1))
+            f[28] = cid in share0
+            f[29] = cid in share1
+            f[30] = f32(cw) * inv_wc
+            f[31] = wsites(a, b) * f32(1.0 / (64.0 * 2 * FAR))
             X[t, j] = f
     return X
 

@@ -37,8 +37,12 @@ def test_features_bit_exact(sel_case):
     H = P["tgt_haps"].copy()
     H ^= (rng.random(H.shape) < 0.02).astype(np.uint8)   # sampled haplotypes are noisy copies
     pool, K = _pools(rng, T, P["N"], 200, 12)
-    got = GP.select_features(H, gl, flat, pool, K, depth=12)
-    ref = select_ref.features(P, H, gl, flat, pool, K, 12)
+    common_sites = np.nonzero(P["common"])[0]
+    site = rng.choice(common_sites, pool.shape).astype(np.int32)
+    site[:, 0] = common_sites[0]
+    site[:, 1] = common_sites[-1]
+    got = GP.select_features(H, gl, flat, pool, site, K, depth=12)
+    ref = select_ref.features(P, H, gl, flat, pool, site, K, 12)
     for t in range(T):
         bad = np.argwhere(got[t] != ref[t])
         assert bad.size == 0, "target %d: %d features differ, first at %s: %r vs %r" % (
@@ -96,10 +100,16 @@ def test_collected_rows_are_distributions(trained):
     groups, wts, feats = y.reshape(-1, ks), w.reshape(-1, ks), X.reshape(-1, ks, 32)
     for g, ww, f in zip(groups, wts, feats):
         K = int(ww.sum())
-        assert K > 0 and np.all(ww[:K] == 1) and np.all(ww[K:] == 0)
-        m = (np.exp(g[:K].astype(np.float64)) - select_ref.LABEL_EPS) / K
-        assert abs(m.sum() - 1.0) < 2e-3
-        assert np.all(f[K:] == 0)
+        assert K > 0 and set(np.unique(ww)) <= {0.0, 1.0} and np.all(ww[K:] == 0) if K < ks else K > 0
+        # labels are ln(K m + eps) with m a candidate's average posterior mass over its segment
+        m = (np.exp(g[ww > 0].astype(np.float64)) - select_ref.LABEL_EPS) / K
+        assert m.min() > -1e-6 and m.max() < 1 + 1e-3
+        # candidates listed in the same segment share one window: their masses sum to at most 1
+        for pos in np.unique(f[ww > 0][:, 30]):
+            same = (ww > 0) & (f[:, 30] == pos) & (f[:, 31] == f[ww > 0][f[ww > 0][:, 30] == pos][0, 31])
+            ms = (np.exp(g[same].astype(np.float64)) - select_ref.LABEL_EPS) / K
+            assert ms.sum() < 1 + 1e-2
+        assert np.all(f[ww == 0] == 0)
     assert losses[-1] < losses[0]
 
 
@@ -140,11 +150,12 @@ def _collect(seed, T=10, K=400, n_haps=2000, n_sites=4000):
 
 
 def test_learned_selector_gains_over_pbwt_order():
-    """On a held-out panel and targets, the trained network's top 10% of the
-    PBWT pool holds more of the large-K HMM's posterior mass than the first
-    10% in PBWT order (the states plain PBWT selection at that K keeps)."""
+    """On a held-out panel and targets, the trained network ranks the candidates
+    listed around each stored site better than PBWT order does: its first 20%
+    hold more of the large-K HMM's posterior mass in that segment than the
+    first 20% in PBWT order (the ones plain PBWT selection keeps at that K)."""
     from glimpse_amd import hip
-    K, k = 400, 40
+    K = 400
     _, run_a, _ = _collect(101)
     net = hip.Mlp(seed=7)
     losses = net.train_run(run_a, epochs=10, batch=2048, lr=2e-3, seed=8)
@@ -153,12 +164,18 @@ def test_learned_selector_gains_over_pbwt_order():
     learned, pbwt = [], []
     for g in range(X.shape[0] // K):
         sl = slice(g * K, (g + 1) * K)
-        n = int(w[sl].sum())
-        lab, sc = y[sl][:n], pred[sl][:n]
-        learned.append(select_ref.captured_mass(lab, sc, k))
-        pbwt.append(select_ref.captured_mass(lab, np.arange(n), k))
+        keep = w[sl] > 0
+        Xg, lab, sc = X[sl][keep], y[sl][keep], pred[sl][keep]
+        seg = Xg[:, 30] * 1e4 + Xg[:, 31]          # candidates listed in the same segment
+        for s_ in np.unique(seg):
+            idx = np.nonzero(seg == s_)[0]          # in PBWT insertion order
+            if len(idx) < 10:
+                continue
+            q = max(1, len(idx) // 5)
+            learned.append(select_ref.captured_mass(lab[idx], sc[idx], q))
+            pbwt.append(select_ref.captured_mass(lab[idx], np.arange(len(idx)), q))
     learned, pbwt = np.array(learned), np.array(pbwt)
-    print("train loss %s; captured mass at k=%d of %d: learned %.3f, PBWT order %.3f (won %d of %d)" % (
-        np.round(losses, 3), k, K, learned.mean(), pbwt.mean(), np.sum(learned > pbwt), len(pbwt)))
-    assert learned.mean() > pbwt.mean() + 0.05
-    assert np.mean(learned > pbwt) > 0.8
+    print("train loss %s; segment mass in the first 20%%: learned %.3f, PBWT order %.3f (won %d of %d)" % (
+        np.round(losses, 3), learned.mean(), pbwt.mean(), np.sum(learned > pbwt), len(pbwt)))
+    assert learned.mean() > pbwt.mean() + 0.02
+    assert np.mean(learned >= pbwt) > 0.6

@@ -105,19 +105,23 @@ def test_features_closed_forms():
     flat[0, c2t[:10]] = 1                # ... except 10 sites without data
     pool = np.zeros((1, 4), np.int32)
     pool[0, :3] = [5 | (2 << 20), 9 | (1 << 26) | (1 << 27), carrier]
-    X = select_ref.features(P, H, gl, flat, pool, np.array([3]), depth=4)
+    site = np.zeros((1, 4), np.int32)
+    site[0, :3] = [c2t[70], c2t[0], c2t[129]]   # listed in words 1, 0 and 2 of 3
+    X = select_ref.features(P, H, gl, flat, pool, site, np.array([3]), depth=4)
     f5, f9, fc = X[0, 0], X[0, 1], X[0, 2]
     assert f5[0] == np.float32(0.5) and f9[2] == 1 and f9[3] == 1 and f5[2] == 0
     assert f5[1] == 0 and f9[1] == np.float32(1) * (np.float32(1) / np.float32(3))
-    # identical to H0: no mismatches anywhere, one run over all 3 words
-    assert f5[4] == 0 and f5[6] == 0 and f5[8] == 1 and np.all(f5[16:24] == 0)
+    # identical to H0: no mismatches anywhere, one run over all 3 words, 1 word back and 2 forward
+    assert f5[4] == 0 and f5[6] == 0 and f5[8] == 1 and np.all(f5[10:20:2] == 0)
+    assert f5[20] == np.float32(1 / 64) and f5[22] == np.float32(2 / 64)
     # complement of H1: every common site mismatches, no clean word
-    # (3 words over 8 windows: windows 2, 5 and 7 hold one word each, the rest are empty and read 0)
-    assert f9[7] == 1 and f9[5] == 1 and f9[9] == 0 and list(f9[24:32]) == [0, 0, 1, 0, 0, 1, 0, 1]
-    # alt rate and confident-RR alt counts are the candidate's own allele counts
-    assert f5[13] == np.float32(A[:, 5].sum()) * (np.float32(1) / np.float32(130))
-    assert f5[10] == np.float32(A[10:, 5].sum()) * (np.float32(1) / np.float32(120))
-    assert fc[14] == 1 and fc[15] == 0 and np.all(X[0, 3] == 0)
+    assert f9[5] == 1 and f9[7] == 1 and f9[9] == 0 and f9[21] == 0 and f9[23] == 0
+    assert f9[11] == 0 and f9[13] == 1 and f9[15] == 0 and f9[17] == 1   # listed in word 0: nothing behind it
+    # the segment covers the whole chunk here: alt rate and confident-RR alt counts are whole-chunk counts
+    assert f5[27] == np.float32(A[:, 5].sum()) / np.float32(130)
+    assert f5[24] == np.float32(A[10:, 5].sum()) / np.float32(120)
+    assert f5[30] == np.float32(1) * (np.float32(1) / np.float32(3)) and f5[31] == np.float32(130 / 2048)
+    assert fc[28] == 1 and fc[29] == 0 and np.all(X[0, 3] == 0)
 
 
 def test_topk_rule():
