@@ -20,13 +20,17 @@ plain run records: a separate panel of 4,000 haplotypes x 10,000 SNPs (2 Mb),
 48 targets at 1x, Kpbwt 2,000, every PBWT iteration; labels are the HMM's own
 posterior masses.
 
-metric value: target-site imputations/s of the learned path = sum over chunks
-of targets x chunk sites / wall time of one step; the plain path's figure,
-both dosage r2 / NRC against the simulated truth, and the learned path's mean
-|DS - DS(plain)| (the plain run at Kpbwt judges its dosage error) are in the
-same line.  The roofline object reports the dominant kernel (haploid
-forward-backward) of the learned path against HBM: algorithmic bytes per launch
-/ average launch duration (HIP events on the launch stream).  cpu_baseline:
+metric value: target-site imputations/s = sum over chunks of targets x chunk
+sites / wall time of one step, for the learned path when its dosage r2 is
+within 0.001 of the plain path's (the "equal r2" condition), else for the
+plain path (config.path says which).  Both paths' figures, dosage r2 / NRC
+against the simulated truth, and the learned path's mean |DS - DS(plain)| (the
+plain run at Kpbwt judges its dosage error) are in the same line.  The roofline
+object reports the dominant kernel (haploid forward-backward) against HBM:
+algorithmic bytes per launch / average launch duration (HIP events on the
+launch stream); `traffic` is the HBM bytes per launch measured with rocprofv3
+counters for this workload (profiles/fb_kernel_c2_<path>_traffic.md).
+cpu_baseline:
 the CPU oracle on a bounded sample of the same workload (one target of the
 smallest chunk, INIT + 1 burn-in + 1 main iteration, scaled to the 21
 iterations of a full run), one core.
@@ -179,7 +183,20 @@ def timed(runs, a, label):
     return dt, [r.fetch() for r in runs]
 
 
-def fb_roofline(outs):
+def profiled_traffic(path):
+    """HBM bytes per forward-backward launch from the committed rocprofv3 counter
+    summary of this workload and path (FETCH_SIZE and WRITE_SIZE runs, scaled by
+    a 1 GiB copy's counter readings), or None when there is none."""
+    f = os.path.join(ROOT, "profiles", "fb_kernel_c2_%s_traffic.md" % path)
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def fb_roofline(outs, path=None):
     fb_bytes = sum(o["stats"]["fb_bytes"] for o in outs)
     fb_ms = sum(o["stats"]["fb_kernel_ms"] for o in outs)
     fb_n = sum(o["stats"]["fb_launches"] for o in outs)
@@ -187,7 +204,7 @@ def fb_roofline(outs):
     avg_ms = fb_ms / fb_n
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(path) if path else None,
             "kernel": "fb_kernel (haploid forward-backward)",
             "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches_per_step": fb_n}
 
@@ -254,7 +271,12 @@ def main():
     for k, v in res.items():
         v["accuracy"] = accuracy(v["outs"], chunks)
         v["stage_ms_summed_over_chunks"] = stage_ms(v["outs"])
+    # the headline path: the learned selector when its dosage r2 is within 0.001
+    # of the plain path's, else the plain path
     main_path = "learned" if "learned" in res else "plain"
+    if "learned" in res and "plain" in res and \
+            abs(res["learned"]["accuracy"]["dosage_r2"] - res["plain"]["accuracy"]["dosage_r2"]) > 0.001:
+        main_path = "plain"
     M = res[main_path]
     line = {
         "metric": METRIC,
@@ -271,7 +293,7 @@ def main():
         "data": "synthetic (seeded simulator: founder mosaics + Poisson reads)",
         "config": dict(workload=desc, path=main_path,
                        concurrency="serial" if a.serial else "chunks on concurrent streams", **meta),
-        "roofline": fb_roofline(M["outs"]),
+        "roofline": fb_roofline(M["outs"], main_path if a.quick is False else None),
         "accuracy": M["accuracy"],
         "stage_ms_summed_over_chunks": M["stage_ms_summed_over_chunks"],
         "setup_s": setup_s,
@@ -279,7 +301,9 @@ def main():
     if "learned" in res:
         L_ = res["learned"]
         st = [o["stats"] for o in L_["outs"]]
-        line["learned"] = dict(k_frac=a.k_frac, min_mass=a.min_mass, training=train,
+        line["learned"] = dict(value=L_["value"], ms_per_step=L_["ms_per_step"], accuracy=L_["accuracy"],
+                               roofline=fb_roofline(L_["outs"], None if a.quick else "learned"),
+                               k_frac=a.k_frac, min_mass=a.min_mass, training=train,
                                target_iterations_learned=sum(s["learned_sel"] for s in st),
                                target_iterations_pool_kept=sum(s["pool_kept"] for s in st),
                                mean_states=float(np.mean(np.concatenate(
@@ -287,7 +311,7 @@ def main():
     if "plain" in res:
         Pn = res["plain"]
         line["plain_pbwt"] = dict(value=Pn["value"], ms_per_step=Pn["ms_per_step"], accuracy=Pn["accuracy"],
-                                  roofline=fb_roofline(Pn["outs"]),
+                                  roofline=fb_roofline(Pn["outs"], None if a.quick else "plain"),
                                   stage_ms_summed_over_chunks=Pn["stage_ms_summed_over_chunks"],
                                   mean_states=float(np.mean(np.concatenate(
                                       [o["nstates"][:, 1:].ravel() for o in Pn["outs"]]))))
