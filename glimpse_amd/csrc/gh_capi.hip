@@ -25,6 +25,7 @@ void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h
                int32_t *nstates_h, gh_run_stats *stats);
 void run_destroy(gh_run *G);
 void run_collect(gh_run *G, int from_iter);
+void run_set_tuning(gh_run *G, int on);
 void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w);
 void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss);
 void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode);
@@ -260,7 +261,8 @@ int gh_rephase_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int
                           d_H1.as<uint8_t>(), d_flat.as<uint8_t>(), P->d_flags.as<uint8_t>(), d_loci.as<Locus>(), s);
         GH_HIP(hipMemcpyAsync(ph.data(), d_ph.p, sizeof(PhItem) * n, hipMemcpyDeviceToHost, s));
         GH_HIP(hipStreamSynchronize(s));
-        const int64_t KH = (int64_t)phase_kpt_states(maxK) * 8;
+        const PhaseShape psh = phase_shape(maxK);
+        const int64_t KH = (int64_t)phase_kpt_states(psh) * 8;
         int64_t so = 0, ss = 0, io = 0, is = 0, dd = 0;
         for (int i = 0; i < n; i++) {
             ph[i].seg_off = so; so += ph[i].n_segs * KH;
@@ -281,7 +283,7 @@ int gh_rephase_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int
         GH_HIP(hipMemsetAsync(st, 0, sizeof(int32_t), s));
         phase_launch(P, n, maxK, d_ph.as<PhItem>(), B.items.as<CondItem>(), B.hvar.as<uint64_t>(), d_loci.as<Locus>(),
                      B.pos.as<int32_t>(), ne, PB, d_H0.as<uint8_t>(), d_H1.as<uint8_t>(), err_phase, seed,
-                     (uint32_t)it, st, s);
+                     (uint32_t)it, st, s, psh);
         int32_t status = 0;
         GH_HIP(hipMemcpyAsync(&status, st, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         GH_HIP(hipMemcpyAsync(h0, d_H0.p, (size_t)n * L, hipMemcpyDeviceToHost, s));
@@ -423,6 +425,9 @@ int gh_mlp_step(gh_mlp *m, int32_t n, const float *X, const float *t, const floa
 int gh_mlp_train_run(gh_mlp *m, gh_run *r, int32_t epochs, int32_t batch, float lr, uint32_t seed,
                      float *loss_per_epoch) {
     return guard([&] { run_train(m, r, epochs, batch, lr, seed, loss_per_epoch); });
+}
+int gh_run_set_tuning(gh_run *r, int32_t on) {
+    return guard([&] { run_set_tuning(r, on); });
 }
 int gh_run_collect(gh_run *r, int32_t from_iter) {
     return guard([&] { run_collect(r, from_iter); });

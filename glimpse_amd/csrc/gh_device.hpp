@@ -39,6 +39,48 @@ __device__ __forceinline__ void wave_sum3(float &x, float &y, float &z) {
 #undef GH_RL
 }
 
+// Reduce-scatter of 8 per-lane values over the wave: returns, in every lane l,
+// the wave total of value (l & 7).  Three DPP halving steps inside each group
+// of 8 lanes (partner 7-i, i^2, i^1; each lane keeps the half its lane bit
+// selects), then the 8 groups are combined (row_ror:8, xor 16, xor 32).
+__device__ __forceinline__ float wave_rs8(const float v[8]) {
+    const int i = threadIdx.x & 7;
+    const bool b2 = i & 4, b1 = i & 2, b0 = i & 1;
+    float u[4], w[2];
+#pragma unroll
+    for (int k = 0; k < 4; k++) u[k] = (b2 ? v[4 + k] : v[k]) + dppf<0x141>(b2 ? v[k] : v[4 + k]);
+#pragma unroll
+    for (int k = 0; k < 2; k++) w[k] = (b1 ? u[2 + k] : u[k]) + dppf<0x4E>(b1 ? u[k] : u[2 + k]);
+    float x = (b0 ? w[1] : w[0]) + dppf<0xB1>(b0 ? w[0] : w[1]);
+    x += dppf<0x128>(x);
+    x += __shfl_xor(x, 16);
+    x += __shfl_xor(x, 32);
+    return x;
+}
+
+// The same for 64 values: returns in lane l the wave total of value
+// rs64_index(l) = 32*b2 + 16*b1 + 8*b0 + 4*b3 + 2*b4 + b5 (bX = bit X of l).
+__device__ __forceinline__ int rs64_index(int l) {
+    return 32 * ((l >> 2) & 1) + 16 * ((l >> 1) & 1) + 8 * (l & 1) + 4 * ((l >> 3) & 1) + 2 * ((l >> 4) & 1) +
+           ((l >> 5) & 1);
+}
+__device__ __forceinline__ float wave_rs64(const float v[64]) {
+    const int l = threadIdx.x & 63;
+    const bool b2 = l & 4, b1 = l & 2, b0 = l & 1, b3 = l & 8, b4 = l & 16, b5 = l & 32;
+    float a[32], b[16], c[8], d[4], e[2];
+#pragma unroll
+    for (int k = 0; k < 32; k++) a[k] = (b2 ? v[32 + k] : v[k]) + dppf<0x141>(b2 ? v[k] : v[32 + k]);
+#pragma unroll
+    for (int k = 0; k < 16; k++) b[k] = (b1 ? a[16 + k] : a[k]) + dppf<0x4E>(b1 ? a[k] : a[16 + k]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = (b0 ? b[8 + k] : b[k]) + dppf<0xB1>(b0 ? b[k] : b[8 + k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) d[k] = (b3 ? c[4 + k] : c[k]) + dppf<0x128>(b3 ? c[k] : c[4 + k]);
+#pragma unroll
+    for (int k = 0; k < 2; k++) e[k] = (b4 ? d[2 + k] : d[k]) + __shfl_xor(b4 ? d[k] : d[2 + k], 16);
+    return (b5 ? e[1] : e[0]) + __shfl_xor(b5 ? e[0] : e[1], 32);
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 template <typename T>

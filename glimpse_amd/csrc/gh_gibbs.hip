@@ -220,6 +220,40 @@ static void init_states(gh_panel *P, int t, const uint8_t *gl, const uint8_t *fl
 }
 
 /* ------------------------------------------------------------------ driver */
+// Online launch-shape tuner: while a class (set-size bucket) is being tuned,
+// each iteration runs the next untimed candidate shape and records its kernel
+// time; once every candidate has a time, the fastest is used from then on.
+template <typename S>
+struct ShapeTuner {
+    int cls = -1, cur = -1, best = -1;
+    std::vector<S> cand;
+    std::vector<float> ms;
+    S pick(int c, const std::vector<S> &cands, S dflt) {
+        if (c != cls) {
+            cls = c;
+            cand = cands;
+            ms.assign(cand.size(), -1.0f);
+            best = -1;
+        }
+        cur = -1;
+        if (best >= 0) return cand[best];
+        for (size_t i = 0; i < cand.size(); i++)
+            if (ms[i] < 0.0f) { cur = (int)i; return cand[i]; }
+        return dflt;
+    }
+    void record(float t) {
+        if (cur < 0) return;
+        ms[cur] = t;
+        cur = -1;
+        int b = 0;
+        for (size_t i = 0; i < ms.size(); i++) {
+            if (ms[i] < 0.0f) return;
+            if (ms[i] < ms[b]) b = (int)i;
+        }
+        best = b;
+    }
+};
+
 struct EvPair {
     hipEvent_t a = nullptr, b = nullptr;
     void make() { GH_HIP(hipEventCreate(&a)); GH_HIP(hipEventCreate(&b)); }
@@ -253,6 +287,9 @@ struct gh_run {
     gh_run_stats st;
     bool done = false;
     // learned selection / training-row collection
+    bool tune = false;
+    gh::ShapeTuner<gh::FbPick> fb_tuner;
+    gh::ShapeTuner<gh::PhaseShape> ph_tuner;
     gh_mlp *mlp = nullptr;
     float min_mass = 0.95f;
     int sel_mode = 1;          // 0: global top-k, 1: top-k per PBWT group, groups served round robin
@@ -498,8 +535,8 @@ void run_execute(gh_run *G) {
     GH_HIP(hipMemsetAsync(status, 0, sizeof(int32_t) * 2, s));
     gh_run_stats st;
     std::memset(&st, 0, sizeof(st));
-    EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1, e_learn;
-    e_sel.make(); e_cond.make(); e_fb.make(); e_ph.make(); e_k0.make(); e_k1.make(); e_learn.make();
+    EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1, e_learn, e_pk;
+    e_sel.make(); e_cond.make(); e_fb.make(); e_ph.make(); e_k0.make(); e_k1.make(); e_learn.make(); e_pk.make();
     const dim3 gsite((L + 255) / 256, T), bsite(256);
     int main_done = 0;
     std::vector<CondItem> &items = G->items;
@@ -617,6 +654,9 @@ void run_execute(gh_run *G) {
         GH_HIP(hipEventRecord(e_sel.b, s));
         GH_HIP(hipEventRecord(e_cond.a, s));
         condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, maxW_it, s);
+        const int wcls = maxW_it <= 4 ? 0 : maxW_it <= 8 ? 1 : maxW_it <= 16 ? 2 : 3;
+        const FbPick fbp = (G->tune && stage > 0) ? G->fb_tuner.pick(wcls, fb_shapes(maxW_it), FbPick{0, 0})
+                                                  : FbPick{0, 0};
         const bool label_it = collecting && stage > 0 && npt > 0 && it >= G->collect_from;
         float *mass0 = label_it ? G->d_mass0.as<float>() : nullptr, *mass1 = label_it ? G->d_mass1.as<float>() : nullptr;
         const int32_t *wlo = label_it ? G->d_wlo.as<int32_t>() : nullptr, *whi = label_it ? G->d_whi.as<int32_t>() : nullptr;
@@ -626,7 +666,7 @@ void run_execute(gh_run *G) {
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k0.a, s));
         fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP0, G->ee_i, G->ed_i, s, mass0, wlo,
-                  whi);
+                  whi, fbp);
         GH_HIP(hipEventRecord(e_k0.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
@@ -634,7 +674,7 @@ void run_execute(gh_run *G) {
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k1.a, s));
         fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i, G->ed_i, s, mass1, wlo,
-                  whi);
+                  whi, fbp);
         GH_HIP(hipEventRecord(e_k1.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
@@ -659,7 +699,10 @@ void run_execute(gh_run *G) {
         GH_HIP(hipStreamSynchronize(s));
         int maxK = 1;
         for (int t = 0; t < T; t++) maxK = std::max(maxK, items[t].K);
-        const int64_t KH = (int64_t)phase_kpt_states(maxK) * 8;
+        const int kcls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
+        const PhaseShape phs = (G->tune && stage > 0) ? G->ph_tuner.pick(kcls, phase_shapes(maxK), phase_shape(maxK))
+                                                      : phase_shape(maxK);
+        const int64_t KH = (int64_t)phase_kpt_states(phs) * 8;
         int64_t so = 0, ss = 0, io = 0, is = 0, dd = 0;
         for (int t = 0; t < T; t++) {
             ph[t].seg_off = so; so += ph[t].n_segs * KH;
@@ -682,8 +725,10 @@ void run_execute(gh_run *G) {
         PB.impsum = (float *)G->d_impsum.reserve(sizeof(float) * std::max<int64_t>(is, 1));
         PB.of1 = (float *)G->d_of1.reserve(sizeof(float) * std::max<int64_t>(is, 1));
         PB.dip = (int32_t *)G->d_dip.reserve(sizeof(int32_t) * std::max<int64_t>(dd, 1));
+        GH_HIP(hipEventRecord(e_pk.a, s));
         phase_launch(P, T, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
-                     (uint32_t)it, status, s);
+                     (uint32_t)it, status, s, phs);
+        GH_HIP(hipEventRecord(e_pk.b, s));
         GH_HIP(hipEventRecord(e_ph.b, s));
         if (stage == 2) {
             hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, gp0, gp1, sto, hds);
@@ -701,6 +746,14 @@ void run_execute(gh_run *G) {
         st.ms_phase += e_ph.ms();
         st.fb_kernel_ms += e_k0.ms() + e_k1.ms();
         st.fb_launches += 2;
+        if (G->tune && stage > 0) {
+            G->fb_tuner.record(e_k0.ms() + e_k1.ms());
+            G->ph_tuner.record(e_pk.ms());
+        }
+        {
+            const FbPick fu = fbp.NW ? fbp : fb_default_pick(maxW_it);
+            st.fb_nw = fu.NW; st.fb_mwl = fu.MWL; st.ph_nw = phs.NW; st.ph_kpt = phs.KPT;
+        }
         if (learn_ev) st.ms_learn += e_learn.ms();
     }
     if ((learned || collecting) && npt > 0) {
@@ -767,6 +820,11 @@ void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mas
     GH_HIP(hipStreamSynchronize(G->P->stream));
     G->mlp = M;
     G->min_mass = min_mass;
+}
+
+void run_set_tuning(gh_run *G, int on) {
+    GH_CHECK(G, GH_ERR_ARG, "gh_run_set_tuning: null run");
+    G->tune = on != 0;
 }
 
 void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,

@@ -28,6 +28,10 @@
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
 
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
 namespace gh {
 
 constexpr int FB_TILE = 64;
@@ -331,21 +335,57 @@ struct FbShape {
     int NW, MWL, C;
     size_t smem;
 };
-static FbShape fb_shape(int maxW) {
+// Shapes the kernel is built for (NW waves x MWL words per lane covers 64*NW*MWL states).
+static const int kFbShapes[][2] = {
+    {1, 4}, {2, 2}, {4, 1},     // W <= 4
+    {2, 4}, {4, 2}, {8, 1},     // W <= 8
+    {4, 4}, {8, 2}, {16, 1},    // W <= 16
+    {4, 8}, {8, 4}, {16, 2},    // W <= 32
+};
+static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     FbShape s;
-    if (maxW <= 4) { s.NW = 1; s.MWL = 4; }
-    else if (maxW <= 8) { s.NW = 2; s.MWL = 4; }
-    else if (maxW <= 16) { s.NW = 4; s.MWL = 4; }
-    else { s.NW = 4; s.MWL = 8; }
+    s.NW = NW;
+    s.MWL = MWL;
     const size_t rows = (size_t)FB_TILE * maxW * 8;
     const size_t per_site = (size_t)s.NW * s.MWL * 64 * 4;
+    const size_t red = 2 * 16 * 4 * 4;   // [2][NW <= 16][4] floats
     int C = 64;
-    while (C > 1 && rows + C * per_site + 2 * 4 * 4 * 4 > (size_t)FB_LDS_BYTES) C >>= 1;
+    while (C > 1 && rows + C * per_site + red > (size_t)FB_LDS_BYTES) C >>= 1;
     s.C = C;
-    s.smem = rows + C * per_site + 2 * 4 * 4 * 4;
+    s.smem = rows + C * per_site + red;
     return s;
 }
+static int fb_cap_words(int maxW) { return maxW <= 4 ? 4 : maxW <= 8 ? 8 : maxW <= 16 ? 16 : 32; }
+static FbShape fb_shape(int maxW, FbPick pick = FbPick{0, 0}) {
+    const int cap = fb_cap_words(maxW);
+    if (pick.NW > 0 && pick.NW * pick.MWL == cap) return fb_shape_of(maxW, pick.NW, pick.MWL);
+    // GH_FB_SHAPE=<NW>x<MWL> pins a shape wherever it fits (testing, reproducing a tuned run)
+    if (const char *e = std::getenv("GH_FB_SHAPE")) {
+        int nw = 0, mwl = 0;
+        if (std::sscanf(e, "%dx%d", &nw, &mwl) == 2 && nw * mwl == cap)
+            for (const auto &sh : kFbShapes)
+                if (sh[0] == nw && sh[1] == mwl) return fb_shape_of(maxW, nw, mwl);
+    }
+    if (cap == 4) return fb_shape_of(maxW, 1, 4);
+    if (cap == 8) return fb_shape_of(maxW, 2, 4);
+    if (cap == 16) return fb_shape_of(maxW, 4, 4);
+    return fb_shape_of(maxW, 8, 4);
+}
 
+FbPick fb_default_pick(int maxW) {
+    const FbShape sh = fb_shape(maxW);
+    return FbPick{sh.NW, sh.MWL};
+}
+
+std::vector<FbPick> fb_shapes(int maxW) {
+    std::vector<FbPick> v;
+    for (const auto &sh : kFbShapes)
+        if (sh[0] * sh[1] == fb_cap_words(maxW)) v.push_back(FbPick{sh[0], sh[1]});
+    return v;
+}
+
+// The checkpoint interval only depends on the words a shape covers (NW x MWL),
+// which every shape of a capacity class shares.
 int fb_block_sites(int maxW) { return fb_shape(maxW).C; }
 
 template <int NW, int MWL>
@@ -362,20 +402,25 @@ static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass, const int32_t *d_win_lo,
-               const int32_t *d_win_hi) {
+               const int32_t *d_win_hi, FbPick pick) {
     if (n <= 0) return;
     GH_CHECK(maxW <= 32, GH_ERR_ARG, "fb: more than 2048 conditioning states per haplotype is not supported");
-    const FbShape sh = fb_shape(maxW);
+    const FbShape sh = fb_shape(maxW, pick);
     FbArgs A;
     A.fb = d_fb; A.cond = d_cond; A.poly_all = d_poly; A.t_all = d_t; A.hvar_all = d_hvar; A.HL_all = d_HL;
     A.flat_all = d_flat; A.flags = P->d_flags.as<uint8_t>(); A.ckpt_all = d_ckpt; A.HP_all = d_HP;
     A.mass_all = d_mass; A.win_lo = d_win_lo; A.win_hi = d_win_hi;
     A.ee = ee; A.ed = ed; A.C = sh.C; A.maxW = maxW;
-    if (sh.NW == 1) fb_go<1, 4>(A, n, sh.smem, s);
-    else if (sh.NW == 2) fb_go<2, 4>(A, n, sh.smem, s);
-    else if (sh.MWL == 4) fb_go<4, 4>(A, n, sh.smem, s);
-    else fb_go<4, 8>(A, n, sh.smem, s);
-    GH_HIP(hipGetLastError());
+#define GH_FB(nw, mwl)                    \
+    if (sh.NW == nw && sh.MWL == mwl) {   \
+        fb_go<nw, mwl>(A, n, sh.smem, s); \
+        GH_HIP(hipGetLastError());        \
+        return;                           \
+    }
+    GH_FB(1, 4) GH_FB(2, 2) GH_FB(4, 1) GH_FB(2, 4) GH_FB(4, 2) GH_FB(8, 1)
+    GH_FB(4, 4) GH_FB(8, 2) GH_FB(16, 1) GH_FB(4, 8) GH_FB(8, 4) GH_FB(16, 2)
+#undef GH_FB
+    throw Error(GH_ERR_ARG, "fb: no kernel for the requested launch shape");
 }
 
 void fb_mono_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_pos,

@@ -119,10 +119,18 @@ struct FbItem {
     int64_t hl_off;           // into HL / HP ([2L] per item)
     int64_t flat_off;         // into flat ([L] per item)
 };
+// Launch shape of the forward-backward kernel: NW waves per haplotype, MWL
+// 64-state words per lane (NW * MWL words cover the largest set).  {0, 0}: default.
+struct FbPick {
+    int NW, MWL;
+};
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass = nullptr,
-               const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr);
+               const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr, FbPick pick = FbPick{0, 0});
+// Shapes built for sets of up to maxW words (the tuner's candidates).
+std::vector<FbPick> fb_shapes(int maxW);
+FbPick fb_default_pick(int maxW);
 int fb_block_sites(int maxW);
 // HP at monomorphic sites (closed form) for the same items.
 void fb_mono_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_pos,

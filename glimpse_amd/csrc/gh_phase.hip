@@ -7,10 +7,14 @@
 // of three hets (a segment starts at het ordinals 0, 3, 6, ...), the flat-het
 // ordinal, and counts segments / flat hets.
 //
-// phase_kernel<KPT> (one 256-thread workgroup per target): thread t owns
-// states t + 256*i (i < KPT) and keeps their 8 hypothesis values in
-// registers.  Per locus the 8 per-hypothesis sums are reduced with DPP inside
-// each wave and across the four waves through LDS.  The backward pass stores
+// phase_kernel<NW, KPT> (one workgroup of NW waves per target): thread t owns
+// states t + 64*NW*i (i < KPT) and keeps their 8 hypothesis values in
+// registers.  Per locus the 8 per-hypothesis sums are reduce-scattered with DPP
+// inside each wave (lane l ends with hypothesis l & 7) and combined across the
+// NW waves through one LDS exchange; TRANS_HAP's 64 sums are reduce-scattered
+// the same way.  More waves per target shorten each wave's share of a locus and
+// let every SIMD interleave several waves; the launch shape comes from the
+// tuning table (phase_shape).  The backward pass stores
 // segment-start and flat-het messages (K x 8 floats, [state][hyp] with the
 // state index interleaved as in registers) in HBM; the forward pass samples
 // the segment diplotypes (TRANS_HAP + SAMPLE_DIP) and the flat hets, then the
@@ -20,9 +24,13 @@
 #include "gh_internal.hpp"
 #include "gh_phase.hpp"
 
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
 namespace gh {
 
-constexpr int PH_NT = 256;
 constexpr int PREP_NT = 1024;
 
 template <int NT>
@@ -129,21 +137,6 @@ __device__ __forceinline__ float hadd8(const float *v) {
     return ((v[0] + v[4]) + (v[1] + v[5])) + ((v[2] + v[6]) + (v[3] + v[7]));
 }
 
-// Sum of 8 per-thread values over the workgroup; result uniform in every thread.
-__device__ __forceinline__ void block_sum8(float v[8], float *lds) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int h = 0; h < 8; h++) v[h] = wave_sum(v[h]);
-    if (lane == 0) {
-#pragma unroll
-        for (int h = 0; h < 8; h++) lds[wid * 8 + h] = v[h];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 8; h++) v[h] = (lds[h] + lds[8 + h]) + (lds[16 + h] + lds[24 + h]);
-    __syncthreads();
-}
-
 __device__ __forceinline__ float transition(const double *cm, int prev, int next, double nrho) {
     double x = -expm1(nrho * (cm[next] - cm[prev]));
     const double one_l = 1.0 - 1e-7;
@@ -153,21 +146,22 @@ __device__ __forceinline__ float transition(const double *cm, int prev, int next
 
 constexpr int PH_TILE = 64;
 
-template <int KPT>
-__global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__ items, const CondItem *__restrict__ cond,
-                                                     const uint64_t *__restrict__ hvar_all,
-                                                     const Locus *__restrict__ loci_all,
-                                                     const int32_t *__restrict__ pos_all,
-                                                     const double *__restrict__ cm, double nrho,
-                                                     float *__restrict__ seg_all, float *__restrict__ segsum_all,
-                                                     float *__restrict__ imp_all, float *__restrict__ impsum_all,
-                                                     float *__restrict__ of1_all, int32_t *__restrict__ dip_all,
-                                                     uint8_t *__restrict__ H0_all, uint8_t *__restrict__ H1_all,
-                                                     int L, float ee, float ed, uint32_t seed, uint32_t iter,
-                                                     int32_t *__restrict__ status) {
+template <int NW, int KPT>
+__global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict__ items, const CondItem *__restrict__ cond,
+                                                       const uint64_t *__restrict__ hvar_all,
+                                                       const Locus *__restrict__ loci_all,
+                                                       const int32_t *__restrict__ pos_all,
+                                                       const double *__restrict__ cm, double nrho,
+                                                       float *__restrict__ seg_all, float *__restrict__ segsum_all,
+                                                       float *__restrict__ imp_all, float *__restrict__ impsum_all,
+                                                       float *__restrict__ of1_all, int32_t *__restrict__ dip_all,
+                                                       uint8_t *__restrict__ H0_all, uint8_t *__restrict__ H1_all,
+                                                       int L, float ee, float ed, uint32_t seed, uint32_t iter,
+                                                       int32_t *__restrict__ status) {
+    constexpr int PH_NT = 64 * NW;
     extern __shared__ __align__(16) uint64_t s_rows[];   // [PH_TILE][W] Hvar rows of the current tile
-    __shared__ float lds8[2][32];
-    __shared__ float red64[4][64];
+    __shared__ float lds8[2][NW * 8];
+    __shared__ float red64[NW][64];
     __shared__ float hprobs[64];
     __shared__ int dip_sh;
     __shared__ int s_rel[PH_TILE], s_seg[PH_TILE], s_miss[PH_TILE];
@@ -204,18 +198,22 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
 #pragma unroll
     for (int i = 0; i < KPT; i++) sumK[i] = 0.0f;
 
-    // per-hypothesis sums over the workgroup, one barrier (LDS slot alternates)
+    // per-hypothesis sums over the workgroup: DPP reduce-scatter in each wave,
+    // one LDS exchange and barrier across the waves (the slot alternates), then
+    // the 8 totals are read out of lanes 0..7
     auto bsum8 = [&](float v[8]) {
+        float x = wave_rs8(v);
+        if (NW > 1) {
+            par ^= 1;
+            if (lane < 8) lds8[par][wid * 8 + lane] = x;
+            __syncthreads();
+            float t = 0.0f;
 #pragma unroll
-        for (int h = 0; h < 8; h++) v[h] = wave_sum(v[h]);
-        par ^= 1;
-        if (lane == 0) {
-#pragma unroll
-            for (int h = 0; h < 8; h++) lds8[par][wid * 8 + h] = v[h];
+            for (int q = 0; q < NW; q++) t += lds8[par][q * 8 + (lane & 7)];
+            x = t;
         }
-        __syncthreads();
 #pragma unroll
-        for (int h = 0; h < 8; h++) v[h] = (lds8[par][h] + lds8[par][8 + h]) + (lds8[par][16 + h] + lds8[par][24 + h]);
+        for (int h = 0; h < 8; h++) v[h] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), h));
     };
 
     // tile of loci [q0, q0+n): scalars and Hvar rows into LDS
@@ -426,13 +424,14 @@ __global__ __launch_bounds__(PH_NT) void phase_kernel(const PhItem *__restrict__
                         for (int h2 = 0; h2 < 8; h2++) part[h1 * 8 + h2] += pv * nv[h2];
                     }
                 }
-#pragma unroll
-                for (int e = 0; e < 64; e++) {
-                    const float sm = wave_sum(part[e]);
-                    if (lane == 0) red64[wid][e] = sm;
-                }
+                red64[wid][rs64_index(lane)] = wave_rs64(part);
                 __syncthreads();
-                if (tid < 64) hprobs[tid] = (red64[0][tid] + red64[1][tid]) + (red64[2][tid] + red64[3][tid]);
+                if (tid < 64) {
+                    float t = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < NW; q++) t += red64[q][tid];
+                    hprobs[tid] = t;
+                }
                 __syncthreads();
                 if (tid == 0) {
                     float sumHP = 0.0f;
@@ -539,34 +538,71 @@ void phase_prep_launch(int n, PhItem *d_items, const CondItem *d_cond, const int
     GH_HIP(hipGetLastError());
 }
 
-template <int KPT>
+template <int NW, int KPT>
 static void phase_go(gh_panel *P, int n, const PhItem *d_items, const CondItem *d_cond, const uint64_t *d_hvar,
                      const Locus *d_loci, const int32_t *d_pos, double nrho, PhaseBufs &B, uint8_t *d_H0,
                      uint8_t *d_H1, float ee, float ed, uint32_t seed, uint32_t iter, int32_t *d_status,
                      hipStream_t s) {
-    const size_t smem = (size_t)PH_TILE * ((KPT * PH_NT + 63) / 64) * 8;
-    hipLaunchKernelGGL(phase_kernel<KPT>, dim3(n), dim3(PH_NT), smem, s, d_items, d_cond, d_hvar, d_loci, d_pos,
-                       P->d_cm.as<double>(), nrho, B.seg, B.segsum, B.imp, B.impsum, B.of1, B.dip, d_H0, d_H1,
+    const size_t smem = (size_t)PH_TILE * ((KPT * 64 * NW + 63) / 64) * 8;
+    hipLaunchKernelGGL((phase_kernel<NW, KPT>), dim3(n), dim3(64 * NW), smem, s, d_items, d_cond, d_hvar, d_loci,
+                       d_pos, P->d_cm.as<double>(), nrho, B.seg, B.segsum, B.imp, B.impsum, B.of1, B.dip, d_H0, d_H1,
                        P->L, ee, ed, seed, iter, d_status);
+}
+
+// Launch shapes (waves per target, states per thread) the kernel is built for,
+// by state-count class; the tuning table picks one per class.
+static const PhaseShape kPhaseShapes[] = {
+    {1, 1}, {1, 2}, {1, 4}, {2, 1}, {2, 2}, {4, 1},                 // K <= 256
+    {2, 4}, {4, 2}, {8, 1},                                         // K <= 512
+    {4, 4}, {8, 2}, {16, 1},                                        // K <= 1024
+    {4, 8}, {8, 4}, {16, 2},                                        // K <= 2048
+};
+
+std::vector<PhaseShape> phase_shapes(int maxK) {
+    std::vector<PhaseShape> v;
+    for (const PhaseShape &sh : kPhaseShapes) {
+        const int cap = 64 * sh.NW * sh.KPT;
+        if (cap >= maxK && cap < 2 * std::max(maxK, 128)) v.push_back(sh);
+    }
+    return v;
+}
+
+PhaseShape phase_shape(int maxK) {
+    GH_CHECK(maxK <= 2048, GH_ERR_ARG, "phasing: more than 2048 conditioning states is not supported");
+    // GH_PHASE_SHAPE=<NW>x<KPT> pins a shape wherever it fits (testing, reproducing a tuned run)
+    if (const char *e = std::getenv("GH_PHASE_SHAPE")) {
+        int nw = 0, kpt = 0;
+        if (std::sscanf(e, "%dx%d", &nw, &kpt) == 2)
+            for (const PhaseShape &sh : phase_shapes(maxK))
+                if (sh.NW == nw && sh.KPT == kpt) return sh;
+    }
+    const int cls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
+    static const PhaseShape dflt[4] = {{4, 1}, {4, 2}, {8, 2}, {8, 4}};
+    return dflt[cls];
 }
 
 void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
                   const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
                   uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
-                  hipStream_t s) {
+                  hipStream_t s, PhaseShape sh) {
     if (n <= 0) return;
     const double nrho = (-0.04 * ne) / P->N;
     const float ed = err_phase, ee = (float)(1.0 - (double)err_phase);
-    if (maxK <= 256) phase_go<1>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
-    else if (maxK <= 512) phase_go<2>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
-    else if (maxK <= 1024) phase_go<4>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
-    else if (maxK <= 2048) phase_go<8>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, d_status, s);
-    else throw Error(GH_ERR_ARG, "phasing: more than 2048 conditioning states is not supported");
-    GH_HIP(hipGetLastError());
+    if (sh.NW == 0) sh = phase_shape(maxK);
+    GH_CHECK(64 * sh.NW * sh.KPT >= maxK, GH_ERR_ARG, "phasing: launch shape smaller than the conditioning set");
+#define GH_PH(nw, kpt)                                                                                          \
+    if (sh.NW == nw && sh.KPT == kpt) {                                                                         \
+        phase_go<nw, kpt>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, \
+                          d_status, s);                                                                         \
+        GH_HIP(hipGetLastError());                                                                              \
+        return;                                                                                                 \
+    }
+    GH_PH(1, 1) GH_PH(1, 2) GH_PH(1, 4) GH_PH(2, 1) GH_PH(2, 2) GH_PH(4, 1) GH_PH(2, 4) GH_PH(4, 2) GH_PH(8, 1)
+    GH_PH(4, 4) GH_PH(8, 2) GH_PH(16, 1) GH_PH(4, 8) GH_PH(8, 4) GH_PH(16, 2)
+#undef GH_PH
+    throw Error(GH_ERR_ARG, "phasing: no kernel for the requested launch shape");
 }
 
-int phase_kpt_states(int maxK) {
-    return maxK <= 256 ? 256 : maxK <= 512 ? 512 : maxK <= 1024 ? 1024 : 2048;
-}
+int phase_kpt_states(PhaseShape sh) { return 64 * sh.NW * sh.KPT; }
 
 }  // namespace gh

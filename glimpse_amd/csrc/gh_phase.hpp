@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <vector>
+
 #include "gh_internal.hpp"
 
 namespace gh {
@@ -42,11 +44,20 @@ struct PhaseBufs {
 void phase_prep_launch(int n, PhItem *d_items, const CondItem *d_cond, const int32_t *d_poly, const uint8_t *d_H0,
                        const uint8_t *d_H1, const uint8_t *d_flat, const uint8_t *d_flags, Locus *d_loci,
                        hipStream_t s);
+// Launch shape of the phasing kernel: NW waves per target, KPT states per thread.
+struct PhaseShape {
+    int NW, KPT;
+};
+// Shapes the kernel is built for that fit maxK states (the tuner's candidates).
+std::vector<PhaseShape> phase_shapes(int maxK);
+// The shape for maxK states: the tuned one for its class if set, else the default.
+PhaseShape phase_shape(int maxK);
+// sh.NW == 0: phase_shape(maxK).
 void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
                   const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
                   uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
-                  hipStream_t s);
-// Padded state count the phasing kernel uses for stored messages (K x 8 floats each).
-int phase_kpt_states(int maxK);
+                  hipStream_t s, PhaseShape sh = PhaseShape{0, 0});
+// Padded state count of a shape (stored messages are that many x 8 floats).
+int phase_kpt_states(PhaseShape sh);
 
 }  // namespace gh

@@ -40,7 +40,8 @@ class RunStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "fb_state_sites", "fb_poly_sites", "ph_state_sites", "ph_segments", "pbwt_hap_sites",
         "ms_select", "ms_condset", "ms_fb", "ms_phase", "ms_other", "ms_total",
-        "fb_kernel_ms", "fb_launches", "fb_bytes", "ms_learn", "learned_sel", "pool_kept", "collected_rows")]
+        "fb_kernel_ms", "fb_launches", "fb_bytes", "ms_learn", "learned_sel", "pool_kept", "collected_rows",
+        "fb_nw", "fb_mwl", "ph_nw", "ph_kpt")]
 
 MLP_IN, MLP_HIDDEN, MLP_NPARAM = 32, 128, 20865
 
@@ -77,6 +78,7 @@ def lib():
         L.gh_mlp_step.argtypes = [vp, i32, vp, vp, vp, f32, vp]
         L.gh_mlp_train_run.argtypes = [vp, vp, i32, i32, f32, u32, vp]
         L.gh_run_collect.argtypes = [vp, i32]
+        L.gh_run_set_tuning.argtypes = [vp, i32]
         L.gh_run_collected.argtypes = [vp, vp, vp, vp, vp]
         L.gh_run_set_selector.argtypes = [vp, vp, vp, f32, i32]
         L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
@@ -326,6 +328,10 @@ class Run:
 
     def execute(self):
         _check(lib().gh_run_execute(self.h))
+
+    def set_tuning(self, on=True):
+        """Launch-shape autotuning over the first executions (see gh_run_set_tuning)."""
+        _check(lib().gh_run_set_tuning(self.h, int(bool(on))))
 
     def collect(self, from_iter=1):
         """Record (features, label, weight) rows for every PBWT pool candidate at

@@ -86,6 +86,8 @@ typedef struct {
     double learned_sel;        /* target-iterations that used the learned selection */
     double pool_kept;          /* target-iterations that fell back to the whole PBWT pool */
     double collected_rows;     /* training rows recorded by gh_run_collect */
+    double fb_nw, fb_mwl;      /* forward-backward launch shape of the last iteration (waves, words per lane) */
+    double ph_nw, ph_kpt;      /* phasing launch shape of the last iteration (waves, states per thread) */
 } gh_run_stats;
 
 const char *gh_last_error(void);
@@ -164,6 +166,15 @@ int gh_run_execute(gh_run *r);
 int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1, int32_t *nstates,
                  gh_run_stats *stats);
 int gh_run_destroy(gh_run *r);
+/* Launch-shape autotuning (waves per target and work per lane of the
+ * forward-backward and phasing kernels), off by default.  When on, the PBWT
+ * iterations of the first executions each run the next candidate shape of the
+ * current set-size class and time it; once all candidates are timed the
+ * fastest is kept for the rest of the run's executions.  Shapes change only
+ * the summation order, so results agree to float rounding, but a run is
+ * bit-reproducible only with tuning off (or with GH_FB_SHAPE=<NW>x<MWL> and
+ * GH_PHASE_SHAPE=<NW>x<KPT> pinning the shapes). */
+int gh_run_set_tuning(gh_run *r, int32_t on);
 
 /* Output fields as genotype_writer::writeGenotypes computes them
  * (phase/src/io/genotype_writer.cpp:109-190), bit for bit, for T diploid

@@ -80,3 +80,18 @@ def test_fb_all_flat_and_lq(panels):
     HP = GP2.impute_batch([s], HL[None], flat[1][None])[0]
     ref = OP2.impute_hap(s, HL, flat[1])
     assert np.abs(HP - ref).max() <= FB_TOL
+
+
+@pytest.mark.parametrize("shape,K", [("1x4", 200), ("2x2", 256), ("4x1", 130), ("2x4", 500), ("4x2", 400),
+                                     ("8x1", 512), ("4x4", 1000), ("8x2", 700), ("16x1", 1024), ("4x8", 2000),
+                                     ("8x4", 1500), ("16x2", 1999)])
+def test_fb_every_launch_shape(panels, monkeypatch, shape, K):
+    """Every forward-backward launch shape (waves x words per lane) against the oracle."""
+    monkeypatch.setenv("GH_FB_SHAPE", shape)
+    P, gl, flat, OP, GP = panels
+    rng = np.random.default_rng(K)
+    s = np.sort(rng.choice(P["N"], K, replace=False))
+    HL = orc.hap_likelihoods(gl[0], flat[0], P["tgt_haps"][1])
+    HP = GP.impute_batch([s], HL[None], flat[0][None])[0]
+    ref = OP.impute_hap(s, HL, flat[0])
+    assert np.abs(HP - ref).max() <= FB_TOL

@@ -41,3 +41,23 @@ def test_rephase_matches_oracle(panels, K):
         assert n0 == 0 and n1 == 0, "target %d K=%d: %d/%d haplotype alleles differ" % (t, K, n0, n1)
         # the genotype (h0 + h1) is never changed by rephasing except at flat hets
         assert np.array_equal((g0[t] ^ g1[t]) | flat[t], (h0s[t] ^ h1s[t]) | flat[t]) or True
+
+
+# every launch shape the phasing kernel is built for, at a set size of its class
+_SHAPES = [(1, 1, 60), (1, 2, 100), (1, 4, 250), (2, 1, 100), (2, 2, 250), (4, 1, 256), (2, 4, 500), (4, 2, 400),
+           (8, 1, 512), (4, 4, 1000), (8, 2, 700), (16, 1, 1024), (4, 8, 2000), (8, 4, 1500), (16, 2, 1999)]
+
+
+@pytest.mark.parametrize("nw,kpt,K", _SHAPES)
+def test_rephase_every_launch_shape(panels, monkeypatch, nw, kpt, K):
+    monkeypatch.setenv("GH_PHASE_SHAPE", "%dx%d" % (nw, kpt))
+    P, gl, flat, OP, GP = panels
+    rng = np.random.default_rng(K + nw)
+    T = 2
+    sets = [np.sort(rng.choice(P["N"], K, replace=False)) for _ in range(T)]
+    h0s = np.stack([_perturb(P["tgt_haps"][2 * t], 0.02, rng) for t in range(T)])
+    h1s = np.stack([_perturb(P["tgt_haps"][2 * t + 1], 0.02, rng) for t in range(T)])
+    g0, g1 = GP.rephase_batch(sets, flat[:T], h0s, h1s, 99, 2, [7, 8])
+    for t in range(T):
+        r0, r1 = OP.rephase(sets[t], flat[t], h0s[t], h1s[t], 99, 2, 7 + t)
+        assert np.array_equal(g0[t], r0) and np.array_equal(g1[t], r1), "shape %dx%d K=%d target %d" % (nw, kpt, K, t)
