@@ -337,10 +337,10 @@ struct FbShape {
 };
 // Shapes the kernel is built for (NW waves x MWL words per lane covers 64*NW*MWL states).
 static const int kFbShapes[][2] = {
-    {1, 4}, {2, 2}, {4, 1},     // W <= 4
-    {2, 4}, {4, 2}, {8, 1},     // W <= 8
-    {4, 4}, {8, 2}, {16, 1},    // W <= 16
-    {4, 8}, {8, 4}, {16, 2},    // W <= 32
+    {1, 4}, {2, 2}, {4, 1},                     // W <= 4
+    {1, 8}, {2, 4}, {4, 2}, {8, 1},             // W <= 8
+    {1, 16}, {2, 8}, {4, 4}, {8, 2}, {16, 1},   // W <= 16
+    {1, 32}, {2, 16}, {4, 8}, {8, 4}, {16, 2},  // W <= 32
 };
 static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     FbShape s;
@@ -417,8 +417,9 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
         GH_HIP(hipGetLastError());        \
         return;                           \
     }
-    GH_FB(1, 4) GH_FB(2, 2) GH_FB(4, 1) GH_FB(2, 4) GH_FB(4, 2) GH_FB(8, 1)
-    GH_FB(4, 4) GH_FB(8, 2) GH_FB(16, 1) GH_FB(4, 8) GH_FB(8, 4) GH_FB(16, 2)
+    GH_FB(1, 4) GH_FB(2, 2) GH_FB(4, 1) GH_FB(1, 8) GH_FB(2, 4) GH_FB(4, 2) GH_FB(8, 1)
+    GH_FB(1, 16) GH_FB(2, 8) GH_FB(4, 4) GH_FB(8, 2) GH_FB(16, 1)
+    GH_FB(1, 32) GH_FB(2, 16) GH_FB(4, 8) GH_FB(8, 4) GH_FB(16, 2)
 #undef GH_FB
     throw Error(GH_ERR_ARG, "fb: no kernel for the requested launch shape");
 }

@@ -82,8 +82,9 @@ def test_fb_all_flat_and_lq(panels):
     assert np.abs(HP - ref).max() <= FB_TOL
 
 
-@pytest.mark.parametrize("shape,K", [("1x4", 200), ("2x2", 256), ("4x1", 130), ("2x4", 500), ("4x2", 400),
-                                     ("8x1", 512), ("4x4", 1000), ("8x2", 700), ("16x1", 1024), ("4x8", 2000),
+@pytest.mark.parametrize("shape,K", [("1x4", 200), ("2x2", 256), ("4x1", 130), ("1x8", 300), ("2x4", 500),
+                                     ("4x2", 400), ("8x1", 512), ("1x16", 900), ("2x8", 600), ("4x4", 1000),
+                                     ("8x2", 700), ("16x1", 1024), ("1x32", 1999), ("2x16", 1300), ("4x8", 2000),
                                      ("8x4", 1500), ("16x2", 1999)])
 def test_fb_every_launch_shape(panels, monkeypatch, shape, K):
     """Every forward-backward launch shape (waves x words per lane) against the oracle."""
