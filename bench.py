@@ -42,6 +42,11 @@ import sys
 import threading
 import time
 
+# One hardware queue per concurrently running chunk stream (HIP's default is 4;
+# with 5 chunks two of them would share a queue and run one after the other).
+# Must be set before the HIP runtime starts.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 

@@ -189,7 +189,7 @@ int gh_impute_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int3
             fb[i].alpha_off = aoff;
             fb[i].hl_off = (int64_t)i * 2 * L;
             fb[i].flat_off = (int64_t)i * L;
-            aoff += (int64_t)((B.h[i].Lp + C - 1) / C) * 64 * B.h[i].W;
+            aoff += fb_ckpt_floats(B.h[i].Lp, B.h[i].W, B.maxW);
         }
         DBuf d_fb, d_HL, d_flat, d_alpha, d_HP;
         dalloc_copy(d_fb, fb.data(), n, s);

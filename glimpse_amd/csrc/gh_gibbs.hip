@@ -373,7 +373,7 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             G->items[t] = CondItem{G->kmax[t], Wt, 0, 0, (int64_t)t * G->kstride, (int64_t)t * L, words};
             words += (int64_t)L * Wt;
             G->fb[t] = FbItem{0, 0, 0, t, aoff, (int64_t)t * 2 * L, (int64_t)t * L};
-            aoff += (int64_t)((L + C - 1) / C) * 64 * Wt;
+            aoff += fb_ckpt_floats(L, Wt, G->maxW);
         }
         dalloc_copy(G->d_items, G->items.data(), T, s);
         dalloc_copy(G->d_fb, G->fb.data(), T, s);

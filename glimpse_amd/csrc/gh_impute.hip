@@ -128,14 +128,46 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     int m_site = 0, m_code = 0;
     float m_e0 = 1.0f, m_e1 = 1.0f, m_h0 = 0.5f, m_h1 = 0.5f;
     float m_f1in = 0.0f, m_ntin = 1.0f, m_f1out = 0.0f, m_ntout = 1.0f;   // t/K and 1-t into / out of the site
-    auto load_tile = [&](int l0, int n) {
-        __syncthreads();   // previous tile's LDS rows are no longer read
+    // The next tile is fetched into registers while the current one is processed:
+    // stage 1 (site ids, transitions, Hvar rows) right after a tile starts,
+    // stage 2 (flags and likelihoods of those sites) once the ids have arrived.
+    // A tile then only costs the LDS stores and two barriers.
+    int pf_site = 0, pf_l0 = 0, pf_n = 0;
+    float pf_tin = 0.0f, pf_tout = 0.0f, pf_h0 = 0.5f, pf_h1 = 0.5f;
+    uint8_t pf_flat = 0, pf_flag = 0;
+    uint64_t pf_rows[MWL];
+    auto prefetch1 = [&](int l0, int n) {
+        pf_l0 = l0;
+        pf_n = n;
         if (lane < n) {
             const int l = l0 + lane;
-            const int site = poly[l];
-            const bool isflat = flat[site] != 0;
-            const bool fl = isflat || (A.flags[site] & SF_LQ);
-            const float h0 = HL[2 * site], h1 = HL[2 * site + 1];
+            pf_site = poly[l];
+            pf_tin = l > 0 ? tr[l - 1] : 0.0f;
+            pf_tout = l < Lp - 1 ? tr[l] : 0.0f;
+        }
+        const uint64_t *src = hv + (size_t)l0 * W;
+#pragma unroll
+        for (int k = 0; k < MWL; k++) {
+            const int e = tid + k * 64 * NW;
+            pf_rows[k] = e < n * W ? src[e] : 0ull;
+        }
+    };
+    auto prefetch2 = [&]() {
+        if (lane < pf_n) {
+            pf_flat = flat[pf_site];
+            pf_flag = A.flags[pf_site];
+            pf_h0 = HL[2 * pf_site];
+            pf_h1 = HL[2 * pf_site + 1];
+        }
+    };
+    // make the fetched tile current (every thread calls it)
+    auto commit_tile = [&]() {
+        __syncthreads();   // previous tile's LDS rows are no longer read
+        if (lane < pf_n) {
+            const int site = pf_site;
+            const bool isflat = pf_flat != 0;
+            const bool fl = isflat || (pf_flag & SF_LQ);
+            const float h0 = pf_h0, h1 = pf_h1;
             const float p0 = h0 * ee + h1 * ed, p1 = h0 * ed + h1 * ee;
             m_site = site;
             m_code = (fl ? 1 : 0) | (isflat ? 2 : 0);
@@ -143,15 +175,22 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
             m_e1 = p1 / (p0 + p1);
             m_h0 = h0;
             m_h1 = h1;
-            const float tin = l > 0 ? tr[l - 1] : 0.0f, tout = l < Lp - 1 ? tr[l] : 0.0f;
-            m_f1in = tin / Kf;
-            m_ntin = 1.0f - tin;
-            m_f1out = tout / Kf;
-            m_ntout = 1.0f - tout;
+            m_f1in = pf_tin / Kf;
+            m_ntin = 1.0f - pf_tin;
+            m_f1out = pf_tout / Kf;
+            m_ntout = 1.0f - pf_tout;
         }
-        const uint64_t *src = hv + (size_t)l0 * W;
-        for (int e = tid; e < n * W; e += 64 * NW) rows[e] = src[e];
+#pragma unroll
+        for (int k = 0; k < MWL; k++) {
+            const int e = tid + k * 64 * NW;
+            if (e < pf_n * W) rows[e] = pf_rows[k];
+        }
         __syncthreads();
+    };
+    auto load_tile = [&](int l0, int n) {
+        prefetch1(l0, n);
+        prefetch2();
+        commit_tile();
     };
 
     float a[MWL];
@@ -186,19 +225,24 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     // ---------------- forward: checkpoints every C sites
     {
         float sp = 1.0f;
+        load_tile(0, min(FB_TILE, Lp));
         for (int l0 = 0; l0 < Lp; l0 += FB_TILE) {
             const int n = min(FB_TILE, Lp - l0);
-            load_tile(l0, n);
+            if (l0 > 0) commit_tile();
+            const bool more = l0 + FB_TILE < Lp;
+            if (more) prefetch1(l0 + FB_TILE, min(FB_TILE, Lp - l0 - FB_TILE));
             for (int j = 0; j < n; j++) {
                 const int l = l0 + j;
+                if (more && j == 24) prefetch2();
                 fwd_site(j, l, sp);
-                if (l % C == 0) {
+                if ((l & (C - 1)) == 0) {
                     float *dst = ck + (size_t)(l / C) * vstride + lane;
 #pragma unroll
                     for (int i = 0; i < MWL; i++)
                         if (wok[i]) dst[64 * wid[i]] = a[i];
                 }
             }
+            if (more && n <= 24) prefetch2();
         }
     }
 
@@ -218,15 +262,26 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     for (int i = 0; i < MWL; i++) { b[i] = 0.0f; ms[i] = 0.0f; }
     float bsum_next = 1.0f;
     const int last_tile = ((Lp - 1) / FB_TILE) * FB_TILE;
+    // checkpoint of the next block to process, fetched one block ahead
+    float nck[MWL];
+    auto fetch_ck = [&](int bs) {
+        const float *srcv = ck + (size_t)(bs / C) * vstride + lane;
+#pragma unroll
+        for (int i = 0; i < MWL; i++) nck[i] = wok[i] ? srcv[64 * wid[i]] : 0.0f;
+    };
+    fetch_ck(((Lp - 1) / C) * C);
+    load_tile(last_tile, Lp - last_tile);
     for (int l0 = last_tile; l0 >= 0; l0 -= FB_TILE) {
         const int n = min(FB_TILE, Lp - l0);
-        load_tile(l0, n);
+        if (l0 < last_tile) commit_tile();
+        const bool more = l0 > 0;
+        if (more) prefetch1(l0 - FB_TILE, FB_TILE);
         for (int q = (n - 1) / C; q >= 0; q--) {
             const int bs = l0 + q * C, be = min(bs + C, l0 + n);
             // recompute alpha on [bs, be) from the checkpoint at bs
-            const float *srcv = ck + (size_t)(bs / C) * vstride + lane;
 #pragma unroll
-            for (int i = 0; i < MWL; i++) a[i] = wok[i] ? srcv[64 * wid[i]] : 0.0f;
+            for (int i = 0; i < MWL; i++) a[i] = nck[i];
+            if (bs > 0) fetch_ck(bs - C);
             float sp;
             {
                 const bool fl0 = rli(m_code, bs - l0) & 1;
@@ -242,6 +297,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
 #pragma unroll
                 for (int i = 0; i < MWL; i++) ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
             }
+            if (more && q == (n - 1) / C) prefetch2();   // the next tile's site ids have arrived by now
             // backward recursion on the block
             for (int l = be - 1; l >= bs; l--) {
                 const int j = l - l0;
@@ -387,6 +443,13 @@ std::vector<FbPick> fb_shapes(int maxW) {
 // The checkpoint interval only depends on the words a shape covers (NW x MWL),
 // which every shape of a capacity class shares.
 int fb_block_sites(int maxW) { return fb_shape(maxW).C; }
+
+// Checkpoint floats of one haplotype item with up to Lp polymorphic sites and W
+// words: every C-th alpha vector (64 W floats).
+int64_t fb_ckpt_floats(int Lp, int W, int maxW) {
+    const int C = fb_block_sites(maxW);
+    return (int64_t)((Lp + C - 1) / C) * 64 * W;
+}
 
 template <int NW, int MWL>
 static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {

@@ -109,7 +109,7 @@ void condset_build(gh_panel *P, int n, CondItem *d_items, const CondItem *h_item
                    int ne, int maxW, hipStream_t s);
 
 // FB over n haploid items.  HL [item][2L], flat [item][L] (device); alpha
-// checkpoints per item: ceil(Lp/C)*64W floats at alpha_off[i], C = fb_block_sites(maxW).
+// checkpoints per item: fb_ckpt_floats(Lp, W, maxW) floats at alpha_off[i].
 // HP out [item][2L]; optional per-state posterior mass at mass + state_off,
 // summed over the polymorphic sites inside the state's [win_lo, win_hi) when
 // those per-state windows are given (at win + state_off), else over all sites.
@@ -132,6 +132,7 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
 std::vector<FbPick> fb_shapes(int maxW);
 FbPick fb_default_pick(int maxW);
 int fb_block_sites(int maxW);
+int64_t fb_ckpt_floats(int Lp, int W, int maxW);
 // HP at monomorphic sites (closed form) for the same items.
 void fb_mono_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_pos,
                     const float *d_HL, const uint8_t *d_flat, float *d_HP, float ee, float ed, hipStream_t s);
