@@ -45,7 +45,8 @@ import time
 # One hardware queue per concurrently running chunk stream (HIP's default is 4;
 # with 5 chunks two of them would share a queue and run one after the other).
 # Must be set before the HIP runtime starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -231,6 +232,8 @@ def main():
     ap.add_argument("--k-frac", type=float, default=0.2, help="learned path: kept fraction of the PBWT pool")
     ap.add_argument("--min-mass", type=float, default=0.0,
                     help="learned path: keep the whole pool below this predicted mass share")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (no autotuning during the warm-up steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB three times (HBM counter calibration)")
     a = ap.parse_args()
@@ -257,6 +260,9 @@ def main():
     panels = [hip.Panel(c["panel"]) for c in chunks]
     prm = hip.default_params()
     runs = [hip.Run(p, c["gl"], c["flat"], prm, kpbwt_per_target=c["K"]) for p, c in zip(panels, chunks)]
+    if not a.no_tune:
+        for r in runs:
+            r.set_tuning(True)   # the warm-up executions time each launch shape and keep the fastest
     setup_s = time.perf_counter() - t_setup
     print("[bench] setup %.1f s (%s)" % (setup_s, desc), file=sys.stderr, flush=True)
     tsites = sum(len(c["targets"]) * c["panel"]["L"] for c in chunks)
@@ -301,6 +307,8 @@ def main():
         "roofline": fb_roofline(M["outs"], main_path if a.quick is False else None),
         "accuracy": M["accuracy"],
         "stage_ms_summed_over_chunks": M["stage_ms_summed_over_chunks"],
+        "launch_shapes": [dict(fb="%dx%d" % (o["stats"]["fb_nw"], o["stats"]["fb_mwl"]),
+                               phase="%dx%d" % (o["stats"]["ph_nw"], o["stats"]["ph_kpt"])) for o in M["outs"]],
         "setup_s": setup_s,
     }
     if "learned" in res:
