@@ -12,6 +12,18 @@ template <int CTRL>
 __device__ __forceinline__ float dppf(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
+// Value of lane l ^ 16 / l ^ 32 (v_permlane16_swap / v_permlane32_swap with
+// both operands v: the first result holds the even rows' (low half's) values in
+// both rows (halves) of a pair, the second the odd rows' (high half's)).
+__device__ __forceinline__ float partner16(float v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float((threadIdx.x & 16) ? r[0] : r[1]);
+}
+__device__ __forceinline__ float partner32(float v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float((threadIdx.x & 32) ? r[0] : r[1]);
+}
+
 // Sum of v over the 64 lanes, returned wave-uniform.  Four DPP adds reduce each
 // 16-lane row (xor1, xor2, half-mirror, mirror), then the four row sums are
 // read with v_readlane.
@@ -42,7 +54,8 @@ __device__ __forceinline__ void wave_sum3(float &x, float &y, float &z) {
 // Reduce-scatter of 8 per-lane values over the wave: returns, in every lane l,
 // the wave total of value (l & 7).  Three DPP halving steps inside each group
 // of 8 lanes (partner 7-i, i^2, i^1; each lane keeps the half its lane bit
-// selects), then the 8 groups are combined (row_ror:8, xor 16, xor 32).
+// selects), then the 8 groups are combined (row_ror:8, then the xor-16 and
+// xor-32 partners by permlane swaps).
 __device__ __forceinline__ float wave_rs8(const float v[8]) {
     const int i = threadIdx.x & 7;
     const bool b2 = i & 4, b1 = i & 2, b0 = i & 1;
@@ -53,8 +66,8 @@ __device__ __forceinline__ float wave_rs8(const float v[8]) {
     for (int k = 0; k < 2; k++) w[k] = (b1 ? u[2 + k] : u[k]) + dppf<0x4E>(b1 ? u[k] : u[2 + k]);
     float x = (b0 ? w[1] : w[0]) + dppf<0xB1>(b0 ? w[0] : w[1]);
     x += dppf<0x128>(x);
-    x += __shfl_xor(x, 16);
-    x += __shfl_xor(x, 32);
+    x += partner16(x);
+    x += partner32(x);
     return x;
 }
 
@@ -77,8 +90,8 @@ __device__ __forceinline__ float wave_rs64(const float v[64]) {
 #pragma unroll
     for (int k = 0; k < 4; k++) d[k] = (b3 ? c[4 + k] : c[k]) + dppf<0x128>(b3 ? c[k] : c[4 + k]);
 #pragma unroll
-    for (int k = 0; k < 2; k++) e[k] = (b4 ? d[2 + k] : d[k]) + __shfl_xor(b4 ? d[k] : d[2 + k], 16);
-    return (b5 ? e[1] : e[0]) + __shfl_xor(b5 ? e[0] : e[1], 32);
+    for (int k = 0; k < 2; k++) e[k] = (b4 ? d[2 + k] : d[k]) + partner16(b4 ? d[k] : d[2 + k]);
+    return (b5 ? e[1] : e[0]) + partner32(b5 ? e[0] : e[1]);
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
