@@ -88,14 +88,14 @@ def build_workload(quick, seed=15052011):
     truth = synth.truth_genotypes(P["tgt_haps"])
     chunks = []
     for c in range(n_chunks):
-        tg = np.arange(c, T, n_chunks)
+        tg = np.arange(T)   # every chunk (a genomic region) is phased for every target sample
         o, ln = offs[c], int(lens[c])
         chunks.append(dict(panel=slice_panel(P, o, ln), targets=tg,
                            gl=np.ascontiguousarray(gl[tg][:, o:o + ln]),
                            flat=np.ascontiguousarray(flat[tg][:, o:o + ln]),
                            K=K[tg].astype(np.int32), truth=truth[tg][:, o:o + ln]))
-    desc = ("batched plain PBWT: %d haps x %d-site panel, %d targets at U(0.1,2)x, %d chunks of U[%d,%d] sites, "
-            "Kpbwt U[%d,%d], 1+5+15 Gibbs iterations" % (N, Lp, T, n_chunks, lo, hi, kmin, kmax))
+    desc = ("batched plain PBWT: %d haps x %d-site panel, %d targets at U(0.1,2)x, each phased on %d chunks of "
+            "U[%d,%d] sites, Kpbwt U[%d,%d] per target, 1+5+15 Gibbs iterations" % (N, Lp, T, n_chunks, lo, hi, kmin, kmax))
     return chunks, desc, dict(N=N, panel_sites=Lp, targets=T, chunks=n_chunks, chunk_sites=[int(x) for x in lens])
 
 
@@ -271,6 +271,8 @@ def main():
     if a.path in ("both", "plain"):
         dt, outs = timed(runs, a, "plain")
         res["plain"] = dict(value=tsites / dt, ms_per_step=dt * 1e3, outs=outs)
+    free_b, total_b = torch.cuda.mem_get_info()
+    hbm_used_gb = (total_b - free_b) / 1e9   # device memory in use with every chunk's run resident
     train = None
     if a.path in ("both", "learned"):
         net, train = train_selector()
@@ -310,6 +312,7 @@ def main():
         "launch_shapes": [dict(fb="%dx%d" % (o["stats"]["fb_nw"], o["stats"]["fb_mwl"]),
                                phase="%dx%d" % (o["stats"]["ph_nw"], o["stats"]["ph_kpt"])) for o in M["outs"]],
         "setup_s": setup_s,
+        "hbm_used_gb": round(hbm_used_gb, 1),
     }
     if "learned" in res:
         L_ = res["learned"]

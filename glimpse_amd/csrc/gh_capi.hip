@@ -262,12 +262,12 @@ int gh_rephase_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int
         GH_HIP(hipMemcpyAsync(ph.data(), d_ph.p, sizeof(PhItem) * n, hipMemcpyDeviceToHost, s));
         GH_HIP(hipStreamSynchronize(s));
         const PhaseShape psh = phase_shape(maxK);
-        const int64_t KH = (int64_t)phase_kpt_states(psh) * 8;
         int64_t so = 0, ss = 0, io = 0, is = 0, dd = 0;
         for (int i = 0; i < n; i++) {
-            ph[i].seg_off = so; so += ph[i].n_segs * KH;
+            ph[i].msg = phase_msg_floats(B.h[i].K);
+            ph[i].seg_off = so; so += (int64_t)ph[i].n_segs * ph[i].msg;
             ph[i].segsum_off = ss; ss += ph[i].n_segs * 9;
-            ph[i].imp_off = io; io += ph[i].n_miss * KH;
+            ph[i].imp_off = io; io += (int64_t)ph[i].n_miss * ph[i].msg;
             ph[i].impsum_off = is; is += ph[i].n_miss * 8;
             ph[i].dip_off = dd; dd += ph[i].n_segs;
         }

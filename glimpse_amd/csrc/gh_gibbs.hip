@@ -702,12 +702,12 @@ void run_execute(gh_run *G) {
         const int kcls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
         const PhaseShape phs = (G->tune && stage > 0) ? G->ph_tuner.pick(kcls, phase_shapes(maxK), phase_shape(maxK))
                                                       : phase_shape(maxK);
-        const int64_t KH = (int64_t)phase_kpt_states(phs) * 8;
         int64_t so = 0, ss = 0, io = 0, is = 0, dd = 0;
         for (int t = 0; t < T; t++) {
-            ph[t].seg_off = so; so += ph[t].n_segs * KH;
+            ph[t].msg = phase_msg_floats(items[t].K);
+            ph[t].seg_off = so; so += (int64_t)ph[t].n_segs * ph[t].msg;
             ph[t].segsum_off = ss; ss += ph[t].n_segs * 9;
-            ph[t].imp_off = io; io += ph[t].n_miss * KH;
+            ph[t].imp_off = io; io += (int64_t)ph[t].n_miss * ph[t].msg;
             ph[t].impsum_off = is; is += ph[t].n_miss * 8;
             ph[t].dip_off = dd; dd += ph[t].n_segs;
             st.fb_state_sites += 2.0 * items[t].K * items[t].Lp;

@@ -183,7 +183,8 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     uint8_t *H0 = H0_all + it.h0_off, *H1 = H1_all + it.h1_off;
     const float Kf = (float)K, invK = 1.0f / Kf;
     const float mism = ed / ee;
-    const size_t KH = (size_t)PH_NT * KPT * 8;  // stored message stride (padded states)
+    const size_t KH = (size_t)it.msg;           // stored message stride (states rounded up to 64)
+    const int Kr = it.msg >> 3;
 
     bool valid[KPT];
     int kk[KPT];
@@ -324,9 +325,11 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
 #pragma unroll
         for (int i = 0; i < KPT; i++) sumK[i] = hadd8(p[i]);
     };
+    // a wave whose 64 states all lie beyond K (rounded up to 64) stores nothing
     auto store_msg = [&](float *dst, float *dsum, float *dsumT) {
 #pragma unroll
         for (int i = 0; i < KPT; i++) {
+            if (PH_NT * i + 64 * wid >= Kr) continue;
             float4 *d4 = (float4 *)(dst + ((size_t)i * PH_NT + tid) * 8);
             d4[0] = make_float4(p[i][0], p[i][1], p[i][2], p[i][3]);
             d4[1] = make_float4(p[i][4], p[i][5], p[i][6], p[i][7]);
@@ -475,8 +478,12 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
 #pragma unroll
                 for (int i = 0; i < KPT; i++) {
                     const bool ah = valid[i] && bit_of(j, i);
-                    const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
-                    const float4 a = i4[0], b = i4[1];
+                    float4 a = make_float4(0, 0, 0, 0), b = a;
+                    if (PH_NT * i + 64 * wid < Kr) {
+                        const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
+                        a = i4[0];
+                        b = i4[1];
+                    }
                     const float iv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
                     for (int h = 0; h < 8; h++) {

@@ -24,7 +24,8 @@ struct Locus {
 struct PhItem {
     int32_t cond;        // CondItem index
     int32_t tgt;         // target index (keys the counter draws)
-    int32_t n_loci, n_segs, n_miss, pad;
+    int32_t n_loci, n_segs, n_miss;
+    int32_t msg;         // floats per stored message: 8 per state, states rounded up to 64
     int64_t h0_off;      // row of H0 in the H0 array
     int64_t h1_off;      // row of H1 in the H1 array
     int64_t flat_off;    // row of the flat flags
@@ -57,7 +58,9 @@ void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const Con
                   const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
                   uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
                   hipStream_t s, PhaseShape sh = PhaseShape{0, 0});
-// Padded state count of a shape (stored messages are that many x 8 floats).
+// Padded state count of a shape.
 int phase_kpt_states(PhaseShape sh);
+// Floats per stored message of a K-state target (8 per state, K rounded up to 64).
+inline int32_t phase_msg_floats(int K) { return ((K + 63) / 64) * 64 * 8; }
 
 }  // namespace gh
