@@ -34,7 +34,7 @@
 
 namespace gh {
 
-constexpr int FB_TILE = 64;
+constexpr int FB_TILE = 32;
 constexpr int FB_LDS_BYTES = 156 * 1024;
 
 struct FbArgs {
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     const FbItem f = A.fb[blockIdx.x];
     const CondItem c = A.cond[f.cond];
     const int K = c.K, W = c.W, Lp = c.Lp, C = A.C;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (Lp <= 0) return;
     uint64_t *rows = (uint64_t *)smem;                                    // [FB_TILE][W]
     float *ablk = (float *)(smem + (size_t)FB_TILE * A.maxW * 8);        // [C][NW*MWL][64]
@@ -82,11 +82,15 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     const float Kf = (float)K, invK = 1.0f / Kf;
     const size_t vstride = (size_t)64 * W;   // one alpha vector (checkpoint) in floats
 
+    // word slot i of this wave is word i*NW + wv; slots i >= nI hold no word of
+    // this item and are skipped (a wave-uniform bound: items with fewer states
+    // than the launch capacity do proportionally less work)
+    const int nI = min(MWL, max(0, (W - wv + NW - 1) / NW));
     int wid[MWL];
     bool wok[MWL], valid[MWL];
 #pragma unroll
     for (int i = 0; i < MWL; i++) {
-        wok[i] = i * NW + wv < W;
+        wok[i] = i < nI;
         wid[i] = wok[i] ? i * NW + wv : 0;   // clamped so row reads stay inside the tile
         valid[i] = wok[i] && (64 * wid[i] + lane < K);
     }
@@ -194,6 +198,8 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     };
 
     float a[MWL];
+#pragma unroll
+    for (int i = 0; i < MWL; i++) a[i] = 0.0f;
     // forward step at site l (tile offset j): alpha in a[], scaling sum in sp
     auto fwd_site = [&](int j, int l, float &sp) {
         const int code = rli(m_code, j);
@@ -203,6 +209,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
         if (l == 0) {
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
+                if (i >= nI) break;
                 const bool b = (rows[j * W + wid[i]] >> lane) & 1ull;
                 const float x = fl ? invK : (b ? e1 : e0) * invK;
                 a[i] = valid[i] ? x : 0.0f;
@@ -213,6 +220,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
             const float f1 = rlf(m_f1in, j), f2 = rlf(m_ntin, j) * __builtin_amdgcn_rcpf(sp);
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
+                if (i >= nI) break;
                 const bool b = (rows[j * W + wid[i]] >> lane) & 1ull;
                 const float x = fmaf(a[i], f2, f1) * (b ? e1 : e0);
                 a[i] = valid[i] ? x : 0.0f;
@@ -233,16 +241,18 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
             if (more) prefetch1(l0 + FB_TILE, min(FB_TILE, Lp - l0 - FB_TILE));
             for (int j = 0; j < n; j++) {
                 const int l = l0 + j;
-                if (more && j == 24) prefetch2();
+                if (more && j == FB_TILE / 2) prefetch2();
                 fwd_site(j, l, sp);
                 if ((l & (C - 1)) == 0) {
                     float *dst = ck + (size_t)(l / C) * vstride + lane;
 #pragma unroll
-                    for (int i = 0; i < MWL; i++)
-                        if (wok[i]) dst[64 * wid[i]] = a[i];
+                    for (int i = 0; i < MWL; i++) {
+                        if (i >= nI) break;
+                        dst[64 * wid[i]] = a[i];
+                    }
                 }
             }
-            if (more && n <= 24) prefetch2();
+            if (more && n <= FB_TILE / 2) prefetch2();
         }
     }
 
@@ -264,10 +274,15 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     const int last_tile = ((Lp - 1) / FB_TILE) * FB_TILE;
     // checkpoint of the next block to process, fetched one block ahead
     float nck[MWL];
+#pragma unroll
+    for (int i = 0; i < MWL; i++) nck[i] = 0.0f;
     auto fetch_ck = [&](int bs) {
         const float *srcv = ck + (size_t)(bs / C) * vstride + lane;
 #pragma unroll
-        for (int i = 0; i < MWL; i++) nck[i] = wok[i] ? srcv[64 * wid[i]] : 0.0f;
+        for (int i = 0; i < MWL; i++) {
+            if (i >= nI) break;
+            nck[i] = srcv[64 * wid[i]];
+        }
     };
     fetch_ck(((Lp - 1) / C) * C);
     load_tile(last_tile, Lp - last_tile);
@@ -287,15 +302,24 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                 const bool fl0 = rli(m_code, bs - l0) & 1;
                 float s = 0.0f;
 #pragma unroll
-                for (int i = 0; i < MWL; i++) s += a[i];
+                for (int i = 0; i < MWL; i++) {
+                    if (i >= nI) break;
+                    s += a[i];
+                }
                 sp = (bs == 0 && fl0) ? 1.0f : xsum1(s);
             }
 #pragma unroll
-            for (int i = 0; i < MWL; i++) ablk[((size_t)0 * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
+            for (int i = 0; i < MWL; i++) {
+                if (i >= nI) break;
+                ablk[((size_t)0 * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
+            }
             for (int l = bs + 1; l < be; l++) {
                 fwd_site(l - l0, l, sp);
 #pragma unroll
-                for (int i = 0; i < MWL; i++) ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
+                for (int i = 0; i < MWL; i++) {
+                    if (i >= nI) break;
+                    ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
+                }
             }
             if (more && q == (n - 1) / C) prefetch2();   // the next tile's site ids have arrived by now
             // backward recursion on the block
@@ -309,6 +333,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                 if (l == Lp - 1) {
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
+                        if (i >= nI) break;
                         const bool bit = (rows[j * W + wid[i]] >> lane) & 1ull;
                         const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
                         const float bv = fl ? invK : (bit ? e1 : e0) * invK;
@@ -322,6 +347,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                     const float f1 = rlf(m_f1out, j), f2 = rlf(m_ntout, j) * __builtin_amdgcn_rcpf(bsum_next);
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
+                        if (i >= nI) break;
                         const bool bit = (rows[j * W + wid[i]] >> lane) & 1ull;
                         const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
                         const float bt = valid[i] ? fmaf(b[i], f2, f1) : 0.0f;
@@ -339,6 +365,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                     const float inv = 1.0f / (p0 + p1);
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
+                        if (i >= nI) break;
                         const int site = rli(m_site, j);
                         const bool in = site >= wlo[i] && site < whi[i];
                         ms[i] = in ? fmaf(dpv[i], inv, ms[i]) : ms[i];
@@ -396,7 +423,7 @@ static const int kFbShapes[][2] = {
     {1, 4}, {2, 2}, {4, 1},                     // W <= 4
     {1, 8}, {2, 4}, {4, 2}, {8, 1},             // W <= 8
     {1, 16}, {2, 8}, {4, 4}, {8, 2}, {16, 1},   // W <= 16
-    {1, 32}, {2, 16}, {4, 8}, {8, 4}, {16, 2},  // W <= 32
+    {2, 16}, {4, 8}, {8, 4}, {16, 2},           // W <= 32 (one wave of 32 words spills)
 };
 static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     FbShape s;
@@ -405,7 +432,11 @@ static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     const size_t rows = (size_t)FB_TILE * maxW * 8;
     const size_t per_site = (size_t)s.NW * s.MWL * 64 * 4;
     const size_t red = 2 * 16 * 4 * 4;   // [2][NW <= 16][4] floats
-    int C = 64;
+    // The longest block (C divides FB_TILE) that leaves room for two workgroups
+    // per CU (the register budget of the larger shapes allows two), unless that
+    // would make blocks shorter than 8 sites; else the longest that fits.
+    int C = FB_TILE;
+    while (C > 8 && rows + C * per_site + red > (size_t)FB_LDS_BYTES / 2) C >>= 1;
     while (C > 1 && rows + C * per_site + red > (size_t)FB_LDS_BYTES) C >>= 1;
     s.C = C;
     s.smem = rows + C * per_site + red;
@@ -482,7 +513,7 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
     }
     GH_FB(1, 4) GH_FB(2, 2) GH_FB(4, 1) GH_FB(1, 8) GH_FB(2, 4) GH_FB(4, 2) GH_FB(8, 1)
     GH_FB(1, 16) GH_FB(2, 8) GH_FB(4, 4) GH_FB(8, 2) GH_FB(16, 1)
-    GH_FB(1, 32) GH_FB(2, 16) GH_FB(4, 8) GH_FB(8, 4) GH_FB(16, 2)
+    GH_FB(2, 16) GH_FB(4, 8) GH_FB(8, 4) GH_FB(16, 2)
 #undef GH_FB
     throw Error(GH_ERR_ARG, "fb: no kernel for the requested launch shape");
 }
