@@ -94,6 +94,28 @@ __device__ __forceinline__ float wave_rs64(const float v[64]) {
     return (b5 ? e[1] : e[0]) + partner32(b5 ? e[0] : e[1]);
 }
 
+// The same for 32 values: five halving steps (lane bits 2, 1, 0, 3, 4), then the
+// two wave halves are added, so lanes l and l ^ 32 both hold the wave total of
+// value rs32_index(l) = 16*b2 + 8*b1 + 4*b0 + 2*b3 + b4.
+__device__ __forceinline__ int rs32_index(int l) {
+    return 16 * ((l >> 2) & 1) + 8 * ((l >> 1) & 1) + 4 * (l & 1) + 2 * ((l >> 3) & 1) + ((l >> 4) & 1);
+}
+__device__ __forceinline__ float wave_rs32(const float v[32]) {
+    const int l = threadIdx.x & 63;
+    const bool b2 = l & 4, b1 = l & 2, b0 = l & 1, b3 = l & 8, b4 = l & 16;
+    float b[16], c[8], d[4], e[2];
+#pragma unroll
+    for (int k = 0; k < 16; k++) b[k] = (b2 ? v[16 + k] : v[k]) + dppf<0x141>(b2 ? v[k] : v[16 + k]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = (b1 ? b[8 + k] : b[k]) + dppf<0x4E>(b1 ? b[k] : b[8 + k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) d[k] = (b0 ? c[4 + k] : c[k]) + dppf<0xB1>(b0 ? c[k] : c[4 + k]);
+#pragma unroll
+    for (int k = 0; k < 2; k++) e[k] = (b3 ? d[2 + k] : d[k]) + dppf<0x128>(b3 ? d[k] : d[2 + k]);
+    const float x = (b4 ? e[1] : e[0]) + partner16(b4 ? e[0] : e[1]);
+    return x + partner32(x);
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 template <typename T>

@@ -453,10 +453,12 @@ static FbShape fb_shape(int maxW, FbPick pick = FbPick{0, 0}) {
             for (const auto &sh : kFbShapes)
                 if (sh[0] == nw && sh[1] == mwl) return fb_shape_of(maxW, nw, mwl);
     }
+    // defaults measured with 1000 items in flight: two workgroups fit per CU
+    // (LDS), and the fewer waves per item the better at that occupancy
     if (cap == 4) return fb_shape_of(maxW, 1, 4);
     if (cap == 8) return fb_shape_of(maxW, 2, 4);
     if (cap == 16) return fb_shape_of(maxW, 4, 4);
-    return fb_shape_of(maxW, 8, 4);
+    return fb_shape_of(maxW, 4, 8);
 }
 
 FbPick fb_default_pick(int maxW) {

@@ -12,9 +12,12 @@
 // registers.  Per locus the 8 per-hypothesis sums are reduce-scattered with DPP
 // inside each wave (lane l ends with hypothesis l & 7) and combined across the
 // NW waves through one LDS exchange; TRANS_HAP's 64 sums are reduce-scattered
-// the same way.  More waves per target shorten each wave's share of a locus and
-// let every SIMD interleave several waves; the launch shape comes from the
-// tuning table (phase_shape).  The backward pass stores
+// the same way, in two halves of 32.  The locus update is specialised by mode
+// (RUN / COLLAPSE) and emission kind, so a state costs about three VALU
+// operations per hypothesis.  More waves per target shorten each wave's share
+// of a locus; fewer let more targets share a CU within its register file,
+// which wins when many targets are in flight; the launch shape comes from
+// phase_shape (measured defaults) or the run's tuner.  The backward pass stores
 // segment-start and flat-het messages (K x 8 floats, [state][hyp] with the
 // state index interleaved as in registers) in HBM; the forward pass samples
 // the segment diplotypes (TRANS_HAP + SAMPLE_DIP) and the flat hets, then the
@@ -27,6 +30,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace gh {
@@ -248,11 +252,51 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
         return (s_rows[j * W + (k >> 6)] >> (k & 63)) & 1ull;
     };
 
-    // One locus update (INIT / RUN / COLLAPSE of the three kinds), tile offset j.
-    // Emissions are an 8-vector per allele for the locus; a state's update is
-    // base(h) * e_allele(h) with base = 1 (INIT) or fma(src, nt/S, tFreq(h)).
-    // Multiplying by exactly 1.0 where the reference skips the multiply keeps
-    // every value identical to the reference's per-kind kernels.
+    // One locus update (INIT / RUN / COLLAPSE), tile offset j.  A state's
+    // update is base(h) * e(h) with base = fma(src, NT, tFreq(h)) (src = its own
+    // value for RUN, its hypothesis sum for COLLAPSE; INIT is NT = 0, tFreq = 1,
+    // so base is exactly 1) and e(h) the locus emission.  At a peak het
+    // hypothesis h carries allele 0 when bit (2 - type) of h is set; its
+    // emission is ee if that matches the state's allele, else ed, so a state
+    // needs two selects (eA for hypotheses with the bit set, eB for the rest)
+    // and the bit of each h is a compile-time constant in the specialised
+    // update.  A peak hom's emission does not depend on h; a flat het has none
+    // (a multiply by exactly 1 in the reference).  States past K get emission
+    // 0.  The values are those of the reference's per-kind kernels.
+    auto update = [&](auto TYc, auto RUNc, int j, float NT, const float tF[8], float eh0, float eh1,
+                      float acc[8]) {
+        constexpr int ty = decltype(TYc)::value;    // 0..2: het bit position, 3: peak hom, 4: flat het
+        constexpr bool run = decltype(RUNc)::value;
+#pragma unroll
+        for (int i = 0; i < KPT; i++) {
+            if (PH_NT * i + 64 * wid >= K) {
+#pragma unroll
+                for (int h = 0; h < 8; h++) p[i][h] = 0.0f;
+                continue;
+            }
+            const bool vi = valid[i];
+            const bool ah = bit_of(j, i);
+            float eA = 1.0f, eB = 1.0f;
+            if constexpr (ty <= 2) {
+                eA = vi ? (ah ? ed : ee) : 0.0f;
+                eB = vi ? (ah ? ee : ed) : 0.0f;
+            } else if constexpr (ty == 3) {
+                eA = vi ? (ah ? eh1 : eh0) : 0.0f;
+            } else {
+                eA = vi ? 1.0f : 0.0f;
+            }
+            const float sk = sumK[i];
+#pragma unroll
+            for (int h = 0; h < 8; h++) {
+                float e = eA;
+                if constexpr (ty <= 2) e = ((h >> ty) & 1) ? eA : eB;
+                const float base = fmaf(run ? p[i][h] : sk, NT, tF[h]);
+                const float x = base * e;
+                p[i][h] = x;
+                acc[h] += x;
+            }
+        }
+    };
     auto step = [&](int j, int mode, float yt) {
         const int type = s_type[j];
         const bool alt = s_alt[j];
@@ -267,53 +311,41 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
             sumT = 1.0f;
             return;
         }
-        float e0v[8], e1v[8];
-#pragma unroll
-        for (int h = 0; h < 8; h++) {
-            if (type >= 0) {
-                const bool hyp0 = (h >> (2 - type)) & 1;   // hypothesis carries allele 0 at this het
-                e0v[h] = hyp0 ? ee : ed;
-                e1v[h] = hyp0 ? ed : ee;
-            } else if (type == -1) {
-                e0v[h] = alt ? mism : 1.0f;
-                e1v[h] = alt ? 1.0f : mism;
-            } else {
-                e0v[h] = 1.0f;
-                e1v[h] = 1.0f;
-            }
-        }
+        const float eh0 = alt ? mism : 1.0f, eh1 = alt ? 1.0f : mism;   // peak hom emissions
         const float nt = 1.0f - yt;
         float tF[8];
         // one reciprocal of the running total serves both scale factors
         const float rS = __builtin_amdgcn_rcpf(sumT);
         const float NT = (mode != 0) ? nt * rS : 0.0f;
-        const float sc = (mode != 0) ? yt * rS * invK : 0.0f;
+        const float sc = yt * rS * invK;
 #pragma unroll
-        for (int h = 0; h < 8; h++) tF[h] = sumH[h] * sc;
-        const bool run = mode == 1, init = mode == 0;
+        for (int h = 0; h < 8; h++) tF[h] = (mode != 0) ? sumH[h] * sc : 1.0f;
         float acc[8];
 #pragma unroll
         for (int h = 0; h < 8; h++) acc[h] = 0.0f;
-        // states tid + 256*i: a wave covers 64 consecutive states, so a wave whose
-        // first state is >= K skips the block entirely (uniform branch)
-#pragma unroll
-        for (int i = 0; i < KPT; i++) {
-            if (PH_NT * i + 64 * wid >= K) {
-#pragma unroll
-                for (int h = 0; h < 8; h++) p[i][h] = 0.0f;
-                continue;
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        using I4 = std::integral_constant<int, 4>;
+        using RN = std::integral_constant<bool, true>;
+        using CL = std::integral_constant<bool, false>;
+        const int ty = type >= 0 ? 2 - type : (type == -1 ? 3 : 4);
+        if (mode == 1) {
+            switch (ty) {
+                case 0: update(I0{}, RN{}, j, NT, tF, eh0, eh1, acc); break;
+                case 1: update(I1{}, RN{}, j, NT, tF, eh0, eh1, acc); break;
+                case 2: update(I2{}, RN{}, j, NT, tF, eh0, eh1, acc); break;
+                case 3: update(I3{}, RN{}, j, NT, tF, eh0, eh1, acc); break;
+                default: update(I4{}, RN{}, j, NT, tF, eh0, eh1, acc); break;
             }
-            const bool ah = valid[i] && bit_of(j, i);
-            const float vm = valid[i] ? 1.0f : 0.0f;
-            const float sk = sumK[i];
-#pragma unroll
-            for (int h = 0; h < 8; h++) {
-                const float e = (ah ? e1v[h] : e0v[h]) * vm;
-                const float src = run ? p[i][h] : sk;
-                const float base = init ? 1.0f : fmaf(src, NT, tF[h]);
-                const float x = base * e;
-                p[i][h] = x;
-                acc[h] += x;
+        } else {
+            switch (ty) {
+                case 0: update(I0{}, CL{}, j, NT, tF, eh0, eh1, acc); break;
+                case 1: update(I1{}, CL{}, j, NT, tF, eh0, eh1, acc); break;
+                case 2: update(I2{}, CL{}, j, NT, tF, eh0, eh1, acc); break;
+                case 3: update(I3{}, CL{}, j, NT, tF, eh0, eh1, acc); break;
+                default: update(I4{}, CL{}, j, NT, tF, eh0, eh1, acc); break;
             }
         }
         bsum8(acc);
@@ -380,31 +412,11 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     __syncthreads();
     int cur_dip = dip_sh;
     // ---------------- forward (phasing_hmm.cpp:142-191)
-    // the stored backward message of the next segment is fetched into registers
-    // when a segment starts, so TRANS_HAP at the segment's end does not wait on HBM
-    float nxtv[KPT][8];
-    auto fetch_next = [&](int seg) {
-        if (seg + 1 >= it.n_segs) return;
-        const float *nxt = segp + (size_t)(seg + 1) * KH;
-#pragma unroll
-        for (int i = 0; i < KPT; i++) {
-            const float4 *n4 = (const float4 *)(nxt + ((size_t)i * PH_NT + tid) * 8);
-            float4 a = make_float4(0, 0, 0, 0), b = a;
-            if (valid[i]) { a = n4[0]; b = n4[1]; }
-            nxtv[i][0] = a.x; nxtv[i][1] = a.y; nxtv[i][2] = a.z; nxtv[i][3] = a.w;
-            nxtv[i][4] = b.x; nxtv[i][5] = b.y; nxtv[i][6] = b.z; nxtv[i][7] = b.w;
-        }
-    };
-#pragma unroll
-    for (int i = 0; i < KPT; i++)
-#pragma unroll
-        for (int h = 0; h < 8; h++) nxtv[i][h] = 0.0f;
     for (int q0 = 0; q0 < nloc; q0 += PH_TILE) {
         const int n = min(PH_TILE, nloc - q0);
         load_tile(q0, n);
         for (int j = 0; j < n; j++) {
             const int q = q0 + j;
-            if (s_first[j]) fetch_next(s_seg[j]);
             step(j, q == 0 ? 0 : (s_first[j] ? 2 : 1), s_yin[j]);
             const bool seg_last = s_last[j];
             if (seg_last) sumk();
@@ -413,21 +425,41 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                 const float yt2 = s_yout[j];
                 const float nt2 = 1.0f - yt2;
                 const float F2 = nt2 / sumT;
-                float part[64];
-#pragma unroll
-                for (int e = 0; e < 64; e++) part[e] = 0.0f;
+                // the next segment's stored backward message (segment-start loci
+                // are rare, so it is loaded here rather than held in registers)
+                const float *nxt = segp + (size_t)(s_seg[j] + 1) * KH;
+                float nv[KPT][8];
 #pragma unroll
                 for (int i = 0; i < KPT; i++) {
-                    const float *nv = nxtv[i];
-#pragma unroll
-                    for (int h1 = 0; h1 < 8; h1++) {
-                        const float F1 = (sumH[h1] / sumT) * yt2 / Kf;
-                        const float pv = fmaf(p[i][h1], F2, F1);
-#pragma unroll
-                        for (int h2 = 0; h2 < 8; h2++) part[h1 * 8 + h2] += pv * nv[h2];
+                    float4 a = make_float4(0, 0, 0, 0), b = a;
+                    if (valid[i]) {
+                        const float4 *n4 = (const float4 *)(nxt + ((size_t)i * PH_NT + tid) * 8);
+                        a = n4[0];
+                        b = n4[1];
                     }
+                    nv[i][0] = a.x; nv[i][1] = a.y; nv[i][2] = a.z; nv[i][3] = a.w;
+                    nv[i][4] = b.x; nv[i][5] = b.y; nv[i][6] = b.z; nv[i][7] = b.w;
                 }
-                red64[wid][rs64_index(lane)] = wave_rs64(part);
+                // the 64 sums (h1, h2) in two halves of 32 (h1 < 4, h1 >= 4)
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++) {
+                    float part[32];
+#pragma unroll
+                    for (int e = 0; e < 32; e++) part[e] = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < KPT; i++) {
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int h1 = 4 * hh + u;
+                            const float F1 = (sumH[h1] / sumT) * yt2 / Kf;
+                            const float pv = fmaf(p[i][h1], F2, F1);
+#pragma unroll
+                            for (int h2 = 0; h2 < 8; h2++) part[u * 8 + h2] += pv * nv[i][h2];
+                        }
+                    }
+                    const float r = wave_rs32(part);
+                    if (lane < 32) red64[wid][32 * hh + rs32_index(lane)] = r;
+                }
                 __syncthreads();
                 if (tid < 64) {
                     float t = 0.0f;
@@ -560,8 +592,8 @@ static void phase_go(gh_panel *P, int n, const PhItem *d_items, const CondItem *
 // by state-count class; the tuning table picks one per class.
 static const PhaseShape kPhaseShapes[] = {
     {1, 1}, {1, 2}, {1, 4}, {2, 1}, {2, 2}, {4, 1},                 // K <= 256
-    {2, 4}, {4, 2}, {8, 1},                                         // K <= 512
-    {4, 4}, {8, 2}, {16, 1},                                        // K <= 1024
+    {1, 8}, {2, 4}, {4, 2}, {8, 1},                                 // K <= 512
+    {2, 8}, {4, 4}, {8, 2}, {16, 1},                                // K <= 1024
     {4, 8}, {8, 4}, {16, 2},                                        // K <= 2048
 };
 
@@ -583,8 +615,11 @@ PhaseShape phase_shape(int maxK) {
             for (const PhaseShape &sh : phase_shapes(maxK))
                 if (sh.NW == nw && sh.KPT == kpt) return sh;
     }
+    // Defaults measured with 1000 targets in flight (five times the CUs): the
+    // fewest waves per target whose register budget still lets two or more
+    // targets share a CU (4 states per thread below 2048 states, 8 at 2048).
     const int cls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
-    static const PhaseShape dflt[4] = {{4, 1}, {4, 2}, {8, 2}, {8, 4}};
+    static const PhaseShape dflt[4] = {{1, 4}, {2, 4}, {4, 4}, {4, 8}};
     return dflt[cls];
 }
 
@@ -605,7 +640,7 @@ void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const Con
         return;                                                                                                 \
     }
     GH_PH(1, 1) GH_PH(1, 2) GH_PH(1, 4) GH_PH(2, 1) GH_PH(2, 2) GH_PH(4, 1) GH_PH(2, 4) GH_PH(4, 2) GH_PH(8, 1)
-    GH_PH(4, 4) GH_PH(8, 2) GH_PH(16, 1) GH_PH(4, 8) GH_PH(8, 4) GH_PH(16, 2)
+    GH_PH(1, 8) GH_PH(2, 8) GH_PH(4, 4) GH_PH(8, 2) GH_PH(16, 1) GH_PH(4, 8) GH_PH(8, 4) GH_PH(16, 2)
 #undef GH_PH
     throw Error(GH_ERR_ARG, "phasing: no kernel for the requested launch shape");
 }

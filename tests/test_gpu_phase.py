@@ -44,8 +44,9 @@ def test_rephase_matches_oracle(panels, K):
 
 
 # every launch shape the phasing kernel is built for, at a set size of its class
-_SHAPES = [(1, 1, 60), (1, 2, 100), (1, 4, 250), (2, 1, 100), (2, 2, 250), (4, 1, 256), (2, 4, 500), (4, 2, 400),
-           (8, 1, 512), (4, 4, 1000), (8, 2, 700), (16, 1, 1024), (4, 8, 2000), (8, 4, 1500), (16, 2, 1999)]
+_SHAPES = [(1, 1, 60), (1, 2, 100), (1, 4, 250), (2, 1, 100), (2, 2, 250), (4, 1, 256), (1, 8, 450), (2, 4, 500),
+           (4, 2, 400), (8, 1, 512), (2, 8, 900), (4, 4, 1000), (8, 2, 700), (16, 1, 1024), (4, 8, 2000), (8, 4, 1500),
+           (16, 2, 1999)]
 
 
 @pytest.mark.parametrize("nw,kpt,K", _SHAPES)
