@@ -314,6 +314,7 @@ def main():
                                phase="%dx%d" % (o["stats"]["ph_nw"], o["stats"]["ph_kpt"])) for o in M["outs"]],
         "setup_s": setup_s,
         "hbm_used_gb": round(hbm_used_gb, 1),
+        "chunk_wall_ms": [round(o["stats"]["ms_total"], 1) for o in M["outs"]],
     }
     if "learned" in res:
         L_ = res["learned"]

@@ -116,6 +116,43 @@ __device__ __forceinline__ float wave_rs32(const float v[32]) {
     return x + partner32(x);
 }
 
+// A wave-uniform copy of a value every lane holds (a work-item descriptor, an
+// LDS broadcast): each 32-bit word goes through v_readfirstlane, so the
+// compiler keeps it in SGPRs and branches on it with scalar branches.
+template <typename T>
+__device__ __forceinline__ T wave_uniform(const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "wave_uniform: size must be a multiple of 4 bytes");
+    T r;
+    const int *src = reinterpret_cast<const int *>(&v);
+    int *dst = reinterpret_cast<int *>(&r);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+    return r;
+}
+
+// A 64-bit value that is the same in every lane, moved to SGPRs.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+// Bit (lane) of a wave-uniform 64-bit mask: the mask itself becomes the lane
+// mask of the select that consumes it (no per-lane shift or compare).
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// A copy of v held in a VGPR (v_mov_b32).  Wave-uniform operands of
+// per-lane selects otherwise get copied from SGPRs at every use.
+__device__ __forceinline__ float in_vgpr(float v) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
+    return r;
+}
+__device__ __forceinline__ uint32_t in_vgpr_u32(uint32_t v) {
+    uint32_t r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
+    return r;
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 template <typename T>

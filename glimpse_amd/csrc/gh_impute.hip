@@ -36,6 +36,7 @@ namespace gh {
 
 constexpr int FB_TILE = 32;
 constexpr int FB_LDS_BYTES = 156 * 1024;
+constexpr int FB_WG_PER_CU = 3;
 
 struct FbArgs {
     const FbItem *fb;
@@ -63,8 +64,8 @@ __device__ __forceinline__ int rli(int v, int j) { return __builtin_amdgcn_readl
 template <int NW, int MWL>
 __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const FbItem f = A.fb[blockIdx.x];
-    const CondItem c = A.cond[f.cond];
+    const FbItem f = wave_uniform(A.fb[blockIdx.x]);
+    const CondItem c = wave_uniform(A.cond[f.cond]);
     const int K = c.K, W = c.W, Lp = c.Lp, C = A.C;
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (Lp <= 0) return;
@@ -86,14 +87,21 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     // this item and are skipped (a wave-uniform bound: items with fewer states
     // than the launch capacity do proportionally less work)
     const int nI = min(MWL, max(0, (W - wv + NW - 1) / NW));
+    // Slots run from the wave's highest word down: slot i holds word
+    // (nI-1-i)*NW + wv.  When K is not a multiple of 64 the item's last word
+    // W-1 is partial, and it is slot 0 of wave (W-1) % NW.  Slot 0 therefore
+    // always uses emissions that are 0 on lanes past K (pv; equal to the
+    // site's emissions elsewhere), which keeps those lanes' alpha and beta
+    // exactly 0, so no other step masks lanes.
+    const bool partW = (K & 63) != 0 && ((W - 1) % NW) == wv;
+    const bool pv = !partW || (64 * (W - 1) + lane < K);
     int wid[MWL];
-    bool wok[MWL], valid[MWL];
 #pragma unroll
-    for (int i = 0; i < MWL; i++) {
-        wok[i] = i < nI;
-        wid[i] = wok[i] ? i * NW + wv : 0;   // clamped so row reads stay inside the tile
-        valid[i] = wok[i] && (64 * wid[i] + lane < K);
-    }
+    for (int i = 0; i < MWL; i++) wid[i] = i < nI ? (nI - 1 - i) * NW + wv : 0;   // clamped: reads stay in the tile
+    auto vlane = [&](int i) { return i < nI && 64 * wid[i] + lane < K; };
+    // the allele bits of word slot i at tile site j: the row word is the same in
+    // every lane, so it is read into SGPRs and used as the select's lane mask
+    auto rbit = [&](int j, int i) { return lane_bit(uniform_u64(rows[j * W + wid[i]])); };
 
     // cross-wave sums (fixed order, so the forward and the recomputation agree bit for bit).
     // The LDS slot alternates with every reduction, so one barrier per reduction suffices.
@@ -201,30 +209,33 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
 #pragma unroll
     for (int i = 0; i < MWL; i++) a[i] = 0.0f;
     // forward step at site l (tile offset j): alpha in a[], scaling sum in sp
+    // (flat and LQ sites have emission 1 for both alleles)
     auto fwd_site = [&](int j, int l, float &sp) {
         const int code = rli(m_code, j);
         const bool fl = code & 1;
-        const float e0 = fl ? 1.0f : rlf(m_e0, j), e1 = fl ? 1.0f : rlf(m_e1, j);
+        const float e0s = fl ? 1.0f : rlf(m_e0, j), e1s = fl ? 1.0f : rlf(m_e1, j);
+        const float e0 = in_vgpr(e0s), e1 = in_vgpr(e1s);
+        const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;   // the partial word's emissions
         float s = 0.0f;
         if (l == 0) {
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
-                if (i >= nI) break;
-                const bool b = (rows[j * W + wid[i]] >> lane) & 1ull;
-                const float x = fl ? invK : (b ? e1 : e0) * invK;
-                a[i] = valid[i] ? x : 0.0f;
-                s += a[i];
+                if (i >= nI) continue;
+                const bool b = rbit(j, i);
+                const float x = (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p)) * invK;
+                a[i] = x;
+                s += x;
             }
             sp = fl ? 1.0f : xsum1(s);
         } else {
             const float f1 = rlf(m_f1in, j), f2 = rlf(m_ntin, j) * __builtin_amdgcn_rcpf(sp);
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
-                if (i >= nI) break;
-                const bool b = (rows[j * W + wid[i]] >> lane) & 1ull;
-                const float x = fmaf(a[i], f2, f1) * (b ? e1 : e0);
-                a[i] = valid[i] ? x : 0.0f;
-                s += a[i];
+                if (i >= nI) continue;
+                const bool b = rbit(j, i);
+                const float x = fmaf(a[i], f2, f1) * (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p));
+                a[i] = x;
+                s += x;
             }
             sp = xsum1(s);
         }
@@ -247,7 +258,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                     float *dst = ck + (size_t)(l / C) * vstride + lane;
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
-                        if (i >= nI) break;
+                        if (i >= nI) continue;
                         dst[64 * wid[i]] = a[i];
                     }
                 }
@@ -263,7 +274,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     for (int i = 0; i < MWL; i++) {
         wlo[i] = 0;
         whi[i] = 0x7FFFFFFF;
-        if (A.win_lo && valid[i]) {
+        if (A.win_lo && vlane(i)) {
             wlo[i] = A.win_lo[c.state_off + 64 * wid[i] + lane];
             whi[i] = A.win_hi[c.state_off + 64 * wid[i] + lane];
         }
@@ -280,7 +291,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
         const float *srcv = ck + (size_t)(bs / C) * vstride + lane;
 #pragma unroll
         for (int i = 0; i < MWL; i++) {
-            if (i >= nI) break;
+            if (i >= nI) continue;
             nck[i] = srcv[64 * wid[i]];
         }
     };
@@ -303,21 +314,21 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                 float s = 0.0f;
 #pragma unroll
                 for (int i = 0; i < MWL; i++) {
-                    if (i >= nI) break;
+                    if (i >= nI) continue;
                     s += a[i];
                 }
                 sp = (bs == 0 && fl0) ? 1.0f : xsum1(s);
             }
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
-                if (i >= nI) break;
+                if (i >= nI) continue;
                 ablk[((size_t)0 * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
             }
             for (int l = bs + 1; l < be; l++) {
                 fwd_site(l - l0, l, sp);
 #pragma unroll
                 for (int i = 0; i < MWL; i++) {
-                    if (i >= nI) break;
+                    if (i >= nI) continue;
                     ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
                 }
             }
@@ -327,17 +338,19 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                 const int j = l - l0;
                 const int code = rli(m_code, j);
                 const bool fl = code & 1, isflat = code & 2;
-                const float e0 = fl ? 1.0f : rlf(m_e0, j), e1 = fl ? 1.0f : rlf(m_e1, j);
+                const float e0s = fl ? 1.0f : rlf(m_e0, j), e1s = fl ? 1.0f : rlf(m_e1, j);
+                const float e0 = in_vgpr(e0s), e1 = in_vgpr(e1s);
+                const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;
                 float p0 = 0.0f, p1 = 0.0f, bs_ = 0.0f;
                 float dpv[MWL];
                 if (l == Lp - 1) {
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
-                        if (i >= nI) break;
-                        const bool bit = (rows[j * W + wid[i]] >> lane) & 1ull;
+                        if (i >= nI) continue;
+                        const bool bit = rbit(j, i);
                         const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
-                        const float bv = fl ? invK : (bit ? e1 : e0) * invK;
-                        b[i] = valid[i] ? bv : 0.0f;
+                        const float bv = (i > 0 ? (bit ? e1 : e0) : (bit ? e1p : e0p)) * invK;
+                        b[i] = bv;
                         dpv[i] = av;
                         p1 += bit ? av : 0.0f;
                         p0 += bit ? 0.0f : av;
@@ -347,15 +360,15 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                     const float f1 = rlf(m_f1out, j), f2 = rlf(m_ntout, j) * __builtin_amdgcn_rcpf(bsum_next);
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
-                        if (i >= nI) break;
-                        const bool bit = (rows[j * W + wid[i]] >> lane) & 1ull;
+                        if (i >= nI) continue;
+                        const bool bit = rbit(j, i);
                         const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
-                        const float bt = valid[i] ? fmaf(b[i], f2, f1) : 0.0f;
+                        const float bt = fmaf(b[i], f2, f1);   // lanes past K: b = 0 and alpha = 0
                         const float dp = av * bt;
                         dpv[i] = dp;
                         p1 += bit ? dp : 0.0f;
                         p0 += bit ? 0.0f : dp;
-                        b[i] = bt * (bit ? e1 : e0);
+                        b[i] = bt * (i > 0 ? (bit ? e1 : e0) : (bit ? e1p : e0p));
                         bs_ += b[i];
                     }
                 }
@@ -365,13 +378,13 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                     const float inv = 1.0f / (p0 + p1);
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
-                        if (i >= nI) break;
+                        if (i >= nI) continue;
                         const int site = rli(m_site, j);
                         const bool in = site >= wlo[i] && site < whi[i];
                         ms[i] = in ? fmaf(dpv[i], inv, ms[i]) : ms[i];
                     }
                 }
-                if (!fl) { p0 /= e0; p1 /= e1; }
+                if (!fl) { p0 /= e0s; p1 /= e1s; }
                 float o0 = p0 * ee + p1 * ed, o1 = p0 * ed + p1 * ee;
                 if (!isflat) { o0 *= rlf(m_h0, j); o1 *= rlf(m_h1, j); }
                 if (tid == 0) {
@@ -387,7 +400,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
         float *mo = A.mass_all + (size_t)c.state_off;
 #pragma unroll
         for (int i = 0; i < MWL; i++)
-            if (valid[i]) mo[64 * wid[i] + lane] = ms[i];
+            if (vlane(i)) mo[64 * wid[i] + lane] = ms[i];
     }
 }
 
@@ -432,11 +445,11 @@ static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     const size_t rows = (size_t)FB_TILE * maxW * 8;
     const size_t per_site = (size_t)s.NW * s.MWL * 64 * 4;
     const size_t red = 2 * 16 * 4 * 4;   // [2][NW <= 16][4] floats
-    // The longest block (C divides FB_TILE) that leaves room for two workgroups
-    // per CU (the register budget of the larger shapes allows two), unless that
-    // would make blocks shorter than 8 sites; else the longest that fits.
+    // The longest block (C divides FB_TILE) that leaves room for FB_WG_PER_CU
+    // workgroups per CU (what the register budget of the 4-wave shapes allows),
+    // unless that would make blocks shorter than 4 sites; else the longest that fits.
     int C = FB_TILE;
-    while (C > 8 && rows + C * per_site + red > (size_t)FB_LDS_BYTES / 2) C >>= 1;
+    while (C > 4 && rows + C * per_site + red > (size_t)FB_LDS_BYTES / FB_WG_PER_CU) C >>= 1;
     while (C > 1 && rows + C * per_site + red > (size_t)FB_LDS_BYTES) C >>= 1;
     s.C = C;
     s.smem = rows + C * per_site + red;

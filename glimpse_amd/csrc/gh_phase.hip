@@ -172,9 +172,9 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     __shared__ signed char s_type[PH_TILE];
     __shared__ unsigned char s_alt[PH_TILE], s_first[PH_TILE], s_last[PH_TILE];
     __shared__ float s_yin[PH_TILE], s_yout[PH_TILE];
-    const PhItem it = items[blockIdx.x];
-    const CondItem c = cond[it.cond];
-    const int K = c.K, W = c.W, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const PhItem it = wave_uniform(items[blockIdx.x]);
+    const CondItem c = wave_uniform(cond[it.cond]);
+    const int K = c.K, W = c.W, tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint64_t *hv = hvar_all + c.hvar_off;
     const Locus *loci = loci_all + it.locus_off;
     const int nloc = it.n_loci;
@@ -247,6 +247,9 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
         }
         __syncthreads();
     };
+    // per-locus metadata is the same for every lane: read it into SGPRs
+    auto ui = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+    auto uf = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
     auto bit_of = [&](int j, int i) -> bool {
         const int k = kk[i];
         return (s_rows[j * W + (k >> 6)] >> (k & 63)) & 1ull;
@@ -298,8 +301,8 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
         }
     };
     auto step = [&](int j, int mode, float yt) {
-        const int type = s_type[j];
-        const bool alt = s_alt[j];
+        const int type = __builtin_amdgcn_readfirstlane(s_type[j]);
+        const bool alt = __builtin_amdgcn_readfirstlane(s_alt[j]);
         if (type == -2 && mode == 0) {
             const float v = 1.0f / (8 * (unsigned)K);
 #pragma unroll
@@ -379,13 +382,14 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
             for (int j = n - 1; j >= 0; j--) {
                 const int q = q0 + j;
                 const bool last = q == nloc - 1;
-                step(j, last ? 0 : (s_last[j] ? 2 : 1), s_yout[j]);
-                if (s_first[j]) {
+                step(j, last ? 0 : (ui(s_last[j]) ? 2 : 1), uf(s_yout[j]));
+                if (ui(s_first[j])) {
                     sumk();
-                    const int sg = s_seg[j];
+                    const int sg = ui(s_seg[j]);
                     store_msg(segp + (size_t)sg * KH, segs + sg * 9, segs + sg * 9 + 8);
                 }
-                if (s_type[j] == -2) store_msg(impp + (size_t)s_miss[j] * KH, imps + s_miss[j] * 8, nullptr);
+                if (ui(s_type[j]) == -2)
+                    store_msg(impp + (size_t)ui(s_miss[j]) * KH, imps + ui(s_miss[j]) * 8, nullptr);
             }
         }
     }
@@ -417,17 +421,17 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
         load_tile(q0, n);
         for (int j = 0; j < n; j++) {
             const int q = q0 + j;
-            step(j, q == 0 ? 0 : (s_first[j] ? 2 : 1), s_yin[j]);
-            const bool seg_last = s_last[j];
+            step(j, q == 0 ? 0 : (ui(s_first[j]) ? 2 : 1), uf(s_yin[j]));
+            const bool seg_last = ui(s_last[j]);
             if (seg_last) sumk();
             if (seg_last && q != nloc - 1) {
                 // TRANS_HAP (phasing_hmm.h:351-373)
-                const float yt2 = s_yout[j];
+                const float yt2 = uf(s_yout[j]);
                 const float nt2 = 1.0f - yt2;
                 const float F2 = nt2 / sumT;
                 // the next segment's stored backward message (segment-start loci
                 // are rare, so it is loaded here rather than held in registers)
-                const float *nxt = segp + (size_t)(s_seg[j] + 1) * KH;
+                const float *nxt = segp + (size_t)(ui(s_seg[j]) + 1) * KH;
                 float nv[KPT][8];
 #pragma unroll
                 for (int i = 0; i < KPT; i++) {
@@ -482,22 +486,22 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                         atomicExch(status, 1);
                     } else {
                         const float u =
-                            u01(philox_draw(seed, iter, P_DIP_SEG, it.tgt, (uint32_t)(s_seg[j] + 1))) * S;
+                            u01(philox_draw(seed, iter, P_DIP_SEG, it.tgt, (uint32_t)(ui(s_seg[j]) + 1))) * S;
                         float cs = D[0];
                         for (int i = 0; i < 7; i++) {
                             if (u <= cs) { r = i; break; }
                             cs += D[i + 1];
                         }
                     }
-                    dip[s_seg[j] + 1] = r;
+                    dip[ui(s_seg[j]) + 1] = r;
                     dip_sh = r;
                 }
                 __syncthreads();
                 cur_dip = dip_sh;
             }
-            if (s_type[j] == -2) {
+            if (ui(s_type[j]) == -2) {
                 // IMPUTE_FLAT_HET (phasing_hmm.h:389-423)
-                const int mi = s_miss[j];
+                const int mi = ui(s_miss[j]);
                 const float *ip = impp + (size_t)mi * KH;
                 float sR[8], sL[8], s0[8], s1[8];
 #pragma unroll
