@@ -250,10 +250,10 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     // per-locus metadata is the same for every lane: read it into SGPRs
     auto ui = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
     auto uf = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
-    auto bit_of = [&](int j, int i) -> bool {
-        const int k = kk[i];
-        return (s_rows[j * W + (k >> 6)] >> (k & 63)) & 1ull;
-    };
+    // allele of state kk[i] at tile locus j: its Hvar word (wid + NW*i) is the
+    // same for the whole wave and serves as the select's lane mask (only called
+    // for slots that hold states, so the word is inside the row)
+    auto bit_of = [&](int j, int i) -> bool { return lane_bit(uniform_u64(s_rows[j * W + wid + NW * i])); };
 
     // One locus update (INIT / RUN / COLLAPSE), tile offset j.  A state's
     // update is base(h) * e(h) with base = fma(src, NT, tFreq(h)) (src = its own
@@ -511,20 +511,18 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                     s0[h] = 0.0f;
                     s1[h] = 0.0f;
                 }
+                // slots past K are skipped; lanes past K hold 0 in both the
+                // forward values and the stored message, so they add exact zeros
 #pragma unroll
                 for (int i = 0; i < KPT; i++) {
-                    const bool ah = valid[i] && bit_of(j, i);
-                    float4 a = make_float4(0, 0, 0, 0), b = a;
-                    if (PH_NT * i + 64 * wid < Kr) {
-                        const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
-                        a = i4[0];
-                        b = i4[1];
-                    }
+                    if (PH_NT * i + 64 * wid >= K) continue;
+                    const bool ah = bit_of(j, i);
+                    const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
+                    const float4 a = i4[0], b = i4[1];
                     const float iv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
                     for (int h = 0; h < 8; h++) {
-                        float v = (iv[h] * sR[h]) * (p[i][h] * sL[h]);
-                        v = valid[i] ? v : 0.0f;
+                        const float v = (iv[h] * sR[h]) * (p[i][h] * sL[h]);
                         s1[h] += ah ? v : 0.0f;
                         s0[h] += ah ? 0.0f : v;
                     }
