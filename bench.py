@@ -189,11 +189,15 @@ def timed(runs, a, label):
     return dt, [r.fetch() for r in runs]
 
 
+# workload tag of the committed rocprofv3 summaries (200 targets x 5 chunks, full size)
+PROFILE_WORKLOAD = "batch200x5"
+
+
 def profiled_traffic(path):
     """HBM bytes per forward-backward launch from the committed rocprofv3 counter
     summary of this workload and path (FETCH_SIZE and WRITE_SIZE runs, scaled by
     a 1 GiB copy's counter readings), or None when there is none."""
-    f = os.path.join(ROOT, "profiles", "fb_kernel_c2_%s_traffic.md" % path)
+    f = os.path.join(ROOT, "profiles", "fb_kernel_%s_%s_traffic.md" % (PROFILE_WORKLOAD, path))
     if not os.path.exists(f):
         return None
     for line in open(f):
