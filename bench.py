@@ -42,11 +42,12 @@ import sys
 import threading
 import time
 
-# One hardware queue per concurrently running chunk stream (HIP's default is 4;
-# with 5 chunks two of them would share a queue and run one after the other).
+# One hardware queue per concurrently running stream (HIP's default is 4; with
+# 5 chunks, each with a second stream for its smaller set-size class, streams
+# would share queues and run one after the other).
 # Must be set before the HIP runtime starts.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -233,9 +234,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serial", action="store_true", help="run chunks one after another")
     ap.add_argument("--path", choices=("both", "plain", "learned"), default="both")
-    ap.add_argument("--k-frac", type=float, default=0.2, help="learned path: kept fraction of the PBWT pool")
-    ap.add_argument("--min-mass", type=float, default=0.0,
-                    help="learned path: keep the whole pool below this predicted mass share")
+    ap.add_argument("--k-frac", type=float, default=0.65,
+                    help="learned path: kept fraction of the PBWT pool (0.65: the smallest tried whose dosage r2 is "
+                         "within 0.001 of the plain path's on this workload)")
+    ap.add_argument("--k-max", type=int, default=1024,
+                    help="learned path: at most this many kept states per target (0: no cap); 1024 keeps the "
+                         "learned sets in the 1024-state launch class")
+    ap.add_argument("--min-mass", type=float, default=0.95,
+                    help="learned path: a target whose kept states hold less than this share of the predicted mass "
+                         "keeps its whole PBWT pool (the plain selection) for that iteration")
+    ap.add_argument("--latency-runs", type=int, default=0,
+                    help="the N chunks with the most work (sites x states) use latency launch shapes (more waves "
+                         "per target); with every chunk in flight the throughput defaults were faster for all, so 0")
     ap.add_argument("--tune", action="store_true",
                     help="autotune the launch shapes during the warm-up steps (each chunk's tuner times its own "
                          "launches; with all chunks in flight the measured defaults are faster)")
@@ -267,7 +277,11 @@ def main():
     runs = [hip.Run(p, c["gl"], c["flat"], prm, kpbwt_per_target=c["K"]) for p, c in zip(panels, chunks)]
     if a.tune:
         for r in runs:
-            r.set_tuning(True)   # the warm-up executions time each launch shape and keep the fastest
+            r.set_tuning(1)   # the warm-up executions time each launch shape and keep the fastest
+    else:
+        work = [c["panel"]["L"] * int(np.sum(c["K"])) for c in chunks]
+        for ci in sorted(range(len(chunks)), key=lambda i: -work[i])[:max(0, a.latency_runs)]:
+            runs[ci].set_tuning("latency")
     setup_s = time.perf_counter() - t_setup
     print("[bench] setup %.1f s (%s)" % (setup_s, desc), file=sys.stderr, flush=True)
     tsites = sum(len(c["targets"]) * c["panel"]["L"] for c in chunks)
@@ -283,7 +297,10 @@ def main():
         net, train = train_selector()
         print("[bench] selector trained: %s" % json.dumps(train), file=sys.stderr, flush=True)
         for r, c in zip(runs, chunks):
-            r.set_selector(net, np.maximum(1, np.round(c["K"] * a.k_frac)).astype(np.int32), a.min_mass)
+            kl = np.maximum(1, np.round(c["K"] * a.k_frac))
+            if a.k_max > 0:
+                kl = np.minimum(kl, a.k_max)
+            r.set_selector(net, kl.astype(np.int32), a.min_mass)
         dt, outs = timed(runs, a, "learned")
         res["learned"] = dict(value=tsites / dt, ms_per_step=dt * 1e3, outs=outs)
     for k, v in res.items():
@@ -325,7 +342,7 @@ def main():
         st = [o["stats"] for o in L_["outs"]]
         line["learned"] = dict(value=L_["value"], ms_per_step=L_["ms_per_step"], accuracy=L_["accuracy"],
                                roofline=fb_roofline(L_["outs"], None if a.quick else "learned"),
-                               k_frac=a.k_frac, min_mass=a.min_mass, training=train,
+                               k_frac=a.k_frac, k_max=a.k_max, min_mass=a.min_mass, training=train,
                                target_iterations_learned=sum(s["learned_sel"] for s in st),
                                target_iterations_pool_kept=sum(s["pool_kept"] for s in st),
                                mean_states=float(np.mean(np.concatenate(

@@ -25,7 +25,7 @@ void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h
                int32_t *nstates_h, gh_run_stats *stats);
 void run_destroy(gh_run *G);
 void run_collect(gh_run *G, int from_iter);
-void run_set_tuning(gh_run *G, int on);
+void run_set_tuning(gh_run *G, int mode);
 void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w);
 void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss);
 void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode);
@@ -426,8 +426,8 @@ int gh_mlp_train_run(gh_mlp *m, gh_run *r, int32_t epochs, int32_t batch, float 
                      float *loss_per_epoch) {
     return guard([&] { run_train(m, r, epochs, batch, lr, seed, loss_per_epoch); });
 }
-int gh_run_set_tuning(gh_run *r, int32_t on) {
-    return guard([&] { run_set_tuning(r, on); });
+int gh_run_set_tuning(gh_run *r, int32_t mode) {
+    return guard([&] { run_set_tuning(r, mode); });
 }
 int gh_run_collect(gh_run *r, int32_t from_iter) {
     return guard([&] { run_collect(r, from_iter); });

@@ -254,6 +254,9 @@ struct ShapeTuner {
     }
 };
 
+// A launch of at most this many items uses the latency launch shapes.
+constexpr int FEW_ITEMS = 32;
+
 struct EvPair {
     hipEvent_t a = nullptr, b = nullptr;
     void make() { GH_HIP(hipEventCreate(&a)); GH_HIP(hipEventCreate(&b)); }
@@ -288,6 +291,14 @@ struct gh_run {
     bool done = false;
     // learned selection / training-row collection
     bool tune = false;
+    bool latency = false;   // latency default shapes (a run on the critical path)
+    // K-class batching: the items of the largest set-size class run in one
+    // launch on the run's stream, the others in a launch sized for their own
+    // class on a second stream beside it
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    gh::DBuf d_ford, d_pord;
+    std::vector<int32_t> ford, pord;
     gh::ShapeTuner<gh::FbPick> fb_tuner;
     gh::ShapeTuner<gh::PhaseShape> ph_tuner;
     gh_mlp *mlp = nullptr;
@@ -298,6 +309,11 @@ struct gh_run {
     bool masks_ready = false;
     gh::DBuf d_klearn, d_ins, d_fx, d_fy, d_cov, d_cnt, d_tmask, d_mass0, d_mass1, d_cx, d_cy, d_cw, d_isite,
         d_sgrp, d_tcls, d_wlo, d_whi;
+    ~gh_run() {
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (s2) (void)hipStreamDestroy(s2);
+    }
 };
 
 namespace gh {
@@ -654,27 +670,74 @@ void run_execute(gh_run *G) {
         GH_HIP(hipEventRecord(e_sel.b, s));
         GH_HIP(hipEventRecord(e_cond.a, s));
         condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, maxW_it, s);
+        // K-class batching (not while tuning): items of the largest word-count
+        // class first in G->ford, the rest after them
+        int fbA = T, fbB = 0, fbWA = maxW_it, fbWB = 0;
+        if (!G->tune) {
+            GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
+            GH_HIP(hipStreamSynchronize(s));
+            auto capw = [](int w) { return w <= 4 ? 4 : w <= 8 ? 8 : w <= 16 ? 16 : 32; };
+            int top = 0;
+            for (int t = 0; t < T; t++) top = std::max(top, capw(items[t].W));
+            G->ford.clear();
+            fbWA = 1; fbWB = 1;
+            for (int t = 0; t < T; t++)
+                if (capw(items[t].W) == top) { G->ford.push_back(t); fbWA = std::max(fbWA, items[t].W); }
+            fbA = (int)G->ford.size();
+            for (int t = 0; t < T; t++)
+                if (capw(items[t].W) < top) { G->ford.push_back(t); fbWB = std::max(fbWB, items[t].W); }
+            fbB = T - fbA;
+            maxW_it = fbWA;
+            if (fbB > 0) dalloc_copy(G->d_ford, G->ford.data(), (size_t)T, s);
+        }
         const int wcls = maxW_it <= 4 ? 0 : maxW_it <= 8 ? 1 : maxW_it <= 16 ? 2 : 3;
+        // a launch of a few items cannot fill the GPU: its chains are the
+        // critical path, so it takes the latency shapes
+        const bool fewA = fbA <= FEW_ITEMS, fewB = fbB <= FEW_ITEMS;
         const FbPick fbp = (G->tune && stage > 0) ? G->fb_tuner.pick(wcls, fb_shapes(maxW_it), FbPick{0, 0})
+                           : (G->latency || fewA) ? fb_default_pick(maxW_it, true)
                                                   : FbPick{0, 0};
         const bool label_it = collecting && stage > 0 && npt > 0 && it >= G->collect_from;
         float *mass0 = label_it ? G->d_mass0.as<float>() : nullptr, *mass1 = label_it ? G->d_mass1.as<float>() : nullptr;
         const int32_t *wlo = label_it ? G->d_wlo.as<int32_t>() : nullptr, *whi = label_it ? G->d_whi.as<int32_t>() : nullptr;
         GH_HIP(hipEventRecord(e_cond.b, s));
         GH_HIP(hipEventRecord(e_fb.a, s));
+        // one forward-backward pass over every item: the top class on s, the rest
+        // beside it on s2 (sized for its own class), joined before s goes on
+        auto ensure_s2 = [&]() {
+            if (G->s2) return;
+            GH_HIP(hipStreamCreateWithFlags(&G->s2, hipStreamNonBlocking));
+            GH_HIP(hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming));
+            GH_HIP(hipEventCreateWithFlags(&G->ev_join, hipEventDisableTiming));
+        };
+        auto fb_all = [&](float *HPx, float *massx) {
+            if (fbB == 0) {
+                fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
+                          massx, wlo, whi, fbp);
+                return;
+            }
+            ensure_s2();
+            const int32_t *ord = G->d_ford.as<int32_t>();
+            GH_HIP(hipEventRecord(G->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
+            fb_launch(P, fbA, fb_d, items_d, fbWA, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
+                      massx, wlo, whi, fbp, ord);
+            fb_launch(P, fbB, fb_d, items_d, fbWB, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, G->s2,
+                      massx, wlo, whi, (G->latency || fewB) ? fb_default_pick(fbWB, true) : FbPick{0, 0}, ord + fbA);
+            GH_HIP(hipEventRecord(G->ev_join, G->s2));
+            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+        };
         // H0 (caller_algorithm.cpp:58-65)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k0.a, s));
-        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP0, G->ee_i, G->ed_i, s, mass0, wlo,
-                  whi, fbp);
+        fb_all(HP0, mass0);
         GH_HIP(hipEventRecord(e_k0.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
         // H1 (caller_algorithm.cpp:66-71)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, HL);
         GH_HIP(hipEventRecord(e_k1.a, s));
-        fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i, G->ed_i, s, mass1, wlo,
-                  whi, fbp);
+        fb_all(HP1, mass1);
         GH_HIP(hipEventRecord(e_k1.b, s));
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
@@ -699,9 +762,24 @@ void run_execute(gh_run *G) {
         GH_HIP(hipStreamSynchronize(s));
         int maxK = 1;
         for (int t = 0; t < T; t++) maxK = std::max(maxK, items[t].K);
+        // K-class batching of the phasing launch, as for forward-backward
+        int phA = T, phB = 0, phKB = 1;
+        if (!G->tune) {
+            auto capk = [](int k) { return k <= 256 ? 256 : k <= 512 ? 512 : k <= 1024 ? 1024 : 2048; };
+            const int top = capk(maxK);
+            G->pord.clear();
+            for (int t = 0; t < T; t++)
+                if (capk(items[t].K) == top) G->pord.push_back(t);
+            phA = (int)G->pord.size();
+            for (int t = 0; t < T; t++)
+                if (capk(items[t].K) < top) { G->pord.push_back(t); phKB = std::max(phKB, items[t].K); }
+            phB = T - phA;
+            if (phB > 0) dalloc_copy(G->d_pord, G->pord.data(), (size_t)T, s);
+        }
         const int kcls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
-        const PhaseShape phs = (G->tune && stage > 0) ? G->ph_tuner.pick(kcls, phase_shapes(maxK), phase_shape(maxK))
-                                                      : phase_shape(maxK);
+        const PhaseShape phs = (G->tune && stage > 0)
+                                   ? G->ph_tuner.pick(kcls, phase_shapes(maxK), phase_shape(maxK, G->latency))
+                                   : phase_shape(maxK, G->latency || phA <= FEW_ITEMS);
         int64_t so = 0, ss = 0, io = 0, is = 0, dd = 0;
         for (int t = 0; t < T; t++) {
             ph[t].msg = phase_msg_floats(items[t].K);
@@ -726,8 +804,21 @@ void run_execute(gh_run *G) {
         PB.of1 = (float *)G->d_of1.reserve(sizeof(float) * std::max<int64_t>(is, 1));
         PB.dip = (int32_t *)G->d_dip.reserve(sizeof(int32_t) * std::max<int64_t>(dd, 1));
         GH_HIP(hipEventRecord(e_pk.a, s));
-        phase_launch(P, T, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
-                     (uint32_t)it, status, s, phs);
+        if (phB == 0) {
+            phase_launch(P, T, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
+                         (uint32_t)it, status, s, phs);
+        } else {
+            ensure_s2();
+            const int32_t *ord = G->d_pord.as<int32_t>();
+            GH_HIP(hipEventRecord(G->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
+            phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
+                         (uint32_t)it, status, s, phs, ord);
+            phase_launch(P, phB, phKB, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
+                         (uint32_t)it, status, G->s2, phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
+            GH_HIP(hipEventRecord(G->ev_join, G->s2));
+            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+        }
         GH_HIP(hipEventRecord(e_pk.b, s));
         GH_HIP(hipEventRecord(e_ph.b, s));
         if (stage == 2) {
@@ -751,7 +842,7 @@ void run_execute(gh_run *G) {
             G->ph_tuner.record(e_pk.ms());
         }
         {
-            const FbPick fu = fbp.NW ? fbp : fb_default_pick(maxW_it);
+            const FbPick fu = fbp.NW ? fbp : fb_default_pick(maxW_it, G->latency);
             st.fb_nw = fu.NW; st.fb_mwl = fu.MWL; st.ph_nw = phs.NW; st.ph_kpt = phs.KPT;
         }
         if (learn_ev) st.ms_learn += e_learn.ms();
@@ -822,9 +913,11 @@ void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mas
     G->min_mass = min_mass;
 }
 
-void run_set_tuning(gh_run *G, int on) {
+void run_set_tuning(gh_run *G, int mode) {
     GH_CHECK(G, GH_ERR_ARG, "gh_run_set_tuning: null run");
-    G->tune = on != 0;
+    GH_CHECK(mode >= 0 && mode <= 2, GH_ERR_ARG, "gh_run_set_tuning: mode must be 0, 1 or 2");
+    G->tune = mode == 1;
+    G->latency = mode == 2;
 }
 
 void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,

@@ -51,6 +51,7 @@ struct FbArgs {
     float *HP_all;
     float *mass_all;   // optional: per-state posterior mass (summed over sites)
     const int32_t *win_lo, *win_hi;   // optional with mass_all: per-state site window [lo, hi) of the sum
+    const int32_t *order;             // optional: workgroup b runs item order[b]
     float ee, ed;
     int C;             // sites per checkpoint block (divides FB_TILE)
     int maxW;          // LDS row stride bound
@@ -64,7 +65,7 @@ __device__ __forceinline__ int rli(int v, int j) { return __builtin_amdgcn_readl
 template <int NW, int MWL>
 __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const FbItem f = wave_uniform(A.fb[blockIdx.x]);
+    const FbItem f = wave_uniform(A.fb[A.order ? A.order[blockIdx.x] : (int)blockIdx.x]);
     const CondItem c = wave_uniform(A.cond[f.cond]);
     const int K = c.K, W = c.W, Lp = c.Lp, C = A.C;
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -456,7 +457,7 @@ static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     return s;
 }
 static int fb_cap_words(int maxW) { return maxW <= 4 ? 4 : maxW <= 8 ? 8 : maxW <= 16 ? 16 : 32; }
-static FbShape fb_shape(int maxW, FbPick pick = FbPick{0, 0}) {
+static FbShape fb_shape(int maxW, FbPick pick = FbPick{0, 0}, bool latency = false) {
     const int cap = fb_cap_words(maxW);
     if (pick.NW > 0 && pick.NW * pick.MWL == cap) return fb_shape_of(maxW, pick.NW, pick.MWL);
     // GH_FB_SHAPE=<NW>x<MWL> pins a shape wherever it fits (testing, reproducing a tuned run)
@@ -466,16 +467,18 @@ static FbShape fb_shape(int maxW, FbPick pick = FbPick{0, 0}) {
             for (const auto &sh : kFbShapes)
                 if (sh[0] == nw && sh[1] == mwl) return fb_shape_of(maxW, nw, mwl);
     }
-    // defaults measured with 1000 items in flight: two workgroups fit per CU
-    // (LDS), and the fewer waves per item the better at that occupancy
+    // Defaults measured with 1000 items in flight (throughput): the fewest waves
+    // per item that fill a CU.  Latency defaults (a run whose chain is the
+    // critical path, with the GPU to itself at the end): more waves per item,
+    // so each wave's share of a site is shorter.
     if (cap == 4) return fb_shape_of(maxW, 1, 4);
     if (cap == 8) return fb_shape_of(maxW, 2, 4);
     if (cap == 16) return fb_shape_of(maxW, 4, 4);
-    return fb_shape_of(maxW, 4, 8);
+    return latency ? fb_shape_of(maxW, 8, 4) : fb_shape_of(maxW, 4, 8);
 }
 
-FbPick fb_default_pick(int maxW) {
-    const FbShape sh = fb_shape(maxW);
+FbPick fb_default_pick(int maxW, bool latency) {
+    const FbShape sh = fb_shape(maxW, FbPick{0, 0}, latency);
     return FbPick{sh.NW, sh.MWL};
 }
 
@@ -511,14 +514,14 @@ static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass, const int32_t *d_win_lo,
-               const int32_t *d_win_hi, FbPick pick) {
+               const int32_t *d_win_hi, FbPick pick, const int32_t *d_order) {
     if (n <= 0) return;
     GH_CHECK(maxW <= 32, GH_ERR_ARG, "fb: more than 2048 conditioning states per haplotype is not supported");
     const FbShape sh = fb_shape(maxW, pick);
     FbArgs A;
     A.fb = d_fb; A.cond = d_cond; A.poly_all = d_poly; A.t_all = d_t; A.hvar_all = d_hvar; A.HL_all = d_HL;
     A.flat_all = d_flat; A.flags = P->d_flags.as<uint8_t>(); A.ckpt_all = d_ckpt; A.HP_all = d_HP;
-    A.mass_all = d_mass; A.win_lo = d_win_lo; A.win_hi = d_win_hi;
+    A.mass_all = d_mass; A.win_lo = d_win_lo; A.win_hi = d_win_hi; A.order = d_order;
     A.ee = ee; A.ed = ed; A.C = sh.C; A.maxW = maxW;
 #define GH_FB(nw, mwl)                    \
     if (sh.NW == nw && sh.MWL == mwl) {   \

@@ -127,10 +127,13 @@ struct FbPick {
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass = nullptr,
-               const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr, FbPick pick = FbPick{0, 0});
+               const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr, FbPick pick = FbPick{0, 0},
+               const int32_t *d_order = nullptr);
 // Shapes built for sets of up to maxW words (the tuner's candidates).
 std::vector<FbPick> fb_shapes(int maxW);
-FbPick fb_default_pick(int maxW);
+// The default shape for sets of up to maxW words: the throughput default, or
+// with latency the one that shortens a single item's chain.
+FbPick fb_default_pick(int maxW, bool latency = false);
 int fb_block_sites(int maxW);
 int64_t fb_ckpt_floats(int Lp, int W, int maxW);
 // HP at monomorphic sites (closed form) for the same items.

@@ -161,7 +161,8 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                                                        float *__restrict__ of1_all, int32_t *__restrict__ dip_all,
                                                        uint8_t *__restrict__ H0_all, uint8_t *__restrict__ H1_all,
                                                        int L, float ee, float ed, uint32_t seed, uint32_t iter,
-                                                       int32_t *__restrict__ status) {
+                                                       int32_t *__restrict__ status,
+                                                       const int32_t *__restrict__ order) {
     constexpr int PH_NT = 64 * NW;
     extern __shared__ __align__(16) uint64_t s_rows[];   // [PH_TILE][W] Hvar rows of the current tile
     __shared__ float lds8[2][NW * 8];
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     __shared__ signed char s_type[PH_TILE];
     __shared__ unsigned char s_alt[PH_TILE], s_first[PH_TILE], s_last[PH_TILE];
     __shared__ float s_yin[PH_TILE], s_yout[PH_TILE];
-    const PhItem it = wave_uniform(items[blockIdx.x]);
+    const PhItem it = wave_uniform(items[order ? order[blockIdx.x] : (int)blockIdx.x]);   // optional item order
     const CondItem c = wave_uniform(cond[it.cond]);
     const int K = c.K, W = c.W, tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint64_t *hv = hvar_all + c.hvar_off;
@@ -583,11 +584,11 @@ template <int NW, int KPT>
 static void phase_go(gh_panel *P, int n, const PhItem *d_items, const CondItem *d_cond, const uint64_t *d_hvar,
                      const Locus *d_loci, const int32_t *d_pos, double nrho, PhaseBufs &B, uint8_t *d_H0,
                      uint8_t *d_H1, float ee, float ed, uint32_t seed, uint32_t iter, int32_t *d_status,
-                     hipStream_t s) {
+                     hipStream_t s, const int32_t *d_order) {
     const size_t smem = (size_t)PH_TILE * ((KPT * 64 * NW + 63) / 64) * 8;
     hipLaunchKernelGGL((phase_kernel<NW, KPT>), dim3(n), dim3(64 * NW), smem, s, d_items, d_cond, d_hvar, d_loci,
                        d_pos, P->d_cm.as<double>(), nrho, B.seg, B.segsum, B.imp, B.impsum, B.of1, B.dip, d_H0, d_H1,
-                       P->L, ee, ed, seed, iter, d_status);
+                       P->L, ee, ed, seed, iter, d_status, d_order);
 }
 
 // Launch shapes (waves per target, states per thread) the kernel is built for,
@@ -608,7 +609,7 @@ std::vector<PhaseShape> phase_shapes(int maxK) {
     return v;
 }
 
-PhaseShape phase_shape(int maxK) {
+PhaseShape phase_shape(int maxK, bool latency) {
     GH_CHECK(maxK <= 2048, GH_ERR_ARG, "phasing: more than 2048 conditioning states is not supported");
     // GH_PHASE_SHAPE=<NW>x<KPT> pins a shape wherever it fits (testing, reproducing a tuned run)
     if (const char *e = std::getenv("GH_PHASE_SHAPE")) {
@@ -620,15 +621,17 @@ PhaseShape phase_shape(int maxK) {
     // Defaults measured with 1000 targets in flight (five times the CUs): the
     // fewest waves per target whose register budget still lets two or more
     // targets share a CU (4 states per thread below 2048 states, 8 at 2048).
+    // Latency defaults (a run on the critical path): more waves per target.
     const int cls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
     static const PhaseShape dflt[4] = {{1, 4}, {2, 4}, {4, 4}, {4, 8}};
-    return dflt[cls];
+    static const PhaseShape fast[4] = {{4, 1}, {4, 2}, {8, 2}, {8, 4}};
+    return latency ? fast[cls] : dflt[cls];
 }
 
 void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
                   const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
                   uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
-                  hipStream_t s, PhaseShape sh) {
+                  hipStream_t s, PhaseShape sh, const int32_t *d_order) {
     if (n <= 0) return;
     const double nrho = (-0.04 * ne) / P->N;
     const float ed = err_phase, ee = (float)(1.0 - (double)err_phase);
@@ -637,7 +640,7 @@ void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const Con
 #define GH_PH(nw, kpt)                                                                                          \
     if (sh.NW == nw && sh.KPT == kpt) {                                                                         \
         phase_go<nw, kpt>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, \
-                          d_status, s);                                                                         \
+                          d_status, s, d_order);                                                                \
         GH_HIP(hipGetLastError());                                                                              \
         return;                                                                                                 \
     }

@@ -51,13 +51,14 @@ struct PhaseShape {
 };
 // Shapes the kernel is built for that fit maxK states (the tuner's candidates).
 std::vector<PhaseShape> phase_shapes(int maxK);
-// The shape for maxK states: the tuned one for its class if set, else the default.
-PhaseShape phase_shape(int maxK);
-// sh.NW == 0: phase_shape(maxK).
+// The shape for maxK states: a GH_PHASE_SHAPE pin if it fits, else the
+// throughput default of the class, or with latency the latency default.
+PhaseShape phase_shape(int maxK, bool latency = false);
+// sh.NW == 0: phase_shape(maxK).  d_order (optional): workgroup b runs item d_order[b].
 void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
                   const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
                   uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
-                  hipStream_t s, PhaseShape sh = PhaseShape{0, 0});
+                  hipStream_t s, PhaseShape sh = PhaseShape{0, 0}, const int32_t *d_order = nullptr);
 // Padded state count of a shape.
 int phase_kpt_states(PhaseShape sh);
 // Floats per stored message of a K-state target (8 per state, K rounded up to 64).

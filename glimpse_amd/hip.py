@@ -329,9 +329,12 @@ class Run:
     def execute(self):
         _check(lib().gh_run_execute(self.h))
 
-    def set_tuning(self, on=True):
-        """Launch-shape autotuning over the first executions (see gh_run_set_tuning)."""
-        _check(lib().gh_run_set_tuning(self.h, int(bool(on))))
+    def set_tuning(self, mode=1):
+        """Launch shapes (gh_run_set_tuning): 0 or False = throughput defaults,
+        1 or True = autotune over the first executions, 2 or "latency" = latency
+        defaults for the run on a batch's critical path."""
+        m = 2 if mode == "latency" else int(mode)
+        _check(lib().gh_run_set_tuning(self.h, m))
 
     def collect(self, from_iter=1):
         """Record (features, label, weight) rows for every PBWT pool candidate at

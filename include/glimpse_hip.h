@@ -166,15 +166,18 @@ int gh_run_execute(gh_run *r);
 int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1, int32_t *nstates,
                  gh_run_stats *stats);
 int gh_run_destroy(gh_run *r);
-/* Launch-shape autotuning (waves per target and work per lane of the
- * forward-backward and phasing kernels), off by default.  When on, the PBWT
- * iterations of the first executions each run the next candidate shape of the
- * current set-size class and time it; once all candidates are timed the
- * fastest is kept for the rest of the run's executions.  Shapes change only
- * the summation order, so results agree to float rounding, but a run is
- * bit-reproducible only with tuning off (or with GH_FB_SHAPE=<NW>x<MWL> and
- * GH_PHASE_SHAPE=<NW>x<KPT> pinning the shapes). */
-int gh_run_set_tuning(gh_run *r, int32_t on);
+/* Launch shapes (waves per target and work per lane of the forward-backward
+ * and phasing kernels).  mode 0 (the default): throughput defaults, the
+ * fewest waves per target that fill a CU, right when many runs share the GPU.
+ * mode 1: autotuning; the PBWT iterations of the first executions each run
+ * the next candidate shape of the current set-size class and time it; once all
+ * candidates are timed the fastest is kept for the rest of the run's
+ * executions.  mode 2: latency defaults, more waves per target, for the run
+ * whose chain is the critical path of a batch (the longest chunk).  Shapes
+ * change only the summation order, so results agree to float rounding; a run
+ * is bit-reproducible with modes 0 and 2 (or with GH_FB_SHAPE=<NW>x<MWL> and
+ * GH_PHASE_SHAPE=<NW>x<KPT> pinning the shapes in mode 0). */
+int gh_run_set_tuning(gh_run *r, int32_t mode);
 
 /* Output fields as genotype_writer::writeGenotypes computes them
  * (phase/src/io/genotype_writer.cpp:109-190), bit for bit, for T diploid
