@@ -340,7 +340,10 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         // per-target K (caller_initialise.cpp:150) and PBWT classes by effective modulo
         G->kp.resize(T);
         for (int t = 0; t < T; t++) G->kp[t] = std::min(kpt ? kpt[t] : prm->kpbwt, N);
-        std::map<float, int> cls_of_mod;
+        // Class ids follow ascending effective modulo: the id keys the class's
+        // stored-site draws, so it must not depend on the order of the targets.
+        std::map<float, std::vector<int>> grp_of_mod;
+        std::vector<float> em_t(T, 0.0f);
         G->hap_cls.assign(2 * T, 0);
         for (int t = 0; t < T; t++) {
             GH_CHECK(G->kp[t] > 0, GH_ERR_STATE, "target " + std::to_string(t) + " has Kpbwt = 0 and no state list");
@@ -349,15 +352,18 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             if (G->kp[t] >= N) continue;
             GH_CHECK(P->pbwt, GH_ERR_ARG, "Kpbwt < N needs a panel created with its sparse-PBWT stream");
             std::vector<int> grp;
-            const float em = pbwt_groups(P, prm->pbwt_depth, prm->pbwt_modulo_cm, G->kp[t], grp);
-            auto it = cls_of_mod.find(em);
-            int c;
-            if (it == cls_of_mod.end()) {
-                c = (int)G->cls_mod.size();
-                cls_of_mod[em] = c;
-                G->cls_mod.push_back(em);
-                G->cls_grp.push_back(grp);
-            } else c = it->second;
+            em_t[t] = pbwt_groups(P, prm->pbwt_depth, prm->pbwt_modulo_cm, G->kp[t], grp);
+            grp_of_mod[em_t[t]] = grp;
+        }
+        std::map<float, int> cls_of_mod;
+        for (auto &g : grp_of_mod) {
+            cls_of_mod[g.first] = (int)G->cls_mod.size();
+            G->cls_mod.push_back(g.first);
+            G->cls_grp.push_back(g.second);
+        }
+        for (int t = 0; t < T; t++) {
+            if (G->kp[t] >= N) continue;
+            const int c = cls_of_mod[em_t[t]];
             G->hap_cls[2 * t] = G->hap_cls[2 * t + 1] = c;
             G->pbwt_targets.push_back(t);
         }

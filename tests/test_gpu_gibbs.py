@@ -53,3 +53,25 @@ def test_long_run_statistical_parity(case):
     gr = ref["h0"].astype(int) + ref["h1"]
     assert np.mean(gg == gr) > 0.98
     assert got["stats"]["fb_state_sites"] > 0
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_mixed_k_run_matches_oracle(case, mode):
+    """Targets whose PBWT K spans three set-size classes.  With shape modes 0
+    (throughput defaults) and 2 (latency defaults) every forward-backward and
+    phasing pass splits into a launch for the largest class and one beside it
+    on a second stream; mode 1 (autotuning) keeps one launch per pass.  All
+    must give the oracle's run."""
+    from glimpse_amd import hip
+    P, gl, flat, OP, GP = case
+    kpt = np.array([100, 250, 600, 950, 100, 400, 1000, 200], np.int32)
+    kw = dict(burnin=0, main_iter=1, kpbwt=int(kpt.max()), kinit=600, seed=99)
+    ref = OP.phase_run(gl, flat, orc.default_params(rng_mode=orc.RNG_COUNTER, **kw), kpbwt_per_target=kpt)
+    run = hip.Run(GP, gl, flat, hip.default_params(**kw), kpbwt_per_target=kpt)
+    run.set_tuning(mode)
+    run.execute()
+    got = run.fetch()
+    run.close()
+    assert np.array_equal(got["nstates"], ref["nstates"]), "conditioning-set sizes differ"
+    d = np.abs(got["gp0"] - ref["gp0"]) + np.abs(got["gp1"] - ref["gp1"])
+    assert np.mean(d < 1e-4) > 0.995, "GP differs at %.4f of sites" % np.mean(d >= 1e-4)
