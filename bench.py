@@ -234,9 +234,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serial", action="store_true", help="run chunks one after another")
     ap.add_argument("--path", choices=("both", "plain", "learned"), default="both")
-    ap.add_argument("--k-frac", type=float, default=0.65,
-                    help="learned path: kept fraction of the PBWT pool (0.65: the smallest tried whose dosage r2 is "
-                         "within 0.001 of the plain path's on this workload)")
+    ap.add_argument("--k-frac", type=float, default=0.75,
+                    help="learned path: kept fraction of the PBWT pool (0.65 was the smallest tried whose dosage r2 "
+                         "is within 0.001 of the plain path's on this workload; 0.75 keeps a margin at equal speed)")
     ap.add_argument("--k-max", type=int, default=1024,
                     help="learned path: at most this many kept states per target (0: no cap); 1024 keeps the "
                          "learned sets in the 1024-state launch class")
