@@ -557,8 +557,9 @@ void run_execute(gh_run *G) {
     GH_HIP(hipMemsetAsync(status, 0, sizeof(int32_t) * 2, s));
     gh_run_stats st;
     std::memset(&st, 0, sizeof(st));
-    EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1, e_learn, e_pk;
+    EvPair e_sel, e_cond, e_fb, e_ph, e_k0, e_k1, e_k0b, e_k1b, e_learn, e_pk;
     e_sel.make(); e_cond.make(); e_fb.make(); e_ph.make(); e_k0.make(); e_k1.make(); e_learn.make(); e_pk.make();
+    e_k0b.make(); e_k1b.make();
     const dim3 gsite((L + 255) / 256, T), bsite(256);
     int main_done = 0;
     std::vector<CondItem> &items = G->items;
@@ -716,35 +717,38 @@ void run_execute(gh_run *G) {
             GH_HIP(hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming));
             GH_HIP(hipEventCreateWithFlags(&G->ev_join, hipEventDisableTiming));
         };
-        auto fb_all = [&](float *HPx, float *massx) {
+        // (each kernel dispatch is timed by its own event pair)
+        auto fb_all = [&](float *HPx, float *massx, EvPair &ea, EvPair &eb) {
             if (fbB == 0) {
+                GH_HIP(hipEventRecord(ea.a, s));
                 fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
                           massx, wlo, whi, fbp);
+                GH_HIP(hipEventRecord(ea.b, s));
                 return;
             }
             ensure_s2();
             const int32_t *ord = G->d_ford.as<int32_t>();
             GH_HIP(hipEventRecord(G->ev_fork, s));
             GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
+            GH_HIP(hipEventRecord(ea.a, s));
             fb_launch(P, fbA, fb_d, items_d, fbWA, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
                       massx, wlo, whi, fbp, ord);
+            GH_HIP(hipEventRecord(ea.b, s));
+            GH_HIP(hipEventRecord(eb.a, G->s2));
             fb_launch(P, fbB, fb_d, items_d, fbWB, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, G->s2,
                       massx, wlo, whi, (G->latency || fewB) ? fb_default_pick(fbWB, true) : FbPick{0, 0}, ord + fbA);
+            GH_HIP(hipEventRecord(eb.b, G->s2));
             GH_HIP(hipEventRecord(G->ev_join, G->s2));
             GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
         };
         // H0 (caller_algorithm.cpp:58-65)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
-        GH_HIP(hipEventRecord(e_k0.a, s));
-        fb_all(HP0, mass0);
-        GH_HIP(hipEventRecord(e_k0.b, s));
+        fb_all(HP0, mass0, e_k0, e_k0b);
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
         // H1 (caller_algorithm.cpp:66-71)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, HL);
-        GH_HIP(hipEventRecord(e_k1.a, s));
-        fb_all(HP1, mass1);
-        GH_HIP(hipEventRecord(e_k1.b, s));
+        fb_all(HP1, mass1, e_k1, e_k1b);
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
         GH_HIP(hipGetLastError());
@@ -841,8 +845,8 @@ void run_execute(gh_run *G) {
         st.ms_condset += e_cond.ms();
         st.ms_fb += e_fb.ms();
         st.ms_phase += e_ph.ms();
-        st.fb_kernel_ms += e_k0.ms() + e_k1.ms();
-        st.fb_launches += 2;
+        st.fb_kernel_ms += e_k0.ms() + e_k1.ms() + (fbB > 0 ? e_k0b.ms() + e_k1b.ms() : 0.0f);
+        st.fb_launches += fbB > 0 ? 4 : 2;   // kernel dispatches
         if (G->tune && stage > 0) {
             G->fb_tuner.record(e_k0.ms() + e_k1.ms());
             G->ph_tuner.record(e_pk.ms());
