@@ -1019,6 +1019,7 @@ struct PhaseHMM {
     V8 probSumH;
     std::vector<float> phasingProb, phasingProbSum, phasingProbSumSum;
     std::vector<float> imputeProb, imputeProbSum, imputeProbSumSum, imputeProbOf1s;
+    std::vector<float> hp_trace;   /* TRANS_HAP's HProbs / sumHProbs at each boundary (for pinning tests) */
     std::vector<int> dip;
     float EMIT0[3][8], EMIT1[3][8];
     float HProbs[64], DProbs[8];
@@ -1088,6 +1089,7 @@ void PhaseHMM::reallocate(const std::vector<uint8_t> &H0, const std::vector<uint
     imputeProbSum.assign(n_miss * HAPN, 0.0f);
     imputeProbSumSum.assign(n_miss, 0.0f);
     imputeProbOf1s.assign(n_miss * HAPN, 0.0f);
+    hp_trace.assign((size_t)(n_segs > 0 ? n_segs - 1 : 0) * HAPN * HAPN, 0.0f);
 }
 
 void PhaseHMM::INIT_PEAK_HET(int het) {
@@ -1211,6 +1213,7 @@ bool PhaseHMM::TRANS_HAP() {
         store8(&HProbs[j], S);
         sumHProbs += hadd8(S);
     }
+    for (int i = 0; i < HAPN * HAPN; i++) hp_trace[(size_t)cseg * HAPN * HAPN + i] = HProbs[i] / sumHProbs;
     return std::isnan(sumHProbs) || std::isinf(sumHProbs) || sumHProbs < std::numeric_limits<float>::min();
 }
 bool PhaseHMM::SAMPLE_DIP(Rng &rng, int tgt) {
@@ -1793,6 +1796,30 @@ int orc_rephase(void *panel, const int32_t *states, int32_t K, int32_t ne, float
         D.rephase(H0, H1, fl, rng, tgt);
         std::memcpy(h0, H0.data(), P.L);
         std::memcpy(h1, H1.data(), P.L);
+    });
+}
+
+int orc_phase_trace(void *panel, const int32_t *states, int32_t K, int32_t ne, float err_phase, const uint8_t *flat,
+                    const uint8_t *h0, const uint8_t *h1, uint32_t seed, int32_t it, int32_t tgt, int32_t cap_segs,
+                    int32_t cap_miss, int32_t *n_segs, int32_t *n_miss, float *seg_sums, float *trans_hap,
+                    float *of1) {
+    ORC_TRY({
+        Panel &P = *(Panel *)panel;
+        CondSet C(P, ne, 1e-12f, err_phase);
+        C.build(std::vector<int>(states, states + K));
+        PhaseHMM D(&C);
+        Rng rng; rng.mode = ORC_RNG_COUNTER; rng.seed = seed; rng.it = it;
+        std::vector<uint8_t> H0(h0, h0 + P.L), H1(h1, h1 + P.L);
+        std::vector<char> fl(flat, flat + P.L);
+        D.rephase(H0, H1, fl, rng, tgt);
+        *n_segs = (int32_t)D.n_segs;
+        *n_miss = (int32_t)D.n_miss;
+        if ((int32_t)D.n_segs > cap_segs || (int32_t)D.n_miss > cap_miss) throw OrcError("orc_phase_trace: capacity");
+        for (unsigned s = 0; s < D.n_segs; s++)
+            for (int h = 0; h < HAPN; h++)
+                seg_sums[s * HAPN + h] = D.phasingProbSum[s * HAPN + h] / D.phasingProbSumSum[s];
+        std::copy(D.hp_trace.begin(), D.hp_trace.end(), trans_hap);
+        std::copy(D.imputeProbOf1s.begin(), D.imputeProbOf1s.end(), of1);
     });
 }
 

@@ -125,6 +125,16 @@ int orc_rephase(void *panel, const int32_t *states, int32_t K, int32_t ne,
                 float err_phase, const uint8_t *flat, uint8_t *h0, uint8_t *h1,
                 uint32_t seed, int32_t it, int32_t tgt);
 
+/* The deterministic quantities of one rephase (for pinning tests): per
+ * segment start the backward pass's per-hypothesis sums over the states
+ * divided by their total (n_segs x 8), per segment boundary TRANS_HAP's 8x8
+ * joint hypothesis probabilities divided by their total ((n_segs-1) x 64), and
+ * per flat het the probability of allele 1 under each hypothesis (n_miss x 8). */
+int orc_phase_trace(void *panel, const int32_t *states, int32_t K, int32_t ne, float err_phase, const uint8_t *flat,
+                    const uint8_t *h0, const uint8_t *h1, uint32_t seed, int32_t it, int32_t tgt, int32_t cap_segs,
+                    int32_t cap_miss, int32_t *n_segs, int32_t *n_miss, float *seg_sums, float *trans_hap,
+                    float *of1);
+
 /* Philox4x32-10 block (for known-answer tests). */
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* The n-th output (1-based) of a default-seeded std::mt19937. */

@@ -66,6 +66,9 @@ def lib():
         L.orc_hap_likelihoods.argtypes = [ctypes.c_int32, vp, vp, vp, ctypes.c_int32, ctypes.c_float, vp]
         L.orc_rephase.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, vp, vp, vp,
                                   ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32]
+        L.orc_phase_trace.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, vp, vp, vp,
+                                      ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      vp, vp, vp, vp, vp]
         L.orc_philox4x32.argtypes = [vp, vp, vp]
         L.orc_mt19937_nth.restype = ctypes.c_uint32
         L.orc_mt19937_nth.argtypes = [ctypes.c_uint32, ctypes.c_int64]
@@ -218,6 +221,24 @@ class Panel:
         _check(lib().orc_rephase(self.h, _p(st), st.size, ne, ctypes.c_float(err_phase),
                                  _p(np.ascontiguousarray(flat, np.uint8)), _p(a), _p(b), seed, it, tgt))
         return a, b
+
+    def phase_trace(self, states, flat, h0, h1, seed=1, it=1, tgt=0, ne=100000, err_phase=1e-4, cap=4096):
+        """The deterministic quantities of one rephase: per segment start the
+        backward sums per hypothesis over their total [n_segs][8], per segment
+        boundary TRANS_HAP's joint hypothesis probabilities [n_segs-1][8][8], per
+        flat het P(allele 1 | hypothesis) [n_miss][8]."""
+        st = np.ascontiguousarray(states, np.int32)
+        ns, nm = ctypes.c_int32(), ctypes.c_int32()
+        seg = np.zeros((cap, 8), np.float32)
+        hp = np.zeros((cap, 8, 8), np.float32)
+        of1 = np.zeros((cap, 8), np.float32)
+        _check(lib().orc_phase_trace(self.h, _p(st), st.size, ne, ctypes.c_float(err_phase),
+                                     _p(np.ascontiguousarray(flat, np.uint8)),
+                                     _p(np.ascontiguousarray(h0, np.uint8)), _p(np.ascontiguousarray(h1, np.uint8)),
+                                     seed, it, tgt, cap, cap, ctypes.byref(ns), ctypes.byref(nm), _p(seg), _p(hp),
+                                     _p(of1)))
+        n, m = ns.value, nm.value
+        return seg[:n], hp[:max(n - 1, 0)], of1[:m]
 
     def phase_run(self, gl, flat, params, kpbwt_per_target=None):
         gl = np.ascontiguousarray(gl, np.uint8)
