@@ -389,6 +389,10 @@ struct Sweep {
     std::vector<unsigned char> tar_hap;
     std::vector<int> rareTar;
     std::vector<std::vector<int>> SvarTar;
+    /* trace for pinning tests: after every HQ common site, per target haplotype
+     * (site, hap, idx, f, g, site of the last reset of [f, g) or -1) */
+    std::vector<int32_t> *trace = nullptr;
+    std::vector<int> creset;
     std::vector<bool> stored;
     std::vector<std::vector<std::vector<int>>> states;  /* [sample][depth] */
 
@@ -493,6 +497,7 @@ void Sweep::select_common(int k, int l_hq, int l_all, int ref_rac_l, int prev_re
             gd = a * P.N + (1 - a) * ref_rac_l;
             last_reset[htr] = k;
             reset = true;
+            if (trace) creset[htr] = k;
         }
         idx[htr] = id; f[htr] = fd; g[htr] = gd;
         if (!reset && stored[l_hq]) {
@@ -500,6 +505,11 @@ void Sweep::select_common(int k, int l_hq, int l_all, int ref_rac_l, int prev_re
             f[htr] = a * ref_rac_l;
             g[htr] = a * P.N + (1 - a) * ref_rac_l;
             last_reset[htr] = k;
+            if (trace) creset[htr] = k;
+        }
+        if (trace) {
+            const int32_t row[6] = {k, htr, idx[htr], f[htr], g[htr], creset[htr]};
+            trace->insert(trace->end(), row, row + 6);
         }
     }
 }
@@ -623,6 +633,7 @@ void Sweep::run(const std::vector<bool> &stored_flags, const std::vector<int> &s
     tar_hap.assign(n_tar_haps, 0);
     stored = stored_flags;
     for (int e = 0; e < n_tar_haps; e++) { f[e] = 0; g[e] = N; idx[e] = start[e]; }
+    creset.assign(n_tar_haps, -1);
     states.assign(n_samples, std::vector<std::vector<int>>(depth));
     /* transposeRareTar: target haps carrying the non-major allele at rare sites */
     SvarTar.assign(P.L, {});
@@ -1708,6 +1719,26 @@ int orc_pbwt_select(void *panel, int32_t T, const uint8_t *hap_alleles, const ui
                 if (o + (int64_t)v.size() > cap) throw OrcError("pbwt_select: output capacity exceeded");
                 for (int x : v) out_ids[o++] = x;
             }
+    });
+}
+
+int orc_pbwt_trace(void *panel, int32_t T, const uint8_t *hap_alleles, const uint8_t *stored,
+                   const int32_t *start_idx, int32_t depth, float modulo_cm, int32_t *rows, int64_t cap,
+                   int64_t *n_rows) {
+    ORC_TRY({
+        Panel &P = *(Panel *)panel;
+        std::vector<std::vector<uint8_t>> haps(2 * T);
+        for (int h = 0; h < 2 * T; h++) haps[h].assign(hap_alleles + (size_t)h * P.L, hap_alleles + (size_t)(h + 1) * P.L);
+        std::vector<bool> st(P.n_com_hq);
+        for (int i = 0; i < P.n_com_hq; i++) st[i] = stored[i];
+        std::vector<int> start(start_idx, start_idx + 2 * T);
+        std::vector<int32_t> tr;
+        Sweep S(P, haps, T, depth, modulo_cm);
+        S.trace = &tr;
+        S.run(st, start);
+        *n_rows = (int64_t)tr.size() / 6;
+        if ((int64_t)tr.size() > cap * 6) throw OrcError("pbwt_trace: capacity");
+        std::copy(tr.begin(), tr.end(), rows);
     });
 }
 

@@ -135,6 +135,13 @@ int orc_phase_trace(void *panel, const int32_t *states, int32_t K, int32_t ne, f
                     int32_t cap_miss, int32_t *n_segs, int32_t *n_miss, float *seg_sums, float *trans_hap,
                     float *of1);
 
+/* The sparse-PBWT sweep of orc_pbwt_select with a trace (for pinning tests):
+ * after every HQ common site, per target haplotype, one row (site, haplotype,
+ * idx, f, g, site of the last reset of [f, g) or -1). */
+int orc_pbwt_trace(void *panel, int32_t T, const uint8_t *hap_alleles, const uint8_t *stored,
+                   const int32_t *start_idx, int32_t depth, float modulo_cm, int32_t *rows, int64_t cap,
+                   int64_t *n_rows);
+
 /* Philox4x32-10 block (for known-answer tests). */
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* The n-th output (1-based) of a default-seeded std::mt19937. */

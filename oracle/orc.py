@@ -56,6 +56,8 @@ def lib():
                                     vp, vp, vp, vp, vp, vp]
         L.orc_pbwt_select.argtypes = [vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, ctypes.c_float,
                                       vp, vp, vp, ctypes.c_int64]
+        L.orc_pbwt_trace.argtypes = [vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, ctypes.c_float, vp,
+                                     ctypes.c_int64, vp]
         L.orc_pbwt_groups.argtypes = [vp, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, vp, vp,
                                       ctypes.c_int32, vp]
         L.orc_condset.argtypes = [vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, ctypes.c_int32,
@@ -154,6 +156,19 @@ class Panel:
         _check(lib().orc_pbwt_groups(self.h, depth, ctypes.c_float(modulo), kpbwt, ctypes.byref(eff),
                                      _p(grp), grp.size, ctypes.byref(n)))
         return eff.value, grp[:n.value]
+
+    def pbwt_trace(self, hap_alleles, stored, start_idx, depth=12, modulo=0.1):
+        """Rows (site, haplotype, idx, f, g, last reset of [f, g) or -1) after every
+        HQ common site of the sweep, per target haplotype."""
+        haps = np.ascontiguousarray(hap_alleles, np.uint8)
+        T = haps.shape[0] // 2
+        cap = int(haps.shape[0]) * int(haps.shape[1]) + 1
+        rows = np.zeros((cap, 6), np.int32)
+        n = ctypes.c_int64()
+        _check(lib().orc_pbwt_trace(self.h, T, _p(haps), _p(np.ascontiguousarray(stored, np.uint8)),
+                                    _p(np.ascontiguousarray(start_idx, np.int32)), depth, ctypes.c_float(modulo),
+                                    _p(rows), cap, ctypes.byref(n)))
+        return rows[:n.value]
 
     def pbwt_select(self, hap_alleles, stored, start_idx, depth=12, modulo=0.1, cap=None):
         haps = np.ascontiguousarray(hap_alleles, np.uint8)
