@@ -4,17 +4,19 @@
 Workload (the batched configuration): a synthetic panel of 10,000 haplotypes x
 60,000 biallelic SNPs (5,000 SNPs per Mb at 1 cM/Mb, 50-founder Li-Stephens
 mosaics, 1/x frequency spectrum, mutation 1e-3), 200 diploid targets with read
-coverage U(0.1, 2)x, cut into 5 chunks of U[20k, 60k] sites (windows of the
-panel, each with its own sparse PBWT), targets spread over the chunks, Kpbwt
-per target U[500, 2000], default Gibbs schedule (1 INIT + 5 burn-in + 15
+coverage U(0.1, 2)x, and 5 chunks of U[20k, 60k] sites (windows of the panel,
+each with its own sparse PBWT); every chunk is phased for all 200 targets,
+Kpbwt per target U[500, 2000], default Gibbs schedule (1 INIT + 5 burn-in + 15
 main).  One step = one complete phase run of every chunk (all targets, all
 iterations) with the inputs already resident in HBM; chunks run concurrently
 on their own HIP streams.
 
 Two paths over the same inputs:
   plain PBWT  -- every target conditions on its Kpbwt PBWT states;
-  learned     -- the selector network keeps Kpbwt/5 states of each target's
-                 PBWT pool (from iteration 1 on; INIT is unchanged).
+  learned     -- from iteration 1 on, the selector network keeps
+                 min(round(0.75 Kpbwt), 1024) states of each target's PBWT
+                 pool, falling back to the whole pool when the kept states'
+                 predicted mass is below 0.95 (INIT is unchanged).
 The network is first trained on the GPU (outside the timed region) on rows a
 plain run records: a separate panel of 4,000 haplotypes x 10,000 SNPs (2 Mb),
 48 targets at 1x, Kpbwt 2,000, every PBWT iteration; labels are the HMM's own
@@ -24,16 +26,17 @@ metric value: target-site imputations/s = sum over chunks of targets x chunk
 sites / wall time of one step, for the learned path when its dosage r2 is
 within 0.001 of the plain path's (the "equal r2" condition), else for the
 plain path (config.path says which).  Both paths' figures, dosage r2 / NRC
-against the simulated truth, and the learned path's mean |DS - DS(plain)| (the
-plain run at Kpbwt judges its dosage error) are in the same line.  The roofline
-object reports the dominant kernel (haploid forward-backward) against HBM:
-algorithmic bytes per launch / average launch duration (HIP events on the
-launch stream); `traffic` is the HBM bytes per launch measured with rocprofv3
-counters for this workload (profiles/fb_kernel_c2_<path>_traffic.md).
-cpu_baseline:
-the CPU oracle on a bounded sample of the same workload (one target of the
-smallest chunk, INIT + 1 burn-in + 1 main iteration, scaled to the 21
-iterations of a full run), one core.
+against the simulated truth, and the learned path's mean |DS - DS(plain)| (its
+dosage error measured against the plain run at Kpbwt) are in the same line.
+The roofline object reports the dominant kernel (haploid forward-backward)
+against HBM: algorithmic bytes per launch / average launch duration (HIP
+events on the launch stream); `traffic` is the HBM bytes per launch measured
+with rocprofv3 counters for this workload
+(profiles/fb_kernel_batch200x5_<path>_traffic.md).
+cpu_baseline: the CPU oracle on a bounded sample of the same workload (targets
+of the smallest chunk, INIT + 1 burn-in + 1 main iteration each, scaled to the
+21 iterations of a full run), one target per thread on up to 16 host threads,
+with the one-core figure beside it.
 """
 import argparse
 import json
@@ -118,8 +121,8 @@ def accuracy(outs, chunks):
 
 def cpu_baseline(chunks):
     """The CPU oracle timed on the host: one target alone (one core), then one
-    target per thread on as many threads as the host share allows (16 on a
-    one-GPU box), each an INIT + 1 burn-in + 1 main run of the smallest chunk,
+    target per thread on as many threads as the host share allows (up to
+    16), each an INIT + 1 burn-in + 1 main run of the smallest chunk,
     scaled by 21/3 to the full 1 + 5 + 15 schedule.  The oracle's C calls
     release the interpreter lock, so the threads run in parallel."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

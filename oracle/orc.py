@@ -112,6 +112,7 @@ class Panel:
         k = self._keep
         d = PanelDesc(P["L"], P["N"], _p(k["cm"]).value, _p(k["cref"]).value, _p(k["calt"]).value,
                       _p(k["lq"]).value, _p(k["cw"]).value, _p(k["rp"]).value, _p(k["ri"]).value)
+        self._free = lib().orc_panel_free   # held so that teardown at interpreter exit can still free
         self.h = lib().orc_panel_create(ctypes.byref(d), ctypes.c_float(sparse_maf))
         if not self.h:
             raise RuntimeError("oracle: " + lib().orc_last_error().decode())
@@ -119,7 +120,7 @@ class Panel:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_panel_free(self.h)
+            self._free(self.h)
             self.h = None
 
     @property

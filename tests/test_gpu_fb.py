@@ -8,7 +8,7 @@ from conftest import hvar_bytes_to_words, mask_words
 
 pytestmark = pytest.mark.gpu
 
-FB_TOL = 1e-4   # absolute, on haplotype posteriors (north-star bound)
+FB_TOL = 1e-4   # absolute, on haplotype posteriors
 
 
 def _state_sets(N, rng):
