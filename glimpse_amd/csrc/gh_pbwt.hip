@@ -35,10 +35,17 @@ namespace gh {
 
 // LDS tiles of the walk: the rank data (column words + popcount prefixes) of
 // consecutive site ops is contiguous, so a tile is one copy of a word range.
-constexpr int WALK_NT = 256;
-constexpr int TILE_WORDS = 8192;
+// A workgroup of the staged walk is one wave of walkers (one target haplotype
+// per lane) and three loader waves; tiles are double buffered, so the loaders
+// copy tile t+1 while the walkers walk tile t.
+constexpr int WALK_WALKERS = 64;
+constexpr int WALK_LOADERS = 192;
+constexpr int WALK_NT = WALK_WALKERS + WALK_LOADERS;
+constexpr int TILE_WORDS = 4096;
 constexpr int TILE_OPS = 64;
-constexpr size_t WALK_LDS = (size_t)TILE_WORDS * 12 + sizeof(SiteOp) * TILE_OPS + (size_t)TILE_OPS * WALK_NT;
+constexpr size_t WALK_BUF = (size_t)TILE_WORDS * 12 + sizeof(SiteOp) * TILE_OPS + (size_t)TILE_OPS * WALK_WALKERS;
+constexpr size_t WALK_LDS = 2 * WALK_BUF;
+static_assert(WALK_BUF % 16 == 0, "tile buffers keep 16-byte alignment");
 
 /* ------------------------------------------------------------------ host: pack3 */
 // pack3 code (ref_haplotype_set.h:45-85): bit 7 = symbol; 0x40 clear -> count in
@@ -356,13 +363,22 @@ __device__ __forceinline__ int rank1(const uint64_t *bits, const uint32_t *cnt, 
     if (b) r += __popcll(bits[off + w] & ((1ull << b) - 1ull));
     return r;
 }
-__device__ __forceinline__ int lower_count(const int32_t *a, int n, int v) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1; else hi = mid;
+// Counts of the entries of the sorted list a[0, n) below v0, v1 and v2 (three
+// lower bounds at once).  The lists are the short carrier sets of rare runs
+// (tens of entries), which the lanes of a wave read at the same addresses, so
+// batches of independent loads beat dependent binary-search steps.
+__device__ __forceinline__ void lower_count3(const int32_t *a, int n, int v0, int v1, int v2, int &c0, int &c1,
+                                             int &c2) {
+    c0 = c1 = c2 = 0;
+    const int vmax = max(v0, max(v1, v2));
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        int x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = j0 + u < n ? a[j0 + u] : INT_MAX;
+#pragma unroll
+        for (int u = 0; u < 8; u++) { c0 += x[u] < v0; c1 += x[u] < v1; c2 += x[u] < v2; }
+        if (x[7] >= vmax) break;
     }
-    return lo;
 }
 
 // STAGED: the rank data, site ops and this tile's target alleles / stored
@@ -371,16 +387,14 @@ __device__ __forceinline__ int lower_count(const int32_t *a, int n, int v) {
 // STAGED every lookup goes to global memory (panels whose single site exceeds
 // a tile).
 template <bool STAGED>
-__global__ __launch_bounds__(WALK_NT) void pbwt_walk_kernel(WalkArgs w) {
+__global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkArgs w) {
     extern __shared__ uint64_t wsm[];
-    uint64_t *s_bits = wsm;
-    uint32_t *s_cnt = (uint32_t *)(wsm + TILE_WORDS);
-    SiteOp *s_ops = (SiteOp *)(s_cnt + TILE_WORDS);
-    uint8_t *s_a = (uint8_t *)(s_ops + TILE_OPS);
-    const int h0 = blockIdx.x * WALK_NT + threadIdx.x;
-    const bool act = h0 < w.n_haps;
+    constexpr int NWALK = STAGED ? WALK_WALKERS : 64;
+    const bool loader = STAGED && (int)threadIdx.x >= NWALK;
+    const int h0 = blockIdx.x * NWALK + (loader ? 0 : (int)threadIdx.x);
+    const bool act = !loader && h0 < w.n_haps;
     if (!STAGED && !act) return;
-    const int h = act ? h0 : 0;   // idle threads only help to fill the tiles
+    const int h = act ? h0 : 0;   // loaders and idle walkers only move tiles
     const int N = w.N, depth = w.depth, rec = 2 + 2 * depth;
     const uint8_t *hap = w.haps + (size_t)(w.hap_row ? w.hap_row[h] : h) * w.L;
     const uint8_t *stored = w.stored + (size_t)w.hap_class[h] * w.Lhq;
@@ -417,59 +431,93 @@ __global__ __launch_bounds__(WALK_NT) void pbwt_walk_kernel(WalkArgs w) {
         nev++;
     };
 
-    const int n_tiles = STAGED ? w.n_tiles : 1;
-    for (int ti = 0; ti < n_tiles; ti++) {
-    int o0 = 0, o1 = w.n_ops;
-    int64_t r0 = 0;
-    if (STAGED) {
-        o0 = w.tile_op[ti];
-        o1 = w.tile_op[ti + 1];
-        r0 = w.ops[o0].roff;
+    // Copies tile ti into buffer b: rank words and counts, the site ops, and
+    // for every walker of this workgroup its allele at each op (bit 0) and, at
+    // common ops, its class's stored flag (bit 1).  Threads t0, t0+nt, ... take
+    // part; batches of independent loads, so a tile costs a few latencies.
+    auto stage = [&](int ti, int b, int t0, int nt) __attribute__((always_inline)) {
+        uint64_t *sb = (uint64_t *)((uint8_t *)wsm + (size_t)b * WALK_BUF);
+        uint32_t *sc = (uint32_t *)(sb + TILE_WORDS);
+        SiteOp *so = (SiteOp *)(sc + TILE_WORDS);
+        uint8_t *sa = (uint8_t *)(so + TILE_OPS);
+        const int o0 = w.tile_op[ti], o1 = w.tile_op[ti + 1], no = o1 - o0;
+        const int64_t r0 = w.ops[o0].roff;
         const int64_t r1 = o1 < w.n_ops ? w.ops[o1].roff : w.total_words;
-        __syncthreads();   // the previous tile is no longer read
-        // batches of independent loads, so a tile costs a few memory latencies
         const int nwd = (int)(r1 - r0);
-        for (int i0 = 0; i0 < nwd; i0 += WALK_NT * 8) {
+        for (int i0 = t0; i0 < nwd; i0 += nt * 8) {
             uint64_t vb[8];
             uint32_t vc[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const int i = i0 + u * WALK_NT + threadIdx.x;
+                const int i = i0 + u * nt;
                 if (i < nwd) { vb[u] = w.bits[r0 + i]; vc[u] = w.cnt[r0 + i]; }
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const int i = i0 + u * WALK_NT + threadIdx.x;
-                if (i < nwd) { s_bits[i] = vb[u]; s_cnt[i] = vc[u]; }
+                const int i = i0 + u * nt;
+                if (i < nwd) { sb[i] = vb[u]; sc[i] = vc[u]; }
             }
         }
-        for (int i = threadIdx.x; i < o1 - o0; i += WALK_NT) s_ops[i] = w.ops[o0 + i];
-        __syncthreads();
-        const int no = o1 - o0;
-        for (int j0 = 0; j0 < no; j0 += 16) {
+        for (int i = t0; i < no; i += nt) so[i] = w.ops[o0 + i];
+        // pairs (walker, op), ops fastest: neighbouring lanes read neighbouring
+        // sites of one haplotype row
+        const int npair = no * NWALK;
+        for (int p0 = t0; p0 < npair; p0 += nt * 16) {
             uint8_t va[16];
 #pragma unroll
             for (int u = 0; u < 16; u++) {
-                const int j = j0 + u;
-                if (j < no) {
-                    const SiteOp &oj = s_ops[j];
-                    va[u] = (uint8_t)(hap[oj.k] | (oj.kind == 0 ? (stored[oj.l] != 0) << 1 : 0));
+                const int p = p0 + u * nt;
+                if (p < npair) {
+                    const int wk = p / no, j = p - wk * no;
+                    const int hw0 = blockIdx.x * NWALK + wk;
+                    const int hw = hw0 < w.n_haps ? hw0 : 0;
+                    const SiteOp &oj = w.ops[o0 + j];
+                    const uint8_t *hp = w.haps + (size_t)(w.hap_row ? w.hap_row[hw] : hw) * w.L;
+                    const uint8_t *st = w.stored + (size_t)w.hap_class[hw] * w.Lhq;
+                    va[u] = (uint8_t)(hp[oj.k] | (oj.kind == 0 ? (st[oj.l] != 0) << 1 : 0));
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 16; u++)
-                if (j0 + u < no) s_a[(j0 + u) * WALK_NT + threadIdx.x] = va[u];
+            for (int u = 0; u < 16; u++) {
+                const int p = p0 + u * nt;
+                if (p < npair) {
+                    const int wk = p / no, j = p - wk * no;
+                    sa[j * NWALK + wk] = va[u];
+                }
+            }
         }
+    };
+
+    const int n_tiles = STAGED ? w.n_tiles : 1;
+    if (STAGED) {
+        stage(0, 0, threadIdx.x, WALK_NT);
         __syncthreads();
     }
-    const uint64_t *Rb = STAGED ? s_bits : w.bits;
-    const uint32_t *Rc = STAGED ? s_cnt : w.cnt;
+    for (int ti = 0; ti < n_tiles; ti++) {
+    int o0 = 0, o1 = w.n_ops;
+    int64_t r0 = 0;
+    const uint64_t *Rb = w.bits;
+    const uint32_t *Rc = w.cnt;
+    const SiteOp *s_ops = nullptr;
+    const uint8_t *s_a = nullptr;
+    if (STAGED) {
+        o0 = w.tile_op[ti];
+        o1 = w.tile_op[ti + 1];
+        r0 = w.ops[o0].roff;
+        const int b = ti & 1;
+        Rb = (const uint64_t *)((const uint8_t *)wsm + (size_t)b * WALK_BUF);
+        Rc = (const uint32_t *)(Rb + TILE_WORDS);
+        s_ops = (const SiteOp *)(Rc + TILE_WORDS);
+        s_a = (const uint8_t *)(s_ops + TILE_OPS);
+        // buffer b ^ 1 was last read during tile ti - 1, before the barrier
+        if (loader && ti + 1 < n_tiles) stage(ti + 1, b ^ 1, (int)threadIdx.x - NWALK, WALK_LOADERS);
+    }
     if (act)
     for (int o = o0; o < o1; o++) {
         const SiteOp op = STAGED ? s_ops[o - o0] : w.ops[o];
         const int k = op.k;
         const int64_t roff = op.roff - r0;
-        const int sa = STAGED ? s_a[(o - o0) * WALK_NT + threadIdx.x] : 0;
+        const int sa = STAGED ? s_a[(o - o0) * NWALK + threadIdx.x] : 0;
         const int a_here = STAGED ? (sa & 1) : hap[k];
         if (op.kind == 0) {
             if (op.init) {
@@ -488,7 +536,8 @@ __global__ __launch_bounds__(WALK_NT) void pbwt_walk_kernel(WalkArgs w) {
                         last_reset = k - 1;
                     }
                 } else {
-                    const int c0 = lower_count(si, ns, f), c1 = lower_count(si, ns, g), c2 = lower_count(si, ns, idx);
+                    int c0, c1, c2;
+                    lower_count3(si, ns, f, g, idx, c0, c1, c2);
                     id = idx - c2; fd = f - c0; gd = g - c1;
                     if (gd <= fd) {
                         const int ps = op.l >= 1 ? w.c2t[op.l - 1] : 0;
@@ -534,16 +583,27 @@ __global__ __launch_bounds__(WALK_NT) void pbwt_walk_kernel(WalkArgs w) {
                 // haplotype_set::init_rare (haplotype_set.cpp:632-680)
                 in_rare = false;
                 if (ns > 0) {
+                    // the run's ops and this haplotype's alleles are in the tile up to o1
                     for (int j = o; j < w.n_ops; j++) {
-                        const SiteOp oj = w.ops[j];
-                        if (oj.kind == 0) break;
-                        if (hap[oj.k] != oj.major) { in_rare = true; break; }
+                        int kind, major, aj;
+                        if (STAGED && j < o1) {
+                            kind = s_ops[j - o0].kind;
+                            major = s_ops[j - o0].major;
+                            aj = s_a[(j - o0) * NWALK + threadIdx.x] & 1;
+                        } else {
+                            const SiteOp &oj = w.ops[j];
+                            kind = oj.kind;
+                            major = oj.major;
+                            aj = kind == 0 ? 0 : hap[oj.k];
+                        }
+                        if (kind == 0) break;
+                        if (aj != major) { in_rare = true; break; }
                     }
                 }
                 if (in_rare) {
                     const int32_t *si = w.aidx + w.aptr[op.l];
-                    int fd = lower_count(si, ns, f), gd = lower_count(si, ns, g);
-                    const int c2 = lower_count(si, ns, idx);
+                    int fd, gd, c2;
+                    lower_count3(si, ns, f, g, idx, fd, gd, c2);
                     if (gd <= fd) { fd = 0; gd = ns; last_reset = k; }
                     sidx = c2; sf = fd; sg = gd;
                 }
@@ -567,6 +627,7 @@ __global__ __launch_bounds__(WALK_NT) void pbwt_walk_kernel(WalkArgs w) {
             }
         }
     }
+    if (STAGED) __syncthreads();   // tile ti + 1 is in place and tile ti is no longer read
     }   // tiles
     if (!act) return;
     w.ev_cnt[h] = nev;
@@ -688,7 +749,8 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
             return true;
         }();
         (void)attr;
-        hipLaunchKernelGGL(pbwt_walk_kernel<true>, dim3((R.n_haps + WALK_NT - 1) / WALK_NT), dim3(WALK_NT), WALK_LDS,
+        hipLaunchKernelGGL(pbwt_walk_kernel<true>, dim3((R.n_haps + WALK_WALKERS - 1) / WALK_WALKERS), dim3(WALK_NT),
+                           WALK_LDS,
                            s, w);
     } else {
         hipLaunchKernelGGL(pbwt_walk_kernel<false>, dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
