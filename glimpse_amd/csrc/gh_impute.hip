@@ -100,9 +100,16 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
 #pragma unroll
     for (int i = 0; i < MWL; i++) wid[i] = i < nI ? (nI - 1 - i) * NW + wv : 0;   // clamped: reads stay in the tile
     auto vlane = [&](int i) { return i < nI && 64 * wid[i] + lane < K; };
-    // the allele bits of word slot i at tile site j: the row word is the same in
-    // every lane, so it is read into SGPRs and used as the select's lane mask
-    auto rbit = [&](int j, int i) { return lane_bit(uniform_u64(rows[j * W + wid[i]])); };
+    // The row words of all word slots at tile site j are read together, before
+    // the slot loop and its wave-uniform skips, so their LDS latencies overlap
+    // (slots past nI read a clamped word of the tile and ignore it).  A row word
+    // is the same in every lane: it is moved to SGPRs and used as the select's
+    // lane mask.
+    auto rwords = [&](int j, uint64_t (&rw)[MWL]) {
+#pragma unroll
+        for (int i = 0; i < MWL; i++) rw[i] = rows[j * W + wid[i]];
+    };
+    auto rbit = [&](const uint64_t (&rw)[MWL], int i) { return lane_bit(uniform_u64(rw[i])); };
 
     // cross-wave sums (fixed order, so the forward and the recomputation agree bit for bit).
     // The LDS slot alternates with every reduction, so one barrier per reduction suffices.
@@ -218,11 +225,13 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
         const float e0 = in_vgpr(e0s), e1 = in_vgpr(e1s);
         const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;   // the partial word's emissions
         float s = 0.0f;
+        uint64_t rw[MWL];
+        rwords(j, rw);
         if (l == 0) {
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
                 if (i >= nI) continue;
-                const bool b = rbit(j, i);
+                const bool b = rbit(rw, i);
                 const float x = (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p)) * invK;
                 a[i] = x;
                 s += x;
@@ -233,7 +242,7 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
                 if (i >= nI) continue;
-                const bool b = rbit(j, i);
+                const bool b = rbit(rw, i);
                 const float x = fmaf(a[i], f2, f1) * (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p));
                 a[i] = x;
                 s += x;
@@ -344,12 +353,18 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
                 const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;
                 float p0 = 0.0f, p1 = 0.0f, bs_ = 0.0f;
                 float dpv[MWL];
+                // row words and recomputed alpha of every slot, read before the slot loop
+                uint64_t rw[MWL];
+                rwords(j, rw);
+                float avs[MWL];
+#pragma unroll
+                for (int i = 0; i < MWL; i++) avs[i] = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
                 if (l == Lp - 1) {
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
                         if (i >= nI) continue;
-                        const bool bit = rbit(j, i);
-                        const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
+                        const bool bit = rbit(rw, i);
+                        const float av = avs[i];
                         const float bv = (i > 0 ? (bit ? e1 : e0) : (bit ? e1p : e0p)) * invK;
                         b[i] = bv;
                         dpv[i] = av;
@@ -362,8 +377,8 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
                         if (i >= nI) continue;
-                        const bool bit = rbit(j, i);
-                        const float av = ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane];
+                        const bool bit = rbit(rw, i);
+                        const float av = avs[i];
                         const float bt = fmaf(b[i], f2, f1);   // lanes past K: b = 0 and alpha = 0
                         const float dp = av * bt;
                         dpv[i] = dp;
