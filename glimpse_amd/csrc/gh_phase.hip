@@ -252,9 +252,14 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     auto ui = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
     auto uf = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
     // allele of state kk[i] at tile locus j: its Hvar word (wid + NW*i) is the
-    // same for the whole wave and serves as the select's lane mask (only called
-    // for slots that hold states, so the word is inside the row)
-    auto bit_of = [&](int j, int i) -> bool { return lane_bit(uniform_u64(s_rows[j * W + wid + NW * i])); };
+    // same for the whole wave and serves as the select's lane mask.  The words
+    // of all slots are read together before a slot loop and its skips, so their
+    // LDS latencies overlap (slots past K read the row's last word and ignore it).
+    auto rwords = [&](int j, uint64_t (&rw)[KPT]) {
+#pragma unroll
+        for (int i = 0; i < KPT; i++) rw[i] = s_rows[j * W + min(wid + NW * i, W - 1)];
+    };
+    auto bit_of = [&](const uint64_t (&rw)[KPT], int i) -> bool { return lane_bit(uniform_u64(rw[i])); };
 
     // One locus update (INIT / RUN / COLLAPSE), tile offset j.  A state's
     // update is base(h) * e(h) with base = fma(src, NT, tFreq(h)) (src = its own
@@ -271,6 +276,8 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                       float acc[8]) {
         constexpr int ty = decltype(TYc)::value;    // 0..2: het bit position, 3: peak hom, 4: flat het
         constexpr bool run = decltype(RUNc)::value;
+        uint64_t rw[KPT];
+        rwords(j, rw);
 #pragma unroll
         for (int i = 0; i < KPT; i++) {
             if (PH_NT * i + 64 * wid >= K) {
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                 continue;
             }
             const bool vi = valid[i];
-            const bool ah = bit_of(j, i);
+            const bool ah = bit_of(rw, i);
             float eA = 1.0f, eB = 1.0f;
             if constexpr (ty <= 2) {
                 eA = vi ? (ah ? ed : ee) : 0.0f;
@@ -514,10 +521,12 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                 }
                 // slots past K are skipped; lanes past K hold 0 in both the
                 // forward values and the stored message, so they add exact zeros
+                uint64_t rw[KPT];
+                rwords(j, rw);
 #pragma unroll
                 for (int i = 0; i < KPT; i++) {
                     if (PH_NT * i + 64 * wid >= K) continue;
-                    const bool ah = bit_of(j, i);
+                    const bool ah = bit_of(rw, i);
                     const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
                     const float4 a = i4[0], b = i4[1];
                     const float iv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
