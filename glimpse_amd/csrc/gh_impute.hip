@@ -37,6 +37,9 @@ namespace gh {
 constexpr int FB_TILE = 32;
 constexpr int FB_LDS_BYTES = 156 * 1024;
 constexpr int FB_WG_PER_CU = 3;
+// LDS words before the Hvar tile: the row-word reads of word slots a wave does
+// not fill may point up to NW*MWL (<= 32) words below a row (see rwords)
+constexpr int FB_ROW_PAD = 32;
 
 struct FbArgs {
     const FbItem *fb;
@@ -63,15 +66,18 @@ __device__ __forceinline__ float rlf(float v, int j) {
 __device__ __forceinline__ int rli(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
 
 template <int NW, int MWL>
-__global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
+// three waves per SIMD (FB_WG_PER_CU workgroups of the 4-wave shapes) must stay
+// possible; the 16-word shapes (tuner candidates only) keep their registers
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 8 ? 3 : 1))) void fb_kernel(FbArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const FbItem f = wave_uniform(A.fb[A.order ? A.order[blockIdx.x] : (int)blockIdx.x]);
     const CondItem c = wave_uniform(A.cond[f.cond]);
     const int K = c.K, W = c.W, Lp = c.Lp, C = A.C;
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (Lp <= 0) return;
-    uint64_t *rows = (uint64_t *)smem;                                    // [FB_TILE][W]
-    float *ablk = (float *)(smem + (size_t)FB_TILE * A.maxW * 8);        // [C][NW*MWL][64]
+    static_assert(NW * MWL <= FB_ROW_PAD, "row pad covers every shape");
+    uint64_t *rows = (uint64_t *)smem + FB_ROW_PAD;                       // [FB_TILE][W]
+    float *ablk = (float *)(smem + (size_t)(FB_ROW_PAD + FB_TILE * A.maxW) * 8);   // [C][NW*MWL][64]
     float *red = ablk + (size_t)C * NW * MWL * 64;                        // [2][NW][4]
     const int32_t *poly = A.poly_all + c.site_off;
     const float *tr = A.t_all + c.site_off;
@@ -102,12 +108,15 @@ __global__ __launch_bounds__(64 * NW) void fb_kernel(FbArgs A) {
     auto vlane = [&](int i) { return i < nI && 64 * wid[i] + lane < K; };
     // The row words of all word slots at tile site j are read together, before
     // the slot loop and its wave-uniform skips, so their LDS latencies overlap
-    // (slots past nI read a clamped word of the tile and ignore it).  A row word
+    // (slots past nI read a word of the tile or of the pad below it and ignore it).  A row word
     // is the same in every lane: it is moved to SGPRs and used as the select's
     // lane mask.
+    // Slot i reads word (nI-1-i)*NW + wv of the row, i.e. rb[(MWL-1-i)*NW] from
+    // one base per site: constant offsets, one address computation per site.
     auto rwords = [&](int j, uint64_t (&rw)[MWL]) {
+        const uint64_t *rb = rows + j * W + (nI - MWL) * NW + wv;
 #pragma unroll
-        for (int i = 0; i < MWL; i++) rw[i] = rows[j * W + wid[i]];
+        for (int i = 0; i < MWL; i++) rw[i] = rb[(MWL - 1 - i) * NW];
     };
     auto rbit = [&](const uint64_t (&rw)[MWL], int i) { return lane_bit(uniform_u64(rw[i])); };
 
@@ -458,7 +467,7 @@ static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     FbShape s;
     s.NW = NW;
     s.MWL = MWL;
-    const size_t rows = (size_t)FB_TILE * maxW * 8;
+    const size_t rows = (size_t)(FB_ROW_PAD + FB_TILE * maxW) * 8;
     const size_t per_site = (size_t)s.NW * s.MWL * 64 * 4;
     const size_t red = 2 * 16 * 4 * 4;   // [2][NW <= 16][4] floats
     // The longest block (C divides FB_TILE) that leaves room for FB_WG_PER_CU

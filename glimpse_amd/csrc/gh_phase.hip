@@ -254,10 +254,13 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     // allele of state kk[i] at tile locus j: its Hvar word (wid + NW*i) is the
     // same for the whole wave and serves as the select's lane mask.  The words
     // of all slots are read together before a slot loop and its skips, so their
-    // LDS latencies overlap (slots past K read the row's last word and ignore it).
+    // LDS latencies overlap; one base per locus and constant slot offsets.  Slots
+    // past K read past the row's end (the next row, or the unused rest of the
+    // tile, which holds PH_TILE rows of NW*KPT >= W words) and ignore the word.
     auto rwords = [&](int j, uint64_t (&rw)[KPT]) {
+        const uint64_t *rb = s_rows + j * W + wid;
 #pragma unroll
-        for (int i = 0; i < KPT; i++) rw[i] = s_rows[j * W + min(wid + NW * i, W - 1)];
+        for (int i = 0; i < KPT; i++) rw[i] = rb[NW * i];
     };
     auto bit_of = [&](const uint64_t (&rw)[KPT], int i) -> bool { return lane_bit(uniform_u64(rw[i])); };
 
