@@ -196,7 +196,12 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
 #pragma unroll
     for (int i = 0; i < KPT; i++) { kk[i] = tid + PH_NT * i; valid[i] = kk[i] < K; }
 
+    // slots past K hold exact zeros throughout: set here, never written again
     float p[KPT][8];
+#pragma unroll
+    for (int i = 0; i < KPT; i++)
+#pragma unroll
+        for (int h = 0; h < 8; h++) p[i][h] = 0.0f;
     float sumK[KPT];
     float sumH[8];
     float sumT = 1.0f;
@@ -283,11 +288,7 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
         rwords(j, rw);
 #pragma unroll
         for (int i = 0; i < KPT; i++) {
-            if (PH_NT * i + 64 * wid >= K) {
-#pragma unroll
-                for (int h = 0; h < 8; h++) p[i][h] = 0.0f;
-                continue;
-            }
+            if (PH_NT * i + 64 * wid >= K) continue;   // p[i] stays 0
             const bool vi = valid[i];
             const bool ah = bit_of(rw, i);
             float eA = 1.0f, eB = 1.0f;
