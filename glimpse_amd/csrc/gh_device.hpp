@@ -25,18 +25,22 @@ __device__ __forceinline__ float partner32(float v) {
 }
 
 // Sum of v over the 64 lanes, returned wave-uniform.  Four DPP adds reduce each
-// 16-lane row (xor1, xor2, half-mirror, mirror), then the four row sums are
-// read with v_readlane.
+// 16-lane row (xor1, xor2, half-mirror, mirror), so every lane holds its row's
+// sum R0..R3; row_bcast:15 adds row 0 into row 1 and row 2 into row 3, then
+// row_bcast:31 adds row 1 into rows 2 and 3, and lane 63 holds
+// (R3 + R2) + (R1 + R0), the same value as (R0 + R1) + (R2 + R3).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dppf_rows(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
 __device__ __forceinline__ float wave_sum(float v) {
     v += dppf<0xB1>(v);   // quad_perm [1,0,3,2]
     v += dppf<0x4E>(v);   // quad_perm [2,3,0,1]
     v += dppf<0x141>(v);  // row_half_mirror
     v += dppf<0x140>(v);  // row_mirror
-    float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-    float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-    float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return (a + b) + (c + d);
+    v += dppf_rows<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v += dppf_rows<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 // Three independent sums at once (interleaved for ILP).
 __device__ __forceinline__ void wave_sum3(float &x, float &y, float &z) {
@@ -44,11 +48,11 @@ __device__ __forceinline__ void wave_sum3(float &x, float &y, float &z) {
     x += dppf<0x4E>(x); y += dppf<0x4E>(y); z += dppf<0x4E>(z);
     x += dppf<0x141>(x); y += dppf<0x141>(y); z += dppf<0x141>(z);
     x += dppf<0x140>(x); y += dppf<0x140>(y); z += dppf<0x140>(z);
-#define GH_RL(v, l) __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l))
-    x = (GH_RL(x, 0) + GH_RL(x, 16)) + (GH_RL(x, 32) + GH_RL(x, 48));
-    y = (GH_RL(y, 0) + GH_RL(y, 16)) + (GH_RL(y, 32) + GH_RL(y, 48));
-    z = (GH_RL(z, 0) + GH_RL(z, 16)) + (GH_RL(z, 32) + GH_RL(z, 48));
-#undef GH_RL
+    x += dppf_rows<0x142, 0xA>(x); y += dppf_rows<0x142, 0xA>(y); z += dppf_rows<0x142, 0xA>(z);
+    x += dppf_rows<0x143, 0xC>(x); y += dppf_rows<0x143, 0xC>(y); z += dppf_rows<0x143, 0xC>(z);
+    x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+    y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y), 63));
+    z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(z), 63));
 }
 
 // Reduce-scatter of 8 per-lane values over the wave: returns, in every lane l,
