@@ -15,6 +15,9 @@ CASES = [
     (2500, 3000, 0.5, 0, 0.1, 4),
     (2500, 2500, 0.6, 6, 0.1, 3),
     (3000, 4000, 0.3, 0, 0.02, 5),
+    # 80 target haplotypes: two walk workgroups of 64 walker lanes, the second
+    # one partly idle (its loaders still stage alleles for the idle lanes)
+    (1500, 3000, 0.4, 0, 0.05, 40),
 ]
 
 
