@@ -29,6 +29,7 @@
 #include "gh_internal.hpp"
 
 #include <cstdio>
+#include <type_traits>
 #include <cstdlib>
 #include <vector>
 
@@ -222,6 +223,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
         commit_tile();
     };
 
+    // A slot loop runs as straight-line code for a wave whose slots all hold
+    // words (nI == MWL: every wave of the launch's largest items, whose chains
+    // are the longest), so the compiler can interleave the slots' instruction
+    // chains; other waves skip their empty slots one by one.
+    auto by_fill = [&](auto &&body) {
+        if (nI == MWL) body(std::true_type{});
+        else body(std::false_type{});
+    };
     float a[MWL];
 #pragma unroll
     for (int i = 0; i < MWL; i++) a[i] = 0.0f;
@@ -248,14 +257,17 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
             sp = fl ? 1.0f : xsum1(s);
         } else {
             const float f1 = rlf(m_f1in, j), f2 = rlf(m_ntin, j) * __builtin_amdgcn_rcpf(sp);
+            by_fill([&](auto FULLc) {
+                constexpr bool full = decltype(FULLc)::value;
 #pragma unroll
-            for (int i = 0; i < MWL; i++) {
-                if (i >= nI) continue;
-                const bool b = rbit(rw, i);
-                const float x = fmaf(a[i], f2, f1) * (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p));
-                a[i] = x;
-                s += x;
-            }
+                for (int i = 0; i < MWL; i++) {
+                    if (!full && i >= nI) continue;
+                    const bool b = rbit(rw, i);
+                    const float x = fmaf(a[i], f2, f1) * (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p));
+                    a[i] = x;
+                    s += x;
+                }
+            });
             sp = xsum1(s);
         }
     };
@@ -345,11 +357,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
             }
             for (int l = bs + 1; l < be; l++) {
                 fwd_site(l - l0, l, sp);
+                by_fill([&](auto FULLc) {
+                    constexpr bool full = decltype(FULLc)::value;
 #pragma unroll
-                for (int i = 0; i < MWL; i++) {
-                    if (i >= nI) continue;
-                    ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
-                }
+                    for (int i = 0; i < MWL; i++) {
+                        if (!full && i >= nI) continue;
+                        ablk[((size_t)(l - bs) * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
+                    }
+                });
             }
             if (more && q == (n - 1) / C) prefetch2();   // the next tile's site ids have arrived by now
             // backward recursion on the block
@@ -383,19 +398,22 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                     }
                 } else {
                     const float f1 = rlf(m_f1out, j), f2 = rlf(m_ntout, j) * __builtin_amdgcn_rcpf(bsum_next);
+                    by_fill([&](auto FULLc) {
+                        constexpr bool full = decltype(FULLc)::value;
 #pragma unroll
-                    for (int i = 0; i < MWL; i++) {
-                        if (i >= nI) continue;
-                        const bool bit = rbit(rw, i);
-                        const float av = avs[i];
-                        const float bt = fmaf(b[i], f2, f1);   // lanes past K: b = 0 and alpha = 0
-                        const float dp = av * bt;
-                        dpv[i] = dp;
-                        p1 += bit ? dp : 0.0f;
-                        p0 += bit ? 0.0f : dp;
-                        b[i] = bt * (i > 0 ? (bit ? e1 : e0) : (bit ? e1p : e0p));
-                        bs_ += b[i];
-                    }
+                        for (int i = 0; i < MWL; i++) {
+                            if (!full && i >= nI) continue;
+                            const bool bit = rbit(rw, i);
+                            const float av = avs[i];
+                            const float bt = fmaf(b[i], f2, f1);   // lanes past K: b = 0 and alpha = 0
+                            const float dp = av * bt;
+                            dpv[i] = dp;
+                            p1 += bit ? dp : 0.0f;
+                            p0 += bit ? 0.0f : dp;
+                            b[i] = bt * (i > 0 ? (bit ? e1 : e0) : (bit ? e1p : e0p));
+                            bs_ += b[i];
+                        }
+                    });
                 }
                 xsum3(p0, p1, bs_);
                 if (fl && l == Lp - 1) bs_ = 1.0f;
