@@ -109,17 +109,24 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     auto vlane = [&](int i) { return i < nI && 64 * wid[i] + lane < K; };
     // The row words of all word slots at tile site j are read together, before
     // the slot loop and its wave-uniform skips, so their LDS latencies overlap
-    // (slots past nI read a word of the tile or of the pad below it and ignore it).  A row word
-    // is the same in every lane: it is moved to SGPRs and used as the select's
-    // lane mask.
+    // (slots past nI read a word of the tile or of the pad below it and ignore it).
     // Slot i reads word (nI-1-i)*NW + wv of the row, i.e. rb[(MWL-1-i)*NW] from
     // one base per site: constant offsets, one address computation per site.
-    auto rwords = [&](int j, uint64_t (&rw)[MWL]) {
-        const uint64_t *rb = rows + j * W + (nI - MWL) * NW + wv;
+    // Each lane reads the 32-bit half of the word that holds its own state's
+    // bit and widens the bit to an all-ones / all-zeros mask (v_bfe_i32); the
+    // selects are then bitwise (v_bfi_b32, v_and_b32), with no trip through
+    // scalar registers.
+    const unsigned lane_sh = lane & 31;
+    auto rwords = [&](int j, uint32_t (&rw)[MWL]) {
+        const uint32_t *rb = (const uint32_t *)(rows + j * W + (nI - MWL) * NW + wv) + (lane >> 5);
 #pragma unroll
-        for (int i = 0; i < MWL; i++) rw[i] = rb[(MWL - 1 - i) * NW];
+        for (int i = 0; i < MWL; i++) rw[i] = rb[2 * (MWL - 1 - i) * NW];
     };
-    auto rbit = [&](const uint64_t (&rw)[MWL], int i) { return lane_bit(uniform_u64(rw[i])); };
+    auto rmask = [&](const uint32_t (&rw)[MWL], int i) { return __builtin_amdgcn_sbfe((int)rw[i], lane_sh, 1u); };
+    auto sel = [](int m, float v1, float v0) {
+        return __int_as_float((m & __float_as_int(v1)) | (~m & __float_as_int(v0)));
+    };
+    auto keep = [](int m, float v) { return __int_as_float(m & __float_as_int(v)); };
 
     // cross-wave sums (fixed order, so the forward and the recomputation agree bit for bit).
     // The LDS slot alternates with every reduction, so one barrier per reduction suffices.
@@ -243,14 +250,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
         const float e0 = in_vgpr(e0s), e1 = in_vgpr(e1s);
         const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;   // the partial word's emissions
         float s = 0.0f;
-        uint64_t rw[MWL];
+        uint32_t rw[MWL];
         rwords(j, rw);
         if (l == 0) {
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
                 if (i >= nI) continue;
-                const bool b = rbit(rw, i);
-                const float x = (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p)) * invK;
+                const int m = rmask(rw, i);
+                const float x = (i > 0 ? sel(m, e1, e0) : sel(m, e1p, e0p)) * invK;
                 a[i] = x;
                 s += x;
             }
@@ -262,8 +269,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
 #pragma unroll
                 for (int i = 0; i < MWL; i++) {
                     if (!full && i >= nI) continue;
-                    const bool b = rbit(rw, i);
-                    const float x = fmaf(a[i], f2, f1) * (i > 0 ? (b ? e1 : e0) : (b ? e1p : e0p));
+                    const int m = rmask(rw, i);
+                    const float x = fmaf(a[i], f2, f1) * (i > 0 ? sel(m, e1, e0) : sel(m, e1p, e0p));
                     a[i] = x;
                     s += x;
                 }
@@ -378,7 +385,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                 float p0 = 0.0f, p1 = 0.0f, bs_ = 0.0f;
                 float dpv[MWL];
                 // row words and recomputed alpha of every slot, read before the slot loop
-                uint64_t rw[MWL];
+                uint32_t rw[MWL];
                 rwords(j, rw);
                 float avs[MWL];
 #pragma unroll
@@ -387,13 +394,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
                         if (i >= nI) continue;
-                        const bool bit = rbit(rw, i);
+                        const int m = rmask(rw, i);
                         const float av = avs[i];
-                        const float bv = (i > 0 ? (bit ? e1 : e0) : (bit ? e1p : e0p)) * invK;
+                        const float bv = (i > 0 ? sel(m, e1, e0) : sel(m, e1p, e0p)) * invK;
                         b[i] = bv;
                         dpv[i] = av;
-                        p1 += bit ? av : 0.0f;
-                        p0 += bit ? 0.0f : av;
+                        p1 += keep(m, av);
+                        p0 += keep(~m, av);
                         bs_ += b[i];
                     }
                 } else {
@@ -403,14 +410,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
 #pragma unroll
                         for (int i = 0; i < MWL; i++) {
                             if (!full && i >= nI) continue;
-                            const bool bit = rbit(rw, i);
+                            const int m = rmask(rw, i);
                             const float av = avs[i];
                             const float bt = fmaf(b[i], f2, f1);   // lanes past K: b = 0 and alpha = 0
                             const float dp = av * bt;
                             dpv[i] = dp;
-                            p1 += bit ? dp : 0.0f;
-                            p0 += bit ? 0.0f : dp;
-                            b[i] = bt * (i > 0 ? (bit ? e1 : e0) : (bit ? e1p : e0p));
+                            p1 += keep(m, dp);
+                            p0 += keep(~m, dp);
+                            b[i] = bt * (i > 0 ? sel(m, e1, e0) : sel(m, e1p, e0p));
                             bs_ += b[i];
                         }
                     });
