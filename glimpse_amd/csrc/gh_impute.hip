@@ -6,11 +6,11 @@
 // row word gives every lane its state's allele as bit j.
 //
 // Latency: the per-site metadata (site id, flat/LQ flags, emissions,
-// transitions, likelihoods) of a 64-site tile is gathered once, one site per
-// lane, and broadcast with v_readlane inside the serial site loop; the tile's
-// Hvar rows are staged in LDS.  Per-site scaling sums use DPP row reductions
-// and v_readlane, then one LDS exchange (double-buffered by site parity, one
-// barrier) across the NW waves.
+// transitions, likelihoods) of a 32-site tile is gathered once, one site per
+// lane, and stored in LDS as three 16-byte records per site, which the serial
+// site loop reads with broadcast loads; the tile's Hvar rows are staged in LDS
+// too.  Per-site scaling sums use DPP row reductions, then one LDS exchange
+// (double-buffered by site parity, one barrier) across the NW waves.
 //
 // Memory: alpha never round-trips through HBM.  The forward pass keeps alpha
 // in registers and writes a checkpoint (the full alpha vector) every C sites;
@@ -61,10 +61,6 @@ struct FbArgs {
     int maxW;          // LDS row stride bound
 };
 
-__device__ __forceinline__ float rlf(float v, int j) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-__device__ __forceinline__ int rli(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
 
 template <int NW, int MWL>
 // three waves per SIMD (FB_WG_PER_CU workgroups of the 4-wave shapes) must stay
@@ -80,6 +76,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     uint64_t *rows = (uint64_t *)smem + FB_ROW_PAD;                       // [FB_TILE][W]
     float *ablk = (float *)(smem + (size_t)(FB_ROW_PAD + FB_TILE * A.maxW) * 8);   // [C][NW*MWL][64]
     float *red = ablk + (size_t)C * NW * MWL * 64;                        // [2][NW][4]
+    float4 *smeta = (float4 *)(red + 2 * 16 * 4);                         // [FB_TILE][3] site records
     const int32_t *poly = A.poly_all + c.site_off;
     const float *tr = A.t_all + c.site_off;
     const uint64_t *hv = A.hvar_all + c.hvar_off;
@@ -161,10 +158,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
         }
     };
 
-    // tile metadata: lane j of every wave describes site l0 + j
-    int m_site = 0, m_code = 0;
-    float m_e0 = 1.0f, m_e1 = 1.0f, m_h0 = 0.5f, m_h1 = 0.5f;
-    float m_f1in = 0.0f, m_ntin = 1.0f, m_f1out = 0.0f, m_ntout = 1.0f;   // t/K and 1-t into / out of the site
+    // Tile metadata: three 16-byte records per site in LDS, read with one
+    // broadcast load each (every lane the same address):
+    //   [0] emissions of allele 0 / 1 (1 at flat and LQ sites), t/K and 1-t into the site
+    //   [1] t/K and 1-t out of the site, the haplotype likelihoods h0 / h1
+    //   [2] site index, flags (bit 0 flat or LQ, bit 1 flat), unused, unused
     // The next tile is fetched into registers while the current one is processed:
     // stage 1 (site ids, transitions, Hvar rows) right after a tile starts,
     // stage 2 (flags and likelihoods of those sites) once the ids have arrived.
@@ -199,23 +197,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     };
     // make the fetched tile current (every thread calls it)
     auto commit_tile = [&]() {
-        __syncthreads();   // previous tile's LDS rows are no longer read
-        if (lane < pf_n) {
+        __syncthreads();   // previous tile's LDS rows and records are no longer read
+        if (wv == 0 && lane < pf_n) {
             const int site = pf_site;
             const bool isflat = pf_flat != 0;
             const bool fl = isflat || (pf_flag & SF_LQ);
             const float h0 = pf_h0, h1 = pf_h1;
             const float p0 = h0 * ee + h1 * ed, p1 = h0 * ed + h1 * ee;
-            m_site = site;
-            m_code = (fl ? 1 : 0) | (isflat ? 2 : 0);
-            m_e0 = p0 / (p0 + p1);
-            m_e1 = p1 / (p0 + p1);
-            m_h0 = h0;
-            m_h1 = h1;
-            m_f1in = pf_tin / Kf;
-            m_ntin = 1.0f - pf_tin;
-            m_f1out = pf_tout / Kf;
-            m_ntout = 1.0f - pf_tout;
+            const float e0 = fl ? 1.0f : p0 / (p0 + p1), e1 = fl ? 1.0f : p1 / (p0 + p1);
+            smeta[3 * lane + 0] = make_float4(e0, e1, pf_tin / Kf, 1.0f - pf_tin);
+            smeta[3 * lane + 1] = make_float4(pf_tout / Kf, 1.0f - pf_tout, h0, h1);
+            smeta[3 * lane + 2] = make_float4(__int_as_float(site), __int_as_float((fl ? 1 : 0) | (isflat ? 2 : 0)),
+                                              0.0f, 0.0f);
         }
 #pragma unroll
         for (int k = 0; k < MWL; k++) {
@@ -244,14 +237,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     // forward step at site l (tile offset j): alpha in a[], scaling sum in sp
     // (flat and LQ sites have emission 1 for both alleles)
     auto fwd_site = [&](int j, int l, float &sp) {
-        const int code = rli(m_code, j);
-        const bool fl = code & 1;
-        const float e0s = fl ? 1.0f : rlf(m_e0, j), e1s = fl ? 1.0f : rlf(m_e1, j);
-        const float e0 = in_vgpr(e0s), e1 = in_vgpr(e1s);
-        const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;   // the partial word's emissions
-        float s = 0.0f;
+        const float4 mf = smeta[3 * j];
         uint32_t rw[MWL];
         rwords(j, rw);
+        const float e0 = mf.x, e1 = mf.y;
+        const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;   // the partial word's emissions
+        float s = 0.0f;
         if (l == 0) {
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
@@ -261,9 +252,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                 a[i] = x;
                 s += x;
             }
+            const bool fl = __builtin_amdgcn_readfirstlane(__float_as_int(smeta[3 * j + 2].y)) & 1;
             sp = fl ? 1.0f : xsum1(s);
         } else {
-            const float f1 = rlf(m_f1in, j), f2 = rlf(m_ntin, j) * __builtin_amdgcn_rcpf(sp);
+            const float f1 = mf.z, f2 = mf.w * __builtin_amdgcn_rcpf(sp);
             by_fill([&](auto FULLc) {
                 constexpr bool full = decltype(FULLc)::value;
 #pragma unroll
@@ -348,7 +340,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
             if (bs > 0) fetch_ck(bs - C);
             float sp;
             {
-                const bool fl0 = rli(m_code, bs - l0) & 1;
+                const bool fl0 = __builtin_amdgcn_readfirstlane(__float_as_int(smeta[3 * (bs - l0) + 2].y)) & 1;
                 float s = 0.0f;
 #pragma unroll
                 for (int i = 0; i < MWL; i++) {
@@ -377,11 +369,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
             // backward recursion on the block
             for (int l = be - 1; l >= bs; l--) {
                 const int j = l - l0;
-                const int code = rli(m_code, j);
-                const bool fl = code & 1, isflat = code & 2;
-                const float e0s = fl ? 1.0f : rlf(m_e0, j), e1s = fl ? 1.0f : rlf(m_e1, j);
-                const float e0 = in_vgpr(e0s), e1 = in_vgpr(e1s);
+                const float4 mf = smeta[3 * j], mb = smeta[3 * j + 1], mi = smeta[3 * j + 2];
+                const float e0 = mf.x, e1 = mf.y;   // 1 at flat and LQ sites
                 const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;
+                const int code = __builtin_amdgcn_readfirstlane(__float_as_int(mi.y));
+                const bool fl = code & 1, isflat = code & 2;
+                const int site = __float_as_int(mi.x);
                 float p0 = 0.0f, p1 = 0.0f, bs_ = 0.0f;
                 float dpv[MWL];
                 // row words and recomputed alpha of every slot, read before the slot loop
@@ -404,7 +397,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                         bs_ += b[i];
                     }
                 } else {
-                    const float f1 = rlf(m_f1out, j), f2 = rlf(m_ntout, j) * __builtin_amdgcn_rcpf(bsum_next);
+                    const float f1 = mb.x, f2 = mb.y * __builtin_amdgcn_rcpf(bsum_next);
                     by_fill([&](auto FULLc) {
                         constexpr bool full = decltype(FULLc)::value;
 #pragma unroll
@@ -429,16 +422,15 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
                         if (i >= nI) continue;
-                        const int site = rli(m_site, j);
                         const bool in = site >= wlo[i] && site < whi[i];
                         ms[i] = in ? fmaf(dpv[i], inv, ms[i]) : ms[i];
                     }
                 }
-                if (!fl) { p0 /= e0s; p1 /= e1s; }
+                p0 /= e0;   // exact at flat and LQ sites, where the emissions are 1
+                p1 /= e1;
                 float o0 = p0 * ee + p1 * ed, o1 = p0 * ed + p1 * ee;
-                if (!isflat) { o0 *= rlf(m_h0, j); o1 *= rlf(m_h1, j); }
+                if (!isflat) { o0 *= mb.z; o1 *= mb.w; }
                 if (tid == 0) {
-                    const int site = rli(m_site, j);
                     HP[2 * site] = o0 / (o0 + o1);
                     HP[2 * site + 1] = o1 / (o0 + o1);
                 }
@@ -494,7 +486,7 @@ static FbShape fb_shape_of(int maxW, int NW, int MWL) {
     s.MWL = MWL;
     const size_t rows = (size_t)(FB_ROW_PAD + FB_TILE * maxW) * 8;
     const size_t per_site = (size_t)s.NW * s.MWL * 64 * 4;
-    const size_t red = 2 * 16 * 4 * 4;   // [2][NW <= 16][4] floats
+    const size_t red = 2 * 16 * 4 * 4 + FB_TILE * 3 * 16;   // [2][NW <= 16][4] floats, then the site records
     // The longest block (C divides FB_TILE) that leaves room for FB_WG_PER_CU
     // workgroups per CU (what the register budget of the 4-wave shapes allows),
     // unless that would make blocks shorter than 4 sites; else the longest that fits.
