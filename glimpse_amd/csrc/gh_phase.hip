@@ -151,7 +151,8 @@ __device__ __forceinline__ float transition(const double *cm, int prev, int next
 constexpr int PH_TILE = 64;
 
 template <int NW, int KPT>
-__global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict__ items, const CondItem *__restrict__ cond,
+// the two-slot shapes keep four waves per SIMD (128 VGPRs)
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(KPT == 2 ? 4 : 1))) void phase_kernel(const PhItem *__restrict__ items, const CondItem *__restrict__ cond,
                                                        const uint64_t *__restrict__ hvar_all,
                                                        const Locus *__restrict__ loci_all,
                                                        const int32_t *__restrict__ pos_all,
@@ -256,18 +257,25 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
     // per-locus metadata is the same for every lane: read it into SGPRs
     auto ui = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
     auto uf = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
-    // allele of state kk[i] at tile locus j: its Hvar word (wid + NW*i) is the
-    // same for the whole wave and serves as the select's lane mask.  The words
-    // of all slots are read together before a slot loop and its skips, so their
-    // LDS latencies overlap; one base per locus and constant slot offsets.  Slots
-    // past K read past the row's end (the next row, or the unused rest of the
-    // tile, which holds PH_TILE rows of NW*KPT >= W words) and ignore the word.
-    auto rwords = [&](int j, uint64_t (&rw)[KPT]) {
-        const uint64_t *rb = s_rows + j * W + wid;
+    // allele of state kk[i] at tile locus j: bit (lane & 63) of its Hvar word
+    // (wid + NW*i).  Each lane reads the 32-bit half that holds its bit and
+    // widens the bit to an all-ones / all-zeros mask (v_bfe_i32); the selects
+    // are then bitwise.  The words of all slots are read together before a slot
+    // loop and its skips, so their LDS latencies overlap; one base per locus and
+    // constant slot offsets.  Slots past K read past the row's end (the next
+    // row, or the unused rest of the tile, which holds PH_TILE rows of
+    // NW*KPT >= W words) and ignore the word.
+    const unsigned lane_sh = lane & 31;
+    auto rwords = [&](int j, uint32_t (&rw)[KPT]) {
+        const uint32_t *rb = (const uint32_t *)(s_rows + j * W + wid) + (lane >> 5);
 #pragma unroll
-        for (int i = 0; i < KPT; i++) rw[i] = rb[NW * i];
+        for (int i = 0; i < KPT; i++) rw[i] = rb[2 * NW * i];
     };
-    auto bit_of = [&](const uint64_t (&rw)[KPT], int i) -> bool { return lane_bit(uniform_u64(rw[i])); };
+    auto bit_of = [&](const uint32_t (&rw)[KPT], int i) { return __builtin_amdgcn_sbfe((int)rw[i], lane_sh, 1u); };
+    auto sel = [](int m, float v1, float v0) {
+        return __int_as_float((m & __float_as_int(v1)) | (~m & __float_as_int(v0)));
+    };
+    auto keep = [](int m, float v) { return __int_as_float(m & __float_as_int(v)); };
 
     // One locus update (INIT / RUN / COLLAPSE), tile offset j.  A state's
     // update is base(h) * e(h) with base = fma(src, NT, tFreq(h)) (src = its own
@@ -284,19 +292,19 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                       float acc[8]) {
         constexpr int ty = decltype(TYc)::value;    // 0..2: het bit position, 3: peak hom, 4: flat het
         constexpr bool run = decltype(RUNc)::value;
-        uint64_t rw[KPT];
+        uint32_t rw[KPT];
         rwords(j, rw);
 #pragma unroll
         for (int i = 0; i < KPT; i++) {
             if (PH_NT * i + 64 * wid >= K) continue;   // p[i] stays 0
             const bool vi = valid[i];
-            const bool ah = bit_of(rw, i);
+            const int ah = bit_of(rw, i);
             float eA = 1.0f, eB = 1.0f;
             if constexpr (ty <= 2) {
-                eA = vi ? (ah ? ed : ee) : 0.0f;
-                eB = vi ? (ah ? ee : ed) : 0.0f;
+                eA = vi ? sel(ah, ed, ee) : 0.0f;
+                eB = vi ? sel(ah, ee, ed) : 0.0f;
             } else if constexpr (ty == 3) {
-                eA = vi ? (ah ? eh1 : eh0) : 0.0f;
+                eA = vi ? sel(ah, eh1, eh0) : 0.0f;
             } else {
                 eA = vi ? 1.0f : 0.0f;
             }
@@ -525,20 +533,20 @@ __global__ __launch_bounds__(64 * NW) void phase_kernel(const PhItem *__restrict
                 }
                 // slots past K are skipped; lanes past K hold 0 in both the
                 // forward values and the stored message, so they add exact zeros
-                uint64_t rw[KPT];
+                uint32_t rw[KPT];
                 rwords(j, rw);
 #pragma unroll
                 for (int i = 0; i < KPT; i++) {
                     if (PH_NT * i + 64 * wid >= K) continue;
-                    const bool ah = bit_of(rw, i);
+                    const int ah = bit_of(rw, i);
                     const float4 *i4 = (const float4 *)(ip + ((size_t)i * PH_NT + tid) * 8);
                     const float4 a = i4[0], b = i4[1];
                     const float iv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
                     for (int h = 0; h < 8; h++) {
                         const float v = (iv[h] * sR[h]) * (p[i][h] * sL[h]);
-                        s1[h] += ah ? v : 0.0f;
-                        s0[h] += ah ? 0.0f : v;
+                        s1[h] += keep(ah, v);
+                        s0[h] += keep(~ah, v);
                     }
                 }
                 bsum8(s0);
