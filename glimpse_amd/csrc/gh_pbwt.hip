@@ -12,12 +12,15 @@
 //                         resident in HBM; the per-iteration sweep then needs no
 //                         decode at all.
 // Per Gibbs iteration (device):
-//   pbwt_walk_kernel   -- one thread per target haplotype replays
+//   pbwt_walk_kernel   -- one lane per target haplotype replays
 //                         haplotype_set::matchHapsFromCompressedPBWTSmall
 //                         (phase/src/containers/haplotype_set.cpp:317-728) for that
 //                         haplotype alone: LF-mapping of the insertion index and
 //                         match interval, resets, rare-run re-indexing, and selectK
-//                         events recorded as (site, phase) keyed records.
+//                         events recorded as (site, phase) keyed records.  A
+//                         workgroup is one walker wave and three loader waves that
+//                         stage the next tile of rank data, site ops and alleles
+//                         into the second of two LDS buffers.
 //   pbwt_compact_kernel -- one wave per target merges its two haplotypes' records
 //                         newest first, rank by rank, into the first Kpbwt distinct
 //                         panel haplotypes (conditioning_set.cpp:96-104), then
