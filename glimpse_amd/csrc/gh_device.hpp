@@ -75,29 +75,6 @@ __device__ __forceinline__ float wave_rs8(const float v[8]) {
     return x;
 }
 
-// The same for 64 values: returns in lane l the wave total of value
-// rs64_index(l) = 32*b2 + 16*b1 + 8*b0 + 4*b3 + 2*b4 + b5 (bX = bit X of l).
-__device__ __forceinline__ int rs64_index(int l) {
-    return 32 * ((l >> 2) & 1) + 16 * ((l >> 1) & 1) + 8 * (l & 1) + 4 * ((l >> 3) & 1) + 2 * ((l >> 4) & 1) +
-           ((l >> 5) & 1);
-}
-__device__ __forceinline__ float wave_rs64(const float v[64]) {
-    const int l = threadIdx.x & 63;
-    const bool b2 = l & 4, b1 = l & 2, b0 = l & 1, b3 = l & 8, b4 = l & 16, b5 = l & 32;
-    float a[32], b[16], c[8], d[4], e[2];
-#pragma unroll
-    for (int k = 0; k < 32; k++) a[k] = (b2 ? v[32 + k] : v[k]) + dppf<0x141>(b2 ? v[k] : v[32 + k]);
-#pragma unroll
-    for (int k = 0; k < 16; k++) b[k] = (b1 ? a[16 + k] : a[k]) + dppf<0x4E>(b1 ? a[k] : a[16 + k]);
-#pragma unroll
-    for (int k = 0; k < 8; k++) c[k] = (b0 ? b[8 + k] : b[k]) + dppf<0xB1>(b0 ? b[k] : b[8 + k]);
-#pragma unroll
-    for (int k = 0; k < 4; k++) d[k] = (b3 ? c[4 + k] : c[k]) + dppf<0x128>(b3 ? c[k] : c[4 + k]);
-#pragma unroll
-    for (int k = 0; k < 2; k++) e[k] = (b4 ? d[2 + k] : d[k]) + partner16(b4 ? d[k] : d[2 + k]);
-    return (b5 ? e[1] : e[0]) + partner32(b5 ? e[0] : e[1]);
-}
-
 // The same for 32 values: five halving steps (lane bits 2, 1, 0, 3, 4), then the
 // two wave halves are added, so lanes l and l ^ 32 both hold the wave total of
 // value rs32_index(l) = 16*b2 + 8*b1 + 4*b0 + 2*b3 + b4.
@@ -132,36 +109,6 @@ __device__ __forceinline__ T wave_uniform(const T &v) {
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(T) / 4); k++) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
     return r;
-}
-
-// A 64-bit value that is the same in every lane, moved to SGPRs.
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-// Bit (lane) of a wave-uniform 64-bit mask: the mask itself becomes the lane
-// mask of the select that consumes it (no per-lane shift or compare).
-__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
-
-// A copy of v held in a VGPR (v_mov_b32).  Wave-uniform operands of
-// per-lane selects otherwise get copied from SGPRs at every use.
-__device__ __forceinline__ float in_vgpr(float v) {
-    float r;
-    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
-    return r;
-}
-__device__ __forceinline__ uint32_t in_vgpr_u32(uint32_t v) {
-    uint32_t r;
-    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
-    return r;
-}
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-template <typename T>
-__device__ __forceinline__ T uniform(T v) {
-    return (T)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // ---------------------------------------------------------------- counter RNG
