@@ -239,9 +239,14 @@ void pbwt_decode(gh_panel *P) {
         SiteOp op;
         std::memset(&op, 0, sizeof(op));
         op.k = k;
+        op.cm_k = P->cm_pos[k];
+        op.cm_km1 = P->cm_pos[k > 0 ? k - 1 : 0];
+        op.cm_ps = P->cm_pos[0];
         if (fl & SF_COMMON) {
             op.kind = 0;
             op.l = l_hq;
+            op.ps = l_hq >= 1 ? P->common2tot[l_hq - 1] : 0;
+            op.cm_ps = P->cm_pos[op.ps];
             const int ns = P->asmall_ptr[l_hq + 1] - P->asmall_ptr[l_hq];
             const int prev_out = l_hq > 0 ? l_hq - 1 : -1;
             if (prev_out < 0 && iota_arr < 0) {
@@ -339,8 +344,7 @@ struct WalkArgs {
     const int32_t *A;
     int N;
     const int32_t *aptr, *aidx;
-    const float *cm_pos;
-    const int32_t *c2t;
+    float cm0;             // cM of site 0 (the walk's first reset position)
     const uint8_t *haps;   // rows of L
     const int32_t *hap_row;
     int L;
@@ -405,6 +409,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
     int32_t *ev = w.ev + (size_t)h * w.cap * rec;
     int nev = 0;
     int idx = w.start[h], f = 0, g = N, last_reset = 0, last_rare = 0, tar = 0;
+    float cm_last = w.cm0;   // cM of site last_reset
     int sidx = 0, sf = 0, sg = 0;
     bool in_rare = false;
     bool overflow = false;
@@ -537,16 +542,18 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                         selectK(2 * k, op.prev_ref, Aprev, tar);
                         fd = 0; gd = N;
                         last_reset = k - 1;
+                        cm_last = op.cm_km1;
                     }
                 } else {
                     int c0, c1, c2;
                     lower_count3(si, ns, f, g, idx, c0, c1, c2);
                     id = idx - c2; fd = f - c0; gd = g - c1;
                     if (gd <= fd) {
-                        const int ps = op.l >= 1 ? w.c2t[op.l - 1] : 0;
-                        if (w.cm_pos[ps] - w.cm_pos[last_reset] > modulo / 5.0f) selectK(2 * k, op.prev_ref, Aprev, tar);
+                        const int ps = op.ps;
+                        if (op.cm_ps - cm_last > modulo / 5.0f) selectK(2 * k, op.prev_ref, Aprev, tar);
                         fd = 0; gd = N - ns;
                         last_reset = ps;
+                        cm_last = op.cm_ps;
                     }
                 }
                 idx = id; f = fd; g = gd;
@@ -566,11 +573,11 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
             bool reset = false;
             const bool st = STAGED ? ((sa >> 1) & 1) != 0 : stored[op.l] != 0;
             if (gd <= fd) {
-                if (st || w.cm_pos[k > 0 ? k - 1 : 0] - w.cm_pos[last_reset] > modulo / 2.0f)
+                if (st || op.cm_km1 - cm_last > modulo / 2.0f)
                     selectK(2 * k + 1, op.prev_ref, w.A + (size_t)op.a_in * N, prev);
                 fd = a * ref;
                 gd = a * N + (1 - a) * ref;
-                last_reset = k;
+                last_reset = k; cm_last = op.cm_k;
                 reset = true;
             }
             idx = id; f = fd; g = gd;
@@ -578,7 +585,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                 selectK(2 * k + 1, ref, w.A + (size_t)op.a_out * N, a);
                 f = a * ref;
                 g = a * N + (1 - a) * ref;
-                last_reset = k;
+                last_reset = k; cm_last = op.cm_k;
             }
         } else {
             const int ns = op.nbits;
@@ -607,7 +614,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                     const int32_t *si = w.aidx + w.aptr[op.l];
                     int fd, gd, c2;
                     lower_count3(si, ns, f, g, idx, fd, gd, c2);
-                    if (gd <= fd) { fd = 0; gd = ns; last_reset = k; }
+                    if (gd <= fd) { fd = 0; gd = ns; last_reset = k; cm_last = op.cm_k; }
                     sidx = c2; sf = fd; sg = gd;
                 }
             }
@@ -621,10 +628,10 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                 if (a_here != op.major) {
                     last_rare = k;
                     si = ref + vi; fd = ref + vf; gd = ref + vg;
-                    if (gd <= fd) { fd = ref; gd = ns; last_reset = k; }
+                    if (gd <= fd) { fd = ref; gd = ns; last_reset = k; cm_last = op.cm_k; }
                 } else {
                     si = sidx - vi; fd = sf - vf; gd = sg - vg;
-                    if (gd <= fd) { fd = 0; gd = ref; last_reset = k; }
+                    if (gd <= fd) { fd = 0; gd = ref; last_reset = k; cm_last = op.cm_k; }
                 }
                 sidx = si; sf = fd; sg = gd;
             }
@@ -740,7 +747,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     w.bits = D->bits.as<uint64_t>(); w.cnt = D->cnt.as<uint32_t>();
     w.A = D->A.as<int32_t>(); w.N = P->N;
     w.aptr = D->aptr.as<int32_t>(); w.aidx = D->aidx.as<int32_t>();
-    w.cm_pos = P->d_cmpos.as<float>(); w.c2t = P->d_c2t.as<int32_t>();
+    w.cm0 = P->cm_pos[0];
     w.haps = R.haps; w.hap_row = R.hap_row; w.L = P->L; w.n_haps = R.n_haps; w.start = R.start;
     w.stored = R.stored; w.hap_class = R.hap_class; w.modulo = R.modulo; w.Lhq = P->Lhq;
     w.depth = R.depth; w.ev = R.ev; w.cap = R.cap; w.ev_cnt = R.ev_cnt; w.status = R.status;

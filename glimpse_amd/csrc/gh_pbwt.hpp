@@ -22,8 +22,15 @@ struct SiteOp {
     uint8_t kind;      // 0 common, 1 rare
     uint8_t init;      // common: rare run just ended (init_common); rare: first site of a run (init_rare)
     uint8_t major;     // rare: major allele
-    uint8_t pad[5];
+    uint8_t pad0;
+    // genetic positions the walk's reset tests compare (so the walk reads no
+    // map from global memory): cM of site k, of site k-1, and of site ps, the
+    // site common2tot[l-1] an init_common reset moves last_reset to
+    int32_t ps;
+    float cm_k, cm_km1, cm_ps;
+    float pad1;
 };
+static_assert(sizeof(SiteOp) == 64, "SiteOp is one 64-byte record");
 
 // Arguments of one selection launch.
 struct PbwtRun {
