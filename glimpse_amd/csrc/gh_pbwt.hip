@@ -270,7 +270,8 @@ void pbwt_decode(gh_panel *P) {
             }
             op.ref = (int)P->cref[k];
             op.prev_ref = prev_ref;
-            op.nbits = N;
+            op.nbits = ns;
+            op.a_off = P->asmall_ptr[l_hq];
             op.roff = decode_col(N, op.ref, A, B);
             std::memcpy(Astore.data() + (size_t)l_hq * N, A.data(), sizeof(int32_t) * N);
             op.a_out = l_hq;
@@ -291,6 +292,7 @@ void pbwt_decode(gh_panel *P) {
             }
             op.ref = op.major ? (int)P->cref[k] : ns - (int)P->calt[k];
             op.nbits = ns;
+            op.a_off = P->asmall_ptr[l_hq];
             op.roff = decode_col(ns, op.ref, sA, sB);
         }
         ops.push_back(op);
@@ -328,10 +330,9 @@ void pbwt_decode(gh_panel *P) {
     dalloc_copy(D->bits, bits.data(), bits.size(), s);
     dalloc_copy(D->cnt, cnt.data(), cnt.size(), s);
     dalloc_copy(D->A, Astore.data(), Astore.size(), s);
-    dalloc_copy(D->aptr, P->asmall_ptr.data(), P->asmall_ptr.size(), s);
     dalloc_copy(D->aidx, P->asmall_idx.data(), P->asmall_idx.size(), s);
     GH_HIP(hipStreamSynchronize(s));
-    P->device_bytes += (int64_t)(D->ops.bytes + D->bits.bytes + D->cnt.bytes + D->A.bytes + D->aptr.bytes +
+    P->device_bytes += (int64_t)(D->ops.bytes + D->bits.bytes + D->cnt.bytes + D->A.bytes +
                                  D->aidx.bytes);
 }
 
@@ -343,7 +344,7 @@ struct WalkArgs {
     const uint32_t *cnt;
     const int32_t *A;
     int N;
-    const int32_t *aptr, *aidx;
+    const int32_t *aidx;
     float cm0;             // cM of site 0 (the walk's first reset position)
     const uint8_t *haps;   // rows of L
     const int32_t *hap_row;
@@ -530,8 +531,8 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
         if (op.kind == 0) {
             if (op.init) {
                 // haplotype_set::init_common (haplotype_set.cpp:562-630)
-                const int ns = w.aptr[op.l + 1] - w.aptr[op.l];
-                const int32_t *si = w.aidx + w.aptr[op.l];
+                const int ns = op.nbits;
+                const int32_t *si = w.aidx + op.a_off;
                 const int32_t *Aprev = w.A + (size_t)op.a_prev * N;
                 int id, fd, gd;
                 if (in_rare) {
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                     }
                 }
                 if (in_rare) {
-                    const int32_t *si = w.aidx + w.aptr[op.l];
+                    const int32_t *si = w.aidx + op.a_off;
                     int fd, gd, c2;
                     lower_count3(si, ns, f, g, idx, fd, gd, c2);
                     if (gd <= fd) { fd = 0; gd = ns; last_reset = k; cm_last = op.cm_k; }
@@ -746,7 +747,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     w.ops = D->ops.as<SiteOp>(); w.n_ops = D->n_ops;
     w.bits = D->bits.as<uint64_t>(); w.cnt = D->cnt.as<uint32_t>();
     w.A = D->A.as<int32_t>(); w.N = P->N;
-    w.aptr = D->aptr.as<int32_t>(); w.aidx = D->aidx.as<int32_t>();
+    w.aidx = D->aidx.as<int32_t>();
     w.cm0 = P->cm_pos[0];
     w.haps = R.haps; w.hap_row = R.hap_row; w.L = P->L; w.n_haps = R.n_haps; w.start = R.start;
     w.stored = R.stored; w.hap_class = R.hap_class; w.modulo = R.modulo; w.Lhq = P->Lhq;

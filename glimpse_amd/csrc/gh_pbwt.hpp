@@ -14,7 +14,7 @@ struct SiteOp {
     int32_t l;         // HQ common index (common) / index of the run's A_small_idx (rare)
     int32_t ref;       // ref_rac_l: count of 0 symbols (full or small PBWT)
     int32_t prev_ref;  // ref of the previous HQ common site
-    int32_t nbits;     // N (common) or small-set size (rare)
+    int32_t nbits;     // size of the small set of A_small_idx at l: of the rare run ending here (common), of this run (rare)
     int32_t a_out;     // prefix array after this common site
     int32_t a_in;      // prefix array before it (after any rare-run rearrangement)
     int32_t a_prev;    // prefix array after the previous common site (iota at the start)
@@ -28,7 +28,7 @@ struct SiteOp {
     // site common2tot[l-1] an init_common reset moves last_reset to
     int32_t ps;
     float cm_k, cm_km1, cm_ps;
-    float pad1;
+    int32_t a_off;     // offset of that small set in A_small_idx (A_small_ptr[l])
 };
 static_assert(sizeof(SiteOp) == 64, "SiteOp is one 64-byte record");
 
@@ -67,7 +67,7 @@ float pbwt_groups(gh_panel *P, int depth, float modulo_in, int kpbwt, std::vecto
 }  // namespace gh
 
 struct gh_panel::PbwtDev {
-    gh::DBuf ops, bits, cnt, A, aptr, aidx, tiles;
+    gh::DBuf ops, bits, cnt, A, aidx, tiles;
     int n_ops = 0, n_runs = 0, n_arrays = 0, n_tiles = 0;
     int64_t total_words = 0;
 };
