@@ -88,9 +88,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     const float Kf = (float)K, invK = 1.0f / Kf;
     const size_t vstride = (size_t)64 * W;   // one alpha vector (checkpoint) in floats
 
-    // word slot i of this wave is word i*NW + wv; slots i >= nI hold no word of
-    // this item and are skipped (a wave-uniform bound: items with fewer states
-    // than the launch capacity do proportionally less work)
+    // this wave owns words wv, wv + NW, wv + 2 NW, ... of the item, nI of them;
+    // slots i >= nI hold no word of this item and are skipped (a wave-uniform
+    // bound: items with fewer states than the launch capacity do proportionally
+    // less work)
     const int nI = min(MWL, max(0, (W - wv + NW - 1) / NW));
     // Slots run from the wave's highest word down: slot i holds word
     // (nI-1-i)*NW + wv.  When K is not a multiple of 64 the item's last word
@@ -111,8 +112,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     // one base per site: constant offsets, one address computation per site.
     // Each lane reads the 32-bit half of the word that holds its own state's
     // bit and widens the bit to an all-ones / all-zeros mask (v_bfe_i32); the
-    // selects are then bitwise (v_bfi_b32, v_and_b32), with no trip through
-    // scalar registers.
+    // selects are then bitwise (one v_bitop3_b32 or v_and_b32), with no trip
+    // through scalar registers.
     const unsigned lane_sh = lane & 31;
     auto rwords = [&](int j, uint32_t (&rw)[MWL]) {
         const uint32_t *rb = (const uint32_t *)(rows + j * W + (nI - MWL) * NW + wv) + (lane >> 5);
