@@ -199,7 +199,7 @@ int gh_impute_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int3
         d_HP.reserve(sizeof(float) * (size_t)n * 2 * L);
         const float ei = std::min(std::max(err_imp, 1e-12f), 1e-3f);   // caller_initialise.cpp:152
         const float ed = ei, ee = (float)(1.0 - (double)ei);
-        fb_launch(P, n, d_fb.as<FbItem>(), B.items.as<CondItem>(), B.maxW, B.poly.as<int32_t>(), B.t.as<float>(),
+        fb_launch(P, n, d_fb.as<FbItem>(), B.items.as<CondItem>(), B.maxW, B.maxW, B.poly.as<int32_t>(), B.t.as<float>(),
                   B.hvar.as<uint64_t>(), d_HL.as<float>(), d_flat.as<uint8_t>(), d_alpha.as<float>(),
                   d_HP.as<float>(), ee, ed, s);
         fb_mono_launch(P, n, d_fb.as<FbItem>(), B.items.as<CondItem>(), B.pos.as<int32_t>(), d_HL.as<float>(),

@@ -721,7 +721,7 @@ void run_execute(gh_run *G) {
         auto fb_all = [&](float *HPx, float *massx, EvPair &ea, EvPair &eb) {
             if (fbB == 0) {
                 GH_HIP(hipEventRecord(ea.a, s));
-                fb_launch(P, T, fb_d, items_d, maxW_it, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
+                fb_launch(P, T, fb_d, items_d, maxW_it, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
                           massx, wlo, whi, fbp);
                 GH_HIP(hipEventRecord(ea.b, s));
                 return;
@@ -731,11 +731,11 @@ void run_execute(gh_run *G) {
             GH_HIP(hipEventRecord(G->ev_fork, s));
             GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
             GH_HIP(hipEventRecord(ea.a, s));
-            fb_launch(P, fbA, fb_d, items_d, fbWA, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
+            fb_launch(P, fbA, fb_d, items_d, fbWA, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
                       massx, wlo, whi, fbp, ord);
             GH_HIP(hipEventRecord(ea.b, s));
             GH_HIP(hipEventRecord(eb.a, G->s2));
-            fb_launch(P, fbB, fb_d, items_d, fbWB, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, G->s2,
+            fb_launch(P, fbB, fb_d, items_d, fbWB, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, G->s2,
                       massx, wlo, whi, (G->latency || fewB) ? fb_default_pick(fbWB, true) : FbPick{0, 0}, ord + fbA);
             GH_HIP(hipEventRecord(eb.b, G->s2));
             GH_HIP(hipEventRecord(G->ev_join, G->s2));

@@ -553,13 +553,19 @@ static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {
     hipLaunchKernelGGL((fb_kernel<NW, MWL>), dim3(n), dim3(64 * NW), smem, s, A);
 }
 
-void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
+void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, int ckpt_maxW,
+               const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass, const int32_t *d_win_lo,
                const int32_t *d_win_hi, FbPick pick, const int32_t *d_order) {
     if (n <= 0) return;
     GH_CHECK(maxW <= 32, GH_ERR_ARG, "fb: more than 2048 conditioning states per haplotype is not supported");
     const FbShape sh = fb_shape(maxW, pick);
+    // the launch writes a checkpoint every sh.C sites into buffers sized for
+    // blocks of fb_block_sites(ckpt_maxW): longer blocks write fewer, shorter
+    // ones would overrun them
+    GH_CHECK(maxW <= ckpt_maxW && sh.C >= fb_block_sites(ckpt_maxW), GH_ERR_ARG,
+             "fb: launch block length shorter than the checkpoint layout");
     FbArgs A;
     A.fb = d_fb; A.cond = d_cond; A.poly_all = d_poly; A.t_all = d_t; A.hvar_all = d_hvar; A.HL_all = d_HL;
     A.flat_all = d_flat; A.flags = P->d_flags.as<uint8_t>(); A.ckpt_all = d_ckpt; A.HP_all = d_HP;

@@ -124,7 +124,9 @@ struct FbItem {
 struct FbPick {
     int NW, MWL;
 };
-void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, const int32_t *d_poly,
+// ckpt_maxW: the word count the checkpoint buffers were sized for (fb_ckpt_floats)
+void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, int ckpt_maxW,
+               const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass = nullptr,
                const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr, FbPick pick = FbPick{0, 0},
