@@ -35,13 +35,6 @@
 
 namespace gh {
 
-constexpr int FB_TILE = 32;
-constexpr int FB_LDS_BYTES = 156 * 1024;
-constexpr int FB_WG_PER_CU = 3;
-// LDS words before the Hvar tile: the row-word reads of word slots a wave does
-// not fill may point up to NW*MWL (<= 32) words below a row (see rwords)
-constexpr int FB_ROW_PAD = 32;
-
 struct FbArgs {
     const FbItem *fb;
     const CondItem *cond;
@@ -469,79 +462,6 @@ __global__ void fb_mono_kernel(const FbItem *__restrict__ fb, const CondItem *__
     HP_all[f.hl_off + 2 * site + 1] = o[1] / (o[0] + o[1]);
 }
 
-// Launch shape: waves per item and words per lane by the largest conditioning set.
-struct FbShape {
-    int NW, MWL, C;
-    size_t smem;
-};
-// Shapes the kernel is built for (NW waves x MWL words per lane covers 64*NW*MWL states).
-static const int kFbShapes[][2] = {
-    {1, 4}, {2, 2}, {4, 1},                     // W <= 4
-    {1, 8}, {2, 4}, {4, 2}, {8, 1},             // W <= 8
-    {1, 16}, {2, 8}, {4, 4}, {8, 2}, {16, 1},   // W <= 16
-    {2, 16}, {4, 8}, {8, 4}, {16, 2},           // W <= 32 (one wave of 32 words spills)
-};
-static FbShape fb_shape_of(int maxW, int NW, int MWL) {
-    FbShape s;
-    s.NW = NW;
-    s.MWL = MWL;
-    const size_t rows = (size_t)(FB_ROW_PAD + FB_TILE * maxW) * 8;
-    const size_t per_site = (size_t)s.NW * s.MWL * 64 * 4;
-    const size_t red = 2 * 16 * 4 * 4 + FB_TILE * 3 * 16;   // [2][NW <= 16][4] floats, then the site records
-    // The longest block (C divides FB_TILE) that leaves room for FB_WG_PER_CU
-    // workgroups per CU (what the register budget of the 4-wave shapes allows),
-    // unless that would make blocks shorter than 4 sites; else the longest that fits.
-    int C = FB_TILE;
-    while (C > 4 && rows + C * per_site + red > (size_t)FB_LDS_BYTES / FB_WG_PER_CU) C >>= 1;
-    while (C > 1 && rows + C * per_site + red > (size_t)FB_LDS_BYTES) C >>= 1;
-    s.C = C;
-    s.smem = rows + C * per_site + red;
-    return s;
-}
-static int fb_cap_words(int maxW) { return maxW <= 4 ? 4 : maxW <= 8 ? 8 : maxW <= 16 ? 16 : 32; }
-static FbShape fb_shape(int maxW, FbPick pick = FbPick{0, 0}, bool latency = false) {
-    const int cap = fb_cap_words(maxW);
-    if (pick.NW > 0 && pick.NW * pick.MWL == cap) return fb_shape_of(maxW, pick.NW, pick.MWL);
-    // GH_FB_SHAPE=<NW>x<MWL> pins a shape wherever it fits (testing, reproducing a tuned run)
-    if (const char *e = std::getenv("GH_FB_SHAPE")) {
-        int nw = 0, mwl = 0;
-        if (std::sscanf(e, "%dx%d", &nw, &mwl) == 2 && nw * mwl == cap)
-            for (const auto &sh : kFbShapes)
-                if (sh[0] == nw && sh[1] == mwl) return fb_shape_of(maxW, nw, mwl);
-    }
-    // Defaults measured with 1000 items in flight (throughput): the fewest waves
-    // per item that fill a CU.  Latency defaults (a run whose chain is the
-    // critical path, with the GPU to itself at the end): more waves per item,
-    // so each wave's share of a site is shorter.
-    if (cap == 4) return fb_shape_of(maxW, 1, 4);
-    if (cap == 8) return fb_shape_of(maxW, 2, 4);
-    if (cap == 16) return fb_shape_of(maxW, 4, 4);
-    return latency ? fb_shape_of(maxW, 8, 4) : fb_shape_of(maxW, 4, 8);
-}
-
-FbPick fb_default_pick(int maxW, bool latency) {
-    const FbShape sh = fb_shape(maxW, FbPick{0, 0}, latency);
-    return FbPick{sh.NW, sh.MWL};
-}
-
-std::vector<FbPick> fb_shapes(int maxW) {
-    std::vector<FbPick> v;
-    for (const auto &sh : kFbShapes)
-        if (sh[0] * sh[1] == fb_cap_words(maxW)) v.push_back(FbPick{sh[0], sh[1]});
-    return v;
-}
-
-// The checkpoint interval only depends on the words a shape covers (NW x MWL),
-// which every shape of a capacity class shares.
-int fb_block_sites(int maxW) { return fb_shape(maxW).C; }
-
-// Checkpoint floats of one haplotype item with up to Lp polymorphic sites and W
-// words: every C-th alpha vector (64 W floats).
-int64_t fb_ckpt_floats(int Lp, int W, int maxW) {
-    const int C = fb_block_sites(maxW);
-    return (int64_t)((Lp + C - 1) / C) * 64 * W;
-}
-
 template <int NW, int MWL>
 static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {
     static bool attr = false;
@@ -564,7 +484,7 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
     // the launch writes a checkpoint every sh.C sites into buffers sized for
     // blocks of fb_block_sites(ckpt_maxW): longer blocks write fewer, shorter
     // ones would overrun them
-    GH_CHECK(maxW <= ckpt_maxW && sh.C >= fb_block_sites(ckpt_maxW), GH_ERR_ARG,
+    GH_CHECK(fb_launch_fits(maxW, ckpt_maxW, sh), GH_ERR_ARG,
              "fb: launch block length shorter than the checkpoint layout");
     FbArgs A;
     A.fb = d_fb; A.cond = d_cond; A.poly_all = d_poly; A.t_all = d_t; A.hvar_all = d_hvar; A.HL_all = d_HL;

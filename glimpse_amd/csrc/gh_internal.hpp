@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "glimpse_hip.h"
+#include "gh_fb_layout.hpp"
 
 namespace gh {
 
@@ -119,11 +120,6 @@ struct FbItem {
     int64_t hl_off;           // into HL / HP ([2L] per item)
     int64_t flat_off;         // into flat ([L] per item)
 };
-// Launch shape of the forward-backward kernel: NW waves per haplotype, MWL
-// 64-state words per lane (NW * MWL words cover the largest set).  {0, 0}: default.
-struct FbPick {
-    int NW, MWL;
-};
 // ckpt_maxW: the word count the checkpoint buffers were sized for (fb_ckpt_floats)
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, int ckpt_maxW,
                const int32_t *d_poly,
@@ -131,13 +127,6 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass = nullptr,
                const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr, FbPick pick = FbPick{0, 0},
                const int32_t *d_order = nullptr);
-// Shapes built for sets of up to maxW words (the tuner's candidates).
-std::vector<FbPick> fb_shapes(int maxW);
-// The default shape for sets of up to maxW words: the throughput default, or
-// with latency the one that shortens a single item's chain.
-FbPick fb_default_pick(int maxW, bool latency = false);
-int fb_block_sites(int maxW);
-int64_t fb_ckpt_floats(int Lp, int W, int maxW);
 // HP at monomorphic sites (closed form) for the same items.
 void fb_mono_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_pos,
                     const float *d_HL, const uint8_t *d_flat, float *d_HP, float ee, float ed, hipStream_t s);

@@ -40,3 +40,27 @@ def small_case():
     P = synth.make_panel(2000, 3000, 4, length_mb=1.0, rare_frac=0.3, flip_frac=0.05, seed=11)
     gl, flat = synth.make_reads(P["tgt_haps"], 1.0, seed=5)
     return P, gl, flat
+
+
+def dosage(gp0, gp1):
+    """DS = gp1 + 2 gp2 with gp2 = 1 - gp0 - gp1."""
+    return gp1 + 2.0 * np.clip(1.0 - gp0 - gp1, 0.0, 1.0)
+
+
+def dosage_r2(gp0, gp1, truth):
+    """Squared Pearson correlation of DS against the true genotype over every
+    target x site pair (concordance/src/containers/call_set_reading.cpp:688-693,
+    common/src/utils/basic_stats.h:61-125)."""
+    c = np.corrcoef(dosage(gp0, gp1).ravel().astype(np.float64), np.asarray(truth, np.float64).ravel())[0, 1]
+    return float(c * c)
+
+
+def nrc(g, truth):
+    """Non-reference concordance 1 - NRD of best-guess genotypes: NRD = mismatches
+    / (mismatches + matches at RA and AA truths)
+    (concordance/src/containers/call_set_writing.cpp:46-55)."""
+    g = np.asarray(g, np.int32).ravel()
+    t = np.asarray(truth, np.int32).ravel()
+    mism = int(np.sum(g != t))
+    match_nonref = int(np.sum((g == t) & (t > 0)))
+    return 1.0 - mism / max(1, mism + match_nonref)

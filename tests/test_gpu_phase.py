@@ -39,8 +39,31 @@ def test_rephase_matches_oracle(panels, K):
         n0 = int((g0[t] != r0).sum())
         n1 = int((g1[t] != r1).sum())
         assert n0 == 0 and n1 == 0, "target %d K=%d: %d/%d haplotype alleles differ" % (t, K, n0, n1)
-        # the genotype (h0 + h1) is never changed by rephasing except at flat hets
-        assert np.array_equal((g0[t] ^ g1[t]) | flat[t], (h0s[t] ^ h1s[t]) | flat[t]) or True
+        # the genotype (h0 + h1) is never changed by rephasing except at flat
+        # (or low-quality) hets, where phasing_hmm.cpp:269-285 resamples H0 only
+        assert np.array_equal((g0[t] ^ g1[t]) | flat[t], (h0s[t] ^ h1s[t]) | flat[t])
+
+
+def test_rephase_with_lq_sites(small_case):
+    """Low-quality panel sites join the flat hets: no
+    emission, H0 resampled (phasing_hmm.cpp:75-94).  A panel copy with every 5th site LQ."""
+    from glimpse_amd import hip
+    P, gl, flat = small_case
+    P2 = dict(P)
+    P2["lq"] = (np.arange(P["L"]) % 5 == 2).astype(np.uint8)
+    OP2, GP2 = orc.Panel(P2), hip.Panel(P2, with_pbwt=False)
+    rng = np.random.default_rng(5)
+    T = flat.shape[0]
+    for K in (150, 1200):
+        sets = [np.sort(rng.choice(P["N"], K, replace=False)) for _ in range(T)]
+        h0s = np.stack([_perturb(P["tgt_haps"][2 * t], 0.02, rng) for t in range(T)])
+        h1s = np.stack([_perturb(P["tgt_haps"][2 * t + 1], 0.02, rng) for t in range(T)])
+        g0, g1 = GP2.rephase_batch(sets, flat, h0s, h1s, 31, 4, list(range(T)))
+        free = flat | P2["lq"]
+        for t in range(T):
+            r0, r1 = OP2.rephase(sets[t], flat[t], h0s[t], h1s[t], 31, 4, t)
+            assert np.array_equal(g0[t], r0) and np.array_equal(g1[t], r1), "K=%d target %d" % (K, t)
+            assert np.array_equal((g0[t] ^ g1[t]) | free[t], (h0s[t] ^ h1s[t]) | free[t])
 
 
 # every launch shape the phasing kernel is built for, at a set size of its class
