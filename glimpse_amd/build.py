@@ -32,7 +32,7 @@ def _run(cmd):
 
 def build_synth():
     out = os.path.join(HERE, "libgh_synth.so")
-    _run(["g++", "-std=c++17", "-O3", "-fPIC", "-shared", "-o", out,
+    _run(["g++", "-std=c++17", "-O3", "-march=x86-64-v2", "-fPIC", "-shared", "-pthread", "-o", out,
           os.path.join(CSRC, "synth.cpp")])
     return out
 

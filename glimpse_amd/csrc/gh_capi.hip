@@ -308,7 +308,7 @@ int gh_pbwt_select(gh_panel *P, int32_t T, const uint8_t *hap_alleles, const uin
         for (int h = 0; h < nh; h++)
             GH_CHECK(start_idx[h] >= 0 && start_idx[h] < P->N, GH_ERR_ARG, "gh_pbwt_select: start index out of range");
         const int evcap = pbwt_event_cap(P, modulo_cm, ngrp);
-        DBuf d_h, d_st, d_stored, d_cls, d_mod, d_ev, d_cnt, d_status, d_kp, d_sh, d_out, d_K;
+        DBuf d_h, d_st, d_stored, d_cls, d_mod, d_ev, d_evci, d_cnt, d_status, d_kp, d_sh, d_out, d_K;
         PbwtRun R;
         std::memset(&R, 0, sizeof(R));
         R.n_haps = nh; R.n_samples = T; R.depth = depth; R.cap = evcap;
@@ -319,6 +319,7 @@ int gh_pbwt_select(gh_panel *P, int32_t T, const uint8_t *hap_alleles, const uin
         R.hap_class = dalloc_copy(d_cls, cls.data(), nh, s);
         R.modulo = dalloc_copy(d_mod, &modulo_cm, 1, s);
         R.ev = (int32_t *)d_ev.reserve(sizeof(int32_t) * (size_t)nh * evcap * rec);
+        R.ev_ci = (int32_t *)d_evci.reserve(sizeof(int32_t) * (size_t)nh * evcap * 2);
         R.ev_cnt = (int32_t *)d_cnt.reserve(sizeof(int32_t) * nh);
         R.status = (int32_t *)d_status.reserve(sizeof(int32_t));
         GH_HIP(hipMemsetAsync(R.status, 0, sizeof(int32_t), s));

@@ -285,7 +285,7 @@ struct gh_run {
     std::vector<int32_t> start_h, nstates;   // nstates [T][n_iter]
     gh::DBuf d_gl, d_flat, d_H, d_HL, d_HP0, d_HP1, d_states, d_K, d_items, d_fb, d_poly, d_pos, d_t, d_mark,
         d_hvar, d_alpha, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_loci, d_seg, d_segsum, d_imp, d_impsum,
-        d_of1, d_dip, d_status, d_pst, d_pstart, d_pcls, d_pmod, d_ev, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
+        d_of1, d_dip, d_status, d_pst, d_pstart, d_pcls, d_pmod, d_ev, d_evci, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
     gh::PbwtRun R;
     gh_run_stats st;
     bool done = false;
@@ -448,6 +448,7 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             R.hap_row = dalloc_copy(G->d_hrow, hrow.data(), hrow.size(), s);
             R.sample_row = dalloc_copy(G->d_prow, srow.data(), srow.size(), s);
             R.ev = (int32_t *)G->d_ev.reserve(sizeof(int32_t) * (size_t)2 * npt * R.cap * (2 + 2 * R.depth));
+            R.ev_ci = (int32_t *)G->d_evci.reserve(sizeof(int32_t) * (size_t)2 * npt * R.cap * 2);
             R.ev_cnt = (int32_t *)G->d_evcnt.reserve(sizeof(int32_t) * 2 * npt);
         }
         dalloc_copy(G->d_ptg, G->pbwt_targets.data(), G->pbwt_targets.size(), s);

@@ -7,25 +7,32 @@
 //                         with a "small" PBWT over the carriers of each rare run.
 //   pbwt_decode        -- replays the stream once and keeps, per non-LQ site, the
 //                         column bits in prefix order with per-word popcount prefixes
-//                         (so the reference's rank array V[x] is one popcount), and
-//                         the prefix arrays A that selectK reads.  All of it stays
-//                         resident in HBM; the per-iteration sweep then needs no
-//                         decode at all.
+//                         (so the reference's rank array V[x] is one popcount), the
+//                         prefix array A after every S-th HQ common site (S = 1 when
+//                         all of them fit the memory budget) and the final small
+//                         order of every rare run.  All of it stays resident in HBM;
+//                         the per-iteration sweep then needs no decode at all.
 // Per Gibbs iteration (device):
 //   pbwt_walk_kernel   -- one lane per target haplotype replays
 //                         haplotype_set::matchHapsFromCompressedPBWTSmall
 //                         (phase/src/containers/haplotype_set.cpp:317-728) for that
 //                         haplotype alone: LF-mapping of the insertion index and
 //                         match interval, resets, rare-run re-indexing, and selectK
-//                         events recorded as (site, phase) keyed records.  A
+//                         events recorded as (site, phase) keyed records of neighbour
+//                         positions in a named prefix order.  A
 //                         workgroup is one walker wave and three loader waves that
 //                         stage the next tile of rank data, site ops and alleles
 //                         into the second of two LDS buffers.
+//   pbwt_resolve_kernel -- panel ids of those positions: a stored prefix array, or
+//                         steps back through the columns (select) and rare-run
+//                         rearrangements to the nearest stored one.
 //   pbwt_compact_kernel -- one wave per target merges its two haplotypes' records
 //                         newest first, rank by rank, into the first Kpbwt distinct
 //                         panel haplotypes (conditioning_set.cpp:96-104), then
 //                         writes them sorted.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -184,6 +191,20 @@ void build_sparse_pbwt(const gh_panel_desc *d, std::vector<uint8_t> &Y, std::vec
 }
 
 /* ------------------------------------------------------------------ host: decode */
+int pbwt_ckpt_every(int Lhq, int N) {
+    if (const char *e = std::getenv("GH_PBWT_CKPT_EVERY")) {
+        const int v = std::atoi(e);
+        if (v >= 1) return v;
+    }
+    double mb = 1024.0;
+    if (const char *e = std::getenv("GH_PBWT_CKPT_MB")) {
+        const double v = std::atof(e);
+        if (v > 0) mb = v;
+    }
+    const double bytes = (double)std::max(Lhq, 1) * N * 4.0, budget = mb * 1048576.0;
+    return bytes <= budget ? 1 : (int)std::ceil(bytes / budget);
+}
+
 void pbwt_decode(gh_panel *P) {
     GH_CHECK(!P->ypacked.empty(), GH_ERR_ARG, "panel has no sparse-PBWT stream");
     const int N = P->N, L = P->L;
@@ -192,19 +213,19 @@ void pbwt_decode(gh_panel *P) {
     std::vector<SiteOp> ops;
     std::vector<uint64_t> bits;
     std::vector<uint32_t> cnt;
-    std::vector<int32_t> Astore;   // [arrays][N]
     const int Lhq = P->Lhq;
-    Astore.reserve((size_t)(Lhq + 2) * N);
-    Astore.resize((size_t)Lhq * N);
+    const int S = pbwt_ckpt_every(Lhq, N);
+    D->ck_every = S;
+    std::vector<ColInfo> cols(std::max(Lhq, 1));
+    std::vector<int32_t> sfin(std::max<size_t>(P->asmall_idx.size(), 1), 0);
+    std::vector<int32_t> Astore;   // [stored arrays][N]: the order after every S-th HQ common site
+    Astore.reserve((size_t)(Lhq / S + 1) * N);
+    int n_ck = 0;
     std::vector<int32_t> A(N), B(N), sA, sB;
+    std::vector<uint8_t> in_small(N, 0);
     std::iota(A.begin(), A.end(), 0);
     const uint8_t *y = P->ypacked.data(), *yend = y + P->ypacked.size();
-    int l_hq = 0, last_k = -1, prev_ref = 0, n_extra = 0, n_runs = 0;
-    auto push_array = [&](const std::vector<int32_t> &a) {
-        Astore.insert(Astore.end(), a.begin(), a.end());
-        return Lhq + n_extra++;
-    };
-    int iota_arr = -1;
+    int l_hq = 0, last_k = -1, prev_ref = 0, n_runs = 0;
     auto decode_col = [&](int n, int ref, std::vector<int32_t> &a, std::vector<int32_t> &b) {
         // appends column bits (prefix order) + popcount prefixes, permutes a -> b -> a
         const int W = (n + 63) / 64;
@@ -218,9 +239,13 @@ void pbwt_decode(gh_panel *P) {
             const uint8_t z = *y++;
             const int c = p3dec(z), sym = z >> 7;
             GH_CHECK(c > 0 && m + c <= n, GH_ERR_ARG, "sparse-PBWT stream inconsistent with the panel");
-            for (int j = m; j < m + c; j++) {
-                if (sym) bits[base + (j >> 6)] |= 1ull << (j & 63);
-                b[occ[sym]++] = a[j];
+            if (sym) {
+                for (int j = m; j < m + c; j++) bits[base + (j >> 6)] |= 1ull << (j & 63);
+                std::memcpy(b.data() + occ[1], a.data() + m, sizeof(int32_t) * c);
+                occ[1] += c;
+            } else {
+                std::memcpy(b.data() + occ[0], a.data() + m, sizeof(int32_t) * c);
+                occ[0] += c;
             }
             m += c;
         }
@@ -248,33 +273,35 @@ void pbwt_decode(gh_panel *P) {
             op.ps = l_hq >= 1 ? P->common2tot[l_hq - 1] : 0;
             op.cm_ps = P->cm_pos[op.ps];
             const int ns = P->asmall_ptr[l_hq + 1] - P->asmall_ptr[l_hq];
-            const int prev_out = l_hq > 0 ? l_hq - 1 : -1;
-            if (prev_out < 0 && iota_arr < 0) {
-                std::vector<int32_t> io(N);
-                std::iota(io.begin(), io.end(), 0);
-                iota_arr = push_array(io);
-            }
-            op.a_prev = prev_out >= 0 ? prev_out : iota_arr;
+            ColInfo &ci = cols[l_hq];
+            ci.ns = 0;
+            ci.a_off = P->asmall_ptr[l_hq];
             if (ns > 0 && last_k >= 0 && !(P->flags[last_k] & SF_COMMON)) {
+                // rare run -> common: the carriers leave their places and follow
+                // the others in the run's final small order (kept in sfin)
                 op.init = 1;
+                GH_CHECK((int)sA.size() == ns, GH_ERR_ARG, "sparse-PBWT small set inconsistent with A_small_idx");
                 const int32_t *si = P->asmall_idx.data() + P->asmall_ptr[l_hq];
-                std::vector<uint8_t> in_small(N, 0);
                 for (int j = 0; j < ns; j++) in_small[si[j]] = 1;
                 int nz = 0;
                 for (int p = 0; p < N; p++) if (!in_small[p]) B[nz++] = A[p];
-                for (size_t j = 0; j < sA.size(); j++) B[nz++] = sA[j];
+                for (int j = 0; j < ns; j++) { B[nz++] = sA[j]; sfin[ci.a_off + j] = sA[j]; in_small[si[j]] = 0; }
                 A.swap(B);
-                op.a_in = push_array(A);
-            } else {
-                op.a_in = op.a_prev;
+                ci.ns = ns;
             }
             op.ref = (int)P->cref[k];
             op.prev_ref = prev_ref;
             op.nbits = ns;
             op.a_off = P->asmall_ptr[l_hq];
             op.roff = decode_col(N, op.ref, A, B);
-            std::memcpy(Astore.data() + (size_t)l_hq * N, A.data(), sizeof(int32_t) * N);
-            op.a_out = l_hq;
+            ci.roff = op.roff;
+            ci.ref = op.ref;
+            if ((l_hq + 1) % S == 0) {
+                ci.ck = n_ck++;
+                Astore.insert(Astore.end(), A.begin(), A.end());
+            } else {
+                ci.ck = -1;
+            }
             prev_ref = op.ref;
             l_hq++;
         } else {
@@ -324,16 +351,18 @@ void pbwt_decode(gh_panel *P) {
             D->n_tiles = 0;   // a single site's rank data exceeds a tile: global lookups
         }
     }
-    D->n_arrays = Lhq + n_extra;
+    D->n_arrays = n_ck;
     hipStream_t s = P->stream;
     dalloc_copy(D->ops, ops.data(), ops.size(), s);
     dalloc_copy(D->bits, bits.data(), bits.size(), s);
     dalloc_copy(D->cnt, cnt.data(), cnt.size(), s);
     dalloc_copy(D->A, Astore.data(), Astore.size(), s);
     dalloc_copy(D->aidx, P->asmall_idx.data(), P->asmall_idx.size(), s);
+    dalloc_copy(D->sfin, sfin.data(), sfin.size(), s);
+    dalloc_copy(D->cols, cols.data(), cols.size(), s);
     GH_HIP(hipStreamSynchronize(s));
-    P->device_bytes += (int64_t)(D->ops.bytes + D->bits.bytes + D->cnt.bytes + D->A.bytes +
-                                 D->aidx.bytes);
+    P->device_bytes += (int64_t)(D->ops.bytes + D->bits.bytes + D->cnt.bytes + D->A.bytes + D->aidx.bytes +
+                                 D->sfin.bytes + D->cols.bytes);
 }
 
 /* ------------------------------------------------------------------ device: walk */
@@ -342,7 +371,6 @@ struct WalkArgs {
     int n_ops;
     const uint64_t *bits;
     const uint32_t *cnt;
-    const int32_t *A;
     int N;
     const int32_t *aidx;
     float cm0;             // cM of site 0 (the walk's first reset position)
@@ -357,6 +385,7 @@ struct WalkArgs {
     int Lhq;
     int depth;
     int32_t *ev;           // [n_haps][cap][rec]
+    int32_t *ev_ci;        // [n_haps][cap][2]: prefix-array code and insertion index
     int cap;
     int32_t *ev_cnt;       // [n_haps]
     int32_t *status;
@@ -408,6 +437,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
     const uint8_t *stored = w.stored + (size_t)w.hap_class[h] * w.Lhq;
     const float modulo = w.modulo[w.hap_class[h]];
     int32_t *ev = w.ev + (size_t)h * w.cap * rec;
+    int32_t *ev_ci = w.ev_ci + (size_t)h * w.cap * 2;
     int nev = 0;
     int idx = w.start[h], f = 0, g = N, last_reset = 0, last_rare = 0, tar = 0;
     float cm_last = w.cm0;   // cM of site last_reset
@@ -415,8 +445,11 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
     bool in_rare = false;
     bool overflow = false;
 
-    // haplotype_set::selectK (haplotype_set.cpp:682-728) as one record
-    auto selectK = [&](int key, int ref, const int32_t *arr, int a) {
+    // haplotype_set::selectK (haplotype_set.cpp:682-728) as one record: the
+    // neighbour positions idx, idx+1, ... (down) and idx-1, idx-2, ... (up) that
+    // carry the target allele; their panel ids are looked up afterwards in the
+    // prefix array named by code (pbwt_resolve_kernel)
+    auto selectK = [&](int key, int ref, int code, int a) {
         const int d_up = idx - f, d_down = g - idx, k1 = depth;
         int nh_up = 0, nh_down = 0;
         if (d_up < k1 && d_down < k1) { nh_up = d_up; nh_down = d_down; }
@@ -428,15 +461,17 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
         int nd = 0, nu = 0;
         for (int j = 0; j < nh_down; j++) {
             const int i2 = idx + j;
-            if (i2 < N && ((i2 >= ref) == (a != 0))) r[2 + nd++] = arr[i2]; else break;
+            if (i2 < N && ((i2 >= ref) == (a != 0))) nd++; else break;
         }
         for (int j = 0; j < nh_up; j++) {
             const int i2 = idx - (j + 1);
-            if (i2 >= 0 && ((i2 >= ref) == (a != 0))) r[2 + depth + nu++] = arr[i2]; else break;
+            if (i2 >= 0 && ((i2 >= ref) == (a != 0))) nu++; else break;
         }
         if (nd == 0 && nu == 0) return;   // nothing pushed: no record needed
         r[0] = key;
         r[1] = nd | (nu << 16);
+        ev_ci[2 * nev] = code;
+        ev_ci[2 * nev + 1] = idx;
         nev++;
     };
 
@@ -533,7 +568,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                 // haplotype_set::init_common (haplotype_set.cpp:562-630)
                 const int ns = op.nbits;
                 const int32_t *si = w.aidx + op.a_off;
-                const int32_t *Aprev = w.A + (size_t)op.a_prev * N;
+                const int Aprev = code_after(op.l - 1);   // the order after the previous HQ common site
                 int id, fd, gd;
                 if (in_rare) {
                     id = N - ns + sidx;
@@ -575,7 +610,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
             const bool st = STAGED ? ((sa >> 1) & 1) != 0 : stored[op.l] != 0;
             if (gd <= fd) {
                 if (st || op.cm_km1 - cm_last > modulo / 2.0f)
-                    selectK(2 * k + 1, op.prev_ref, w.A + (size_t)op.a_in * N, prev);
+                    selectK(2 * k + 1, op.prev_ref, code_entering(op.l), prev);
                 fd = a * ref;
                 gd = a * N + (1 - a) * ref;
                 last_reset = k; cm_last = op.cm_k;
@@ -583,7 +618,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
             }
             idx = id; f = fd; g = gd;
             if (!reset && st) {
-                selectK(2 * k + 1, ref, w.A + (size_t)op.a_out * N, a);
+                selectK(2 * k + 1, ref, code_after(op.l), a);
                 f = a * ref;
                 g = a * N + (1 - a) * ref;
                 last_reset = k; cm_last = op.cm_k;
@@ -643,6 +678,101 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
     if (!act) return;
     w.ev_cnt[h] = nev;
     if (overflow) atomicExch(w.status, 2);
+}
+
+/* ------------------------------------------------------------------ device: resolve */
+// Panel ids of the neighbour positions the walk recorded.  A stored prefix
+// array answers directly; otherwise a position is stepped back site by site
+// (an "after" order to the entering order of the same site by the inverse LF
+// mapping, select on the column; an entering order that a rare run
+// rearranged to the previous "after" order, or to the run's final small
+// order for its carriers) until a stored array or the identity order.
+struct ResolveArgs {
+    const ColInfo *cols;
+    const uint64_t *bits;
+    const uint32_t *cnt;
+    const int32_t *A;      // [stored][N]
+    const int32_t *aidx;   // A_small_idx (positions of the carriers, sorted)
+    const int32_t *sfin;   // final small order of each rare run (aligned with aidx)
+    int N, W;              // haplotypes, column words
+    int32_t *ev;
+    const int32_t *ev_ci;
+    const int32_t *ev_cnt;
+    int cap, depth, n_haps;
+};
+
+// position of the r-th (0-based) set bit of v
+__device__ __forceinline__ int select_in_word(uint64_t v, int r) {
+    int pos = 0;
+#pragma unroll
+    for (int sh = 32; sh > 0; sh >>= 1) {
+        const uint64_t lo = v & ((1ull << sh) - 1ull);
+        const int c = __popcll(lo);
+        if (r >= c) { r -= c; v >>= sh; pos += sh; } else v = lo;
+    }
+    return pos;
+}
+
+// position of the x-th (0-based) 1 (one) or 0 of a column of W words with
+// popcount prefixes cnt (ones before each word)
+__device__ int select_col(const uint64_t *bits, const uint32_t *cnt, int64_t off, int W, int x, bool one) {
+    int lo = 0, hi = W - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int c = one ? (int)cnt[off + mid] : 64 * mid - (int)cnt[off + mid];
+        if (c <= x) lo = mid; else hi = mid - 1;
+    }
+    const uint64_t v = bits[off + lo];
+    const int before = one ? (int)cnt[off + lo] : 64 * lo - (int)cnt[off + lo];
+    return 64 * lo + select_in_word(one ? v : ~v, x - before);
+}
+
+__device__ int resolve_id(const ResolveArgs &a, int code, int p) {
+    int l = code / 2 - 1;
+    bool entering = code & 1;
+    for (;;) {
+        if (!entering) {
+            if (l < 0) return p;   // identity order before the first site
+            const ColInfo c = a.cols[l];
+            if (c.ck >= 0) return a.A[(size_t)c.ck * a.N + p];
+            p = p < c.ref ? select_col(a.bits, a.cnt, c.roff, a.W, p, false)
+                          : select_col(a.bits, a.cnt, c.roff, a.W, p - c.ref, true);
+            entering = true;
+        } else {
+            const ColInfo c = a.cols[l];
+            if (c.ns > 0) {
+                if (p >= a.N - c.ns) return a.sfin[c.a_off + p - (a.N - c.ns)];
+                // the p-th position that is not a carrier's
+                const int32_t *si = a.aidx + c.a_off;
+                int q = p;
+                for (int j = 0; j < c.ns && si[j] <= q; j++) q++;
+                p = q;
+            }
+            l--;
+            entering = false;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void pbwt_resolve_kernel(ResolveArgs a) {
+    const int h = blockIdx.x;
+    if (h >= a.n_haps) return;
+    const int rec = 2 + 2 * a.depth, per = 2 * a.depth;
+    const int nev = a.ev_cnt[h];
+    int32_t *ev = a.ev + (size_t)h * a.cap * rec;
+    const int32_t *ci = a.ev_ci + (size_t)h * a.cap * 2;
+    for (int i = threadIdx.x; i < nev * per; i += blockDim.x) {
+        const int r = i / per, j = i - r * per;
+        int32_t *rr = ev + (size_t)r * rec;
+        const int nd = rr[1] & 0xffff, nu = rr[1] >> 16;
+        const int code = ci[2 * r], idx = ci[2 * r + 1];
+        if (j < a.depth) {
+            if (j < nd) rr[2 + j] = resolve_id(a, code, idx + j);
+        } else {
+            const int u = j - a.depth;
+            if (u < nu) rr[2 + a.depth + u] = resolve_id(a, code, idx - 1 - u);
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ device: compact */
@@ -746,12 +876,13 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     WalkArgs w;
     w.ops = D->ops.as<SiteOp>(); w.n_ops = D->n_ops;
     w.bits = D->bits.as<uint64_t>(); w.cnt = D->cnt.as<uint32_t>();
-    w.A = D->A.as<int32_t>(); w.N = P->N;
+    w.N = P->N;
     w.aidx = D->aidx.as<int32_t>();
     w.cm0 = P->cm_pos[0];
     w.haps = R.haps; w.hap_row = R.hap_row; w.L = P->L; w.n_haps = R.n_haps; w.start = R.start;
     w.stored = R.stored; w.hap_class = R.hap_class; w.modulo = R.modulo; w.Lhq = P->Lhq;
-    w.depth = R.depth; w.ev = R.ev; w.cap = R.cap; w.ev_cnt = R.ev_cnt; w.status = R.status;
+    GH_CHECK(R.ev_ci, GH_ERR_ARG, "pbwt selection: no record index buffer");
+    w.depth = R.depth; w.ev = R.ev; w.ev_ci = R.ev_ci; w.cap = R.cap; w.ev_cnt = R.ev_cnt; w.status = R.status;
     w.tile_op = D->tiles.as<int32_t>(); w.n_tiles = D->n_tiles; w.total_words = D->total_words;
     if (D->n_tiles > 0) {
         static const bool attr = [] {
@@ -767,6 +898,16 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
         hipLaunchKernelGGL(pbwt_walk_kernel<false>, dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
     }
     GH_HIP(hipGetLastError());
+    {
+        ResolveArgs ra;
+        ra.cols = D->cols.as<ColInfo>(); ra.bits = D->bits.as<uint64_t>(); ra.cnt = D->cnt.as<uint32_t>();
+        ra.A = D->A.as<int32_t>(); ra.aidx = D->aidx.as<int32_t>(); ra.sfin = D->sfin.as<int32_t>();
+        ra.N = P->N; ra.W = (P->N + 63) / 64;
+        ra.ev = R.ev; ra.ev_ci = R.ev_ci; ra.ev_cnt = R.ev_cnt; ra.cap = R.cap; ra.depth = R.depth;
+        ra.n_haps = R.n_haps;
+        hipLaunchKernelGGL(pbwt_resolve_kernel, dim3(R.n_haps), dim3(256), 0, s, ra);
+        GH_HIP(hipGetLastError());
+    }
     if (R.out_states) {
         CompactArgs c;
         c.ev = R.ev; c.ev_cnt = R.ev_cnt; c.cap = R.cap; c.depth = R.depth; c.N = P->N;

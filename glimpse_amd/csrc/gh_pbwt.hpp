@@ -15,9 +15,7 @@ struct SiteOp {
     int32_t ref;       // ref_rac_l: count of 0 symbols (full or small PBWT)
     int32_t prev_ref;  // ref of the previous HQ common site
     int32_t nbits;     // size of the small set of A_small_idx at l: of the rare run ending here (common), of this run (rare)
-    int32_t a_out;     // prefix array after this common site
-    int32_t a_in;      // prefix array before it (after any rare-run rearrangement)
-    int32_t a_prev;    // prefix array after the previous common site (iota at the start)
+    int32_t pad1[3];
     int64_t roff;      // offset of this site's column words / popcount prefixes
     uint8_t kind;      // 0 common, 1 rare
     uint8_t init;      // common: rare run just ended (init_common); rare: first site of a run (init_rare)
@@ -32,6 +30,25 @@ struct SiteOp {
 };
 static_assert(sizeof(SiteOp) == 64, "SiteOp is one 64-byte record");
 
+// Prefix arrays of the sweep, named by code: 2*(l+1) is the order after HQ
+// common site l (code 0: the identity order before the first site), 2*(l+1)+1
+// the order entering site l (after the rearrangement of a rare run that ends
+// there).  Only every S-th "after" array is stored; any other entry is found
+// by stepping back through the sites' columns (inverse LF mapping: the p-th 0
+// or 1 of a column in prefix order) and the rare-run rearrangements.
+__host__ __device__ inline int32_t code_after(int l) { return 2 * (l + 1); }
+__host__ __device__ inline int32_t code_entering(int l) { return 2 * (l + 1) + 1; }
+
+// Per HQ common site: where its column lives and how its entering order was made.
+struct ColInfo {
+    int64_t roff;    // column words / popcount prefixes (prefix order entering the site)
+    int32_t ref;     // zeros in the column (cref)
+    int32_t ns;      // carriers of the rare run that ends here (0: none)
+    int32_t a_off;   // their positions in A_small_idx and their final small order in sfin
+    int32_t ck;      // index of the stored "after" array, -1 if not stored
+};
+static_assert(sizeof(ColInfo) == 24, "ColInfo is 24 bytes");
+
 // Arguments of one selection launch.
 struct PbwtRun {
     int n_haps, n_samples, depth, cap;
@@ -42,6 +59,7 @@ struct PbwtRun {
     const int32_t *hap_class;   // [n_haps]
     const float *modulo;        // [n_classes]
     int32_t *ev;                // [n_haps][cap][2 + 2*depth]
+    int32_t *ev_ci;             // [n_haps][cap][2]: prefix-array code and insertion index of each record
     int32_t *ev_cnt;            // [n_haps]
     int32_t *status;
     // optional compaction
@@ -67,7 +85,15 @@ float pbwt_groups(gh_panel *P, int depth, float modulo_in, int kpbwt, std::vecto
 }  // namespace gh
 
 struct gh_panel::PbwtDev {
-    gh::DBuf ops, bits, cnt, A, aidx, tiles;
+    gh::DBuf ops, bits, cnt, A, aidx, sfin, cols, tiles;
     int n_ops = 0, n_runs = 0, n_arrays = 0, n_tiles = 0;
+    int ck_every = 1;            // S: one stored prefix array every S HQ common sites
     int64_t total_words = 0;
 };
+
+namespace gh {
+// Sites between stored prefix arrays for a panel of Lhq HQ common sites and N
+// haplotypes: every site when they fit in budget_mb (GH_PBWT_CKPT_MB, default
+// 1024 MiB), else the smallest S that does; GH_PBWT_CKPT_EVERY pins S.
+int pbwt_ckpt_every(int Lhq, int N);
+}  // namespace gh

@@ -21,12 +21,16 @@
  * equal phred values is flat.
  */
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <sys/mman.h>
 
 namespace {
 
@@ -79,6 +83,61 @@ struct SynthPanel {
     std::vector<uint8_t> ref_alleles;     /* optional dense [N][L] for small panels */
     std::string err;
 };
+
+static inline uint64_t mix64(uint64_t a, uint64_t b) {
+    uint64_t x = a ^ (b + 0x9E3779B97F4A7C15ull + (a << 6) + (a >> 2));
+    return Xoshiro::splitmix(x);
+}
+
+template <typename F>
+static void parallel_for(int n, int n_threads, F &&f) {
+    n_threads = std::max(1, std::min(n_threads, n));
+    if (n_threads == 1) { for (int i = 0; i < n; i++) f(i); return; }
+    std::vector<std::thread> th;
+    std::atomic<int> next(0);
+    for (int t = 0; t < n_threads; t++)
+        th.emplace_back([&]() {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= n) break;
+                f(i);
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
+// Large zeroed scratch arrays, mapped with their pages populated up front:
+// touching gigabytes of fresh pages one fault at a time costs far more than
+// the generator's own work on some hosts.
+template <typename T>
+struct Populated {
+    T *p = nullptr;
+    size_t n = 0;
+    explicit Populated(size_t count) : n(count) {
+        void *m = mmap(nullptr, std::max<size_t>(n, 1) * sizeof(T), PROT_READ | PROT_WRITE,
+                       MAP_PRIVATE | MAP_ANONYMOUS | MAP_POPULATE, -1, 0);
+        if (m == MAP_FAILED) throw std::bad_alloc();
+        p = (T *)m;
+    }
+    ~Populated() { if (p) munmap(p, std::max<size_t>(n, 1) * sizeof(T)); }
+    Populated(const Populated &) = delete;
+    Populated &operator=(const Populated &) = delete;
+    T *data() { return p; }
+    T &operator[](size_t i) { return p[i]; }
+};
+
+// 64x64 bit transpose (Hacker's Delight 7-3, recursive block swaps): bit j of
+// a[i] moves to bit i of a[j].
+static void transpose64(uint64_t a[64]) {
+    uint64_t m = 0x00000000FFFFFFFFull;
+    for (int j = 32; j != 0; j >>= 1, m ^= (m << j)) {
+        for (int k = 0; k < 64; k = ((k | j) + 1) & ~j) {
+            const uint64_t t = ((a[k] >> j) ^ a[k | j]) & m;
+            a[k] ^= t << j;
+            a[k | j] ^= t;
+        }
+    }
+}
 
 } // namespace
 
@@ -182,6 +241,189 @@ void *gh_synth_panel_new(const gh_synth_cfg *c) {
     return P;
 }
 
+/* Founder-segment generator for large panels (the same model as
+ * gh_synth_panel_new, organised for N in the 1e5 range).  Site-level draws
+ * (frequency spectrum, founder alleles, injected rare variants, mutations,
+ * ALT flips) come from a per-site stream and haplotype switch points from a
+ * per-haplotype stream, so the work splits over n_threads threads and the
+ * result does not depend on their number.  Each haplotype's alleles are built
+ * 64 sites at a time from its founder segments (OR of the founder's allele
+ * word, masked to the segment), mutations flip single bits, and a 64x64 bit
+ * transpose gives the site-major rows. */
+void *gh_synth_panel_fast(const gh_synth_cfg *c, int32_t n_threads) {
+    SynthPanel *P = new SynthPanel();
+    const int N = c->n_haps, L = c->n_sites, T = c->n_targets, F = std::max(1, c->n_founders);
+    const int H = N + 2 * T;
+    P->L = L; P->N = N; P->T = T; P->W = (N + 63) / 64;
+    if (F > 64) { P->err = "synth: the fast generator takes at most 64 founders"; P->L = 0; return P; }
+    if (c->keep_dense) { P->err = "synth: the fast generator keeps no dense matrix"; P->L = 0; return P; }
+    Xoshiro rng(c->seed);
+    const double len_bp = c->length_mb * 1e6;
+    const int M = (int)(L * 1.6) + 64;
+    std::vector<int32_t> cand(M);
+    for (int i = 0; i < M; i++) cand[i] = c->start_bp + (int32_t)(rng.u01() * len_bp);
+    std::sort(cand.begin(), cand.end());
+    for (int i = 1; i < M; i++) if (cand[i] <= cand[i - 1]) cand[i] = cand[i - 1] + 1;
+    const double first_cm = cand[0] / 1e6;
+    std::vector<double> cmv(M);
+    for (int i = 0; i < M; i++) cmv[i] = cand[i] / 1e6 - first_cm;
+    const int MB = (M + 63) / 64, HW = (H + 63) / 64;
+    // ---- per-site draws
+    std::vector<uint64_t> famask(M, 0ull);
+    std::vector<uint8_t> injected(M, 0), flip(M, 0);
+    std::vector<std::vector<int32_t>> hits(M);   // mutated haplotypes, or the carriers of an injected site
+    const double lo = std::log(1.0 / (2.0 * N)), hi = std::log(0.5);
+    const double lq = c->mu > 0 ? std::log(1.0 - c->mu) : 0.0;
+    parallel_for(M, n_threads, [&](int i) {
+        Xoshiro r(mix64(c->seed, 0x5173ull + (uint64_t)i));
+        injected[i] = r.u01() < c->rare_frac;
+        if (!injected[i]) {
+            const double f = std::exp(lo + r.u01() * (hi - lo));
+            uint64_t m = 0;
+            for (int j = 0; j < F; j++) if (r.u01() < f) m |= 1ull << j;
+            famask[i] = m;
+            if (c->mu > 0) {
+                long pos = -1;
+                for (;;) {
+                    pos += 1 + (long)std::floor(std::log(1.0 - r.u01()) / lq);
+                    if (pos >= H) break;
+                    hits[i].push_back((int32_t)pos);
+                }
+            }
+        } else {
+            const double mmax = std::max(1.0, N * (double)c->sparse_maf);
+            int mac = (int)std::floor(std::exp(r.u01() * std::log(mmax)));
+            mac = std::max(1, std::min(mac, H - 1));
+            for (int j = 0; j < mac; j++) hits[i].push_back((int32_t)r.below(H));
+        }
+        flip[i] = r.u01() < c->flip_frac;
+    });
+    // ---- founder allele words: fbits[f][b] bit j = allele of founder f at site 64b+j
+    std::vector<uint64_t> fbits((size_t)F * MB, 0ull);
+    for (int i = 0; i < M; i++) {
+        uint64_t m = famask[i];
+        while (m) {
+            const int f = __builtin_ctzll(m);
+            m &= m - 1;
+            fbits[(size_t)f * MB + (i >> 6)] |= 1ull << (i & 63);
+        }
+    }
+    // ---- haplotype-major words from founder segments
+    auto *hwp = new Populated<uint64_t>((size_t)H * MB);
+    Populated<uint64_t> &hw = *hwp;
+    parallel_for((H + 255) / 256, n_threads, [&](int hb) {
+        for (int h = hb * 256; h < std::min(H, hb * 256 + 256); h++) {
+            Xoshiro r(mix64(c->seed, 0xA11CE00000ull + (uint64_t)h));
+            uint64_t *row = hw.data() + (size_t)h * MB;
+            int fo = (int)r.below(F);
+            double next_sw = r.expo(c->rho_per_cm);
+            int a = 0;
+            while (a < M) {
+                // first site at or after the switch point (one switch per site at
+                // most): galloping search from a, segments are short
+                int b;
+                {
+                    int lo2 = a + 1, step = 1;
+                    while (lo2 + step < M && cmv[lo2 + step] < next_sw) { lo2 += step; step <<= 1; }
+                    if (lo2 < M && cmv[lo2] >= next_sw) b = lo2;
+                    else b = (int)(std::lower_bound(cmv.begin() + lo2, cmv.begin() + std::min(M, lo2 + step + 1),
+                                                    next_sw) - cmv.begin());
+                }
+                // sites [a, b) copy founder fo
+                const uint64_t *src = fbits.data() + (size_t)fo * MB;
+                for (int blk = a >> 6; blk <= (b - 1) >> 6; blk++) {
+                    const int s0 = std::max(a, blk * 64) - blk * 64, s1 = std::min(b, blk * 64 + 64) - blk * 64;
+                    const uint64_t mk = (s1 - s0 == 64) ? ~0ull : (((1ull << (s1 - s0)) - 1ull) << s0);
+                    row[blk] |= src[blk] & mk;
+                }
+                if (b >= M) break;
+                fo = (int)r.below(F);
+                next_sw = cmv[b] + r.expo(c->rho_per_cm);
+                a = b;
+            }
+        }
+    });
+    // ---- mutations and injected carriers (each thread owns a range of 64-site blocks)
+    parallel_for(MB, n_threads, [&](int blk) {
+        for (int i = blk * 64; i < std::min(M, blk * 64 + 64); i++)
+            for (int32_t h : hits[i]) {
+                uint64_t &w = hw[(size_t)h * MB + (i >> 6)];
+                if (injected[i]) w |= 1ull << (i & 63); else w ^= 1ull << (i & 63);
+            }
+    });
+    // ---- transpose to site-major rows of HW words
+    Populated<uint64_t> sm((size_t)M * HW);
+    // tiles of 8 x 8 blocks, so every cache line read or written is used whole
+    const int TB = 8, nsb = (MB + TB - 1) / TB, nhb = (HW + TB - 1) / TB;
+    parallel_for(nsb * nhb, n_threads, [&](int tile) {
+        const int sb0 = (tile / nhb) * TB, hb0 = (tile % nhb) * TB;
+        uint64_t a[64];
+        for (int sb = sb0; sb < std::min(MB, sb0 + TB); sb++)
+            for (int hb = hb0; hb < std::min(HW, hb0 + TB); hb++) {
+                for (int k = 0; k < 64; k++) {
+                    const int h = hb * 64 + k;
+                    a[k] = h < H ? hw[(size_t)h * MB + sb] : 0ull;
+                }
+                transpose64(a);
+                for (int k = 0; k < 64; k++) {
+                    const int i = sb * 64 + k;
+                    if (i < M) sm[(size_t)i * HW + hb] = a[k];
+                }
+            }
+    });
+    delete hwp;
+    // ---- flips, monomorphic filter, classification
+    P->rare_ptr.push_back(0);
+    P->tgt.assign((size_t)2 * T * L, 0);
+    const int W = P->W;
+    const uint64_t lastN = (N & 63) ? ((1ull << (N & 63)) - 1ull) : ~0ull;
+    int kept = 0;
+    for (int i = 0; i < M && kept < L; i++) {
+        uint64_t *row = sm.data() + (size_t)i * HW;
+        if (flip[i]) {
+            for (int w = 0; w < HW; w++) row[w] = ~row[w];
+            if (H & 63) row[HW - 1] &= (1ull << (H & 63)) - 1ull;
+        }
+        uint32_t ones = 0;
+        for (int w = 0; w < W - 1; w++) ones += (uint32_t)__builtin_popcountll(row[w]);
+        ones += (uint32_t)__builtin_popcountll(row[W - 1] & lastN);
+        if (ones == 0 || ones == (uint32_t)N) continue;
+        const uint32_t ca = ones, cr = (uint32_t)N - ones;
+        const bool com = is_common_site(cr, ca, c->sparse_maf);
+        const bool maj = ca > cr;
+        P->bp.push_back(cand[i]);
+        P->cref.push_back(cr); P->calt.push_back(ca);
+        P->common.push_back(com); P->major.push_back(maj);
+        if (com) {
+            const size_t base = P->common_words.size();
+            P->common_words.insert(P->common_words.end(), row, row + W);
+            P->common_words[base + W - 1] &= lastN;
+        } else {
+            for (int w = 0; w < W; w++) {
+                uint64_t x = maj ? ~row[w] : row[w];
+                if (w == W - 1) x &= lastN;
+                while (x) {
+                    const int b = __builtin_ctzll(x);
+                    x &= x - 1;
+                    P->rare_idx.push_back(w * 64 + b);
+                }
+            }
+        }
+        P->rare_ptr.push_back((int32_t)P->rare_idx.size());
+        for (int t = 0; t < 2 * T; t++) {
+            const int h = N + t;
+            P->tgt[(size_t)t * L + kept] = (uint8_t)((row[h >> 6] >> (h & 63)) & 1ull);
+        }
+        kept++;
+    }
+    if (kept < L) { P->err = "synth: not enough polymorphic sites"; P->L = kept; }
+    P->cm.resize(P->L);
+    for (int l = 0; l < P->L; l++) P->cm[l] = P->bp[l] * 1.0 / 1e6;
+    const double base = P->L ? P->cm[0] : 0.0;
+    for (int l = 0; l < P->L; l++) P->cm[l] -= base;
+    return P;
+}
+
 void gh_synth_panel_free(void *p) { delete (SynthPanel *)p; }
 const char *gh_synth_error(void *p) { return ((SynthPanel *)p)->err.c_str(); }
 
@@ -242,6 +484,17 @@ int gh_synth_reads(int32_t L, int32_t T, const uint8_t *haps, const float *cover
             flat[(size_t)t * L + l] = (o[0] == o[1] && o[0] == o[2]) ? 1 : 0;
         }
     }
+    return 0;
+}
+
+/* The same reads with one random stream per target (seeded by seed and the
+ * target index), spread over n_threads threads. */
+int gh_synth_reads_mt(int32_t L, int32_t T, const uint8_t *haps, const float *coverage, uint64_t seed,
+                      uint8_t *gl, uint8_t *flat, int32_t n_threads) {
+    parallel_for(T, n_threads, [&](int t) {
+        gh_synth_reads(L, 1, haps + (size_t)(2 * t) * L, coverage + t, mix64(seed, 0xBEEF0000ull + (uint64_t)t),
+                       gl + (size_t)t * L * 3, flat + (size_t)t * L);
+    });
     return 0;
 }
 

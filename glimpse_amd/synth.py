@@ -34,6 +34,10 @@ def lib():
         L.gh_synth_panel_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 11
         L.gh_synth_reads.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
+        L.gh_synth_panel_fast.restype = ctypes.c_void_p
+        L.gh_synth_panel_fast.argtypes = [ctypes.POINTER(SynthCfg), ctypes.c_int32]
+        L.gh_synth_reads_mt.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
         _lib = L
     return _lib
 
@@ -42,20 +46,27 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
+def _threads(n):
+    return int(n) if n else max(1, min(16, os.cpu_count() or 1))
+
+
 def make_panel(n_haps, n_sites, n_targets, n_founders=50, length_mb=2.0, rho_per_cm=None,
                mu=1e-3, rare_frac=0.0, flip_frac=0.05, sparse_maf=0.001, keep_dense=False,
-               seed=15052011, start_bp=1_000_000, ne=100000):
+               seed=15052011, start_bp=1_000_000, ne=100000, fast=False, threads=0):
     """Returns a dict describing one chunk panel plus 2*n_targets held-out haplotypes.
 
     rho_per_cm defaults to 0.04*Ne/F switches per cM (the reference's own
-    recombination scale with F founders)."""
+    recombination scale with F founders).  fast=True uses the founder-segment
+    generator (same model, per-site and per-haplotype random streams, threads
+    host threads; no dense matrix), which large panels need."""
     if rho_per_cm is None:
         rho_per_cm = 0.04 * ne / max(1, n_founders)
     cfg = SynthCfg(n_haps, n_sites, n_targets, n_founders, float(length_mb), start_bp,
                    float(rho_per_cm), float(mu), float(rare_frac), float(flip_frac),
                    float(sparse_maf), int(bool(keep_dense)), int(seed))
     L_ = lib()
-    h = L_.gh_synth_panel_new(ctypes.byref(cfg))
+    h = L_.gh_synth_panel_fast(ctypes.byref(cfg), _threads(threads)) if fast else \
+        L_.gh_synth_panel_new(ctypes.byref(cfg))
     try:
         err = L_.gh_synth_error(h).decode()
         if err:
@@ -86,15 +97,20 @@ def make_panel(n_haps, n_sites, n_targets, n_founders=50, length_mb=2.0, rho_per
         L_.gh_synth_panel_free(h)
 
 
-def make_reads(tgt_haps, coverage, seed=1):
-    """tgt_haps: [2T][L] uint8; coverage: scalar or [T].  Returns (gl [T][L][3], flat [T][L])."""
+def make_reads(tgt_haps, coverage, seed=1, threads=None):
+    """tgt_haps: [2T][L] uint8; coverage: scalar or [T].  Returns (gl [T][L][3], flat [T][L]).
+    threads=None: one random stream for all targets; else one stream per target
+    on that many host threads (0: all, up to 16)."""
     haps = np.ascontiguousarray(tgt_haps, dtype=np.uint8)
     T = haps.shape[0] // 2
     L = haps.shape[1]
     cov = np.broadcast_to(np.asarray(coverage, np.float32), (T,)).copy()
     gl = np.zeros((T, L, 3), np.uint8)
     flat = np.zeros((T, L), np.uint8)
-    lib().gh_synth_reads(L, T, _p(haps), _p(cov), int(seed), _p(gl), _p(flat))
+    if threads is None:
+        lib().gh_synth_reads(L, T, _p(haps), _p(cov), int(seed), _p(gl), _p(flat))
+    else:
+        lib().gh_synth_reads_mt(L, T, _p(haps), _p(cov), int(seed), _p(gl), _p(flat), _threads(threads))
     return gl, flat
 
 

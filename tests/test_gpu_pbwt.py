@@ -54,3 +54,13 @@ def test_pbwt_select_bit_exact(N, L, rare, lq, modulo, T):
         want = OP.condset(ref[t], kpbwt)["states"]
         assert np.array_equal(states[t], want), "target %d compacted state set differs" % t
     assert total > 0
+
+
+@pytest.mark.parametrize("every", [2, 5, 64])
+@pytest.mark.parametrize("N,L,rare,lq,modulo,T", [CASES[2], CASES[3]])
+def test_pbwt_select_sparse_prefix_arrays(monkeypatch, every, N, L, rare, lq, modulo, T):
+    """Prefix arrays stored only every `every` HQ common sites (what large
+    panels use): neighbour ids are found by stepping back through the columns
+    and rare-run rearrangements, and must still be bit-exact."""
+    monkeypatch.setenv("GH_PBWT_CKPT_EVERY", str(every))
+    test_pbwt_select_bit_exact(N, L, rare, lq, modulo, T)
