@@ -213,11 +213,27 @@ int gh_build_sparse_pbwt(const gh_panel_desc *d, uint8_t *ypacked, int64_t *ypac
                          int32_t *asmall_idx, int64_t *asmall_len, int32_t *n_common_hq) {
     return guard([&] {
         GH_CHECK(d && ypacked_len && asmall_len && n_common_hq, GH_ERR_ARG, "gh_build_sparse_pbwt: bad arguments");
-        std::vector<uint8_t> Y;
-        std::vector<int32_t> ap, ai;
-        int nhq = 0;
-        build_sparse_pbwt(d, Y, ap, ai, nhq);
+        // the size query keeps its result for the fill call that follows it on
+        // the same thread with the same description (the build is O(N x sites))
+        struct Built {
+            const gh_panel_desc *d = nullptr;
+            const uint64_t *words = nullptr;
+            int32_t L = 0, N = 0;
+            std::vector<uint8_t> Y;
+            std::vector<int32_t> ap, ai;
+            int nhq = 0;
+        };
+        static thread_local Built last;
         const bool fill = ypacked && asmall_ptr && asmall_idx;
+        const bool same = last.d == d && last.words == d->common_words && last.L == d->n_sites && last.N == d->n_haps;
+        if (!same) {
+            last = Built();
+            build_sparse_pbwt(d, last.Y, last.ap, last.ai, last.nhq);
+            last.d = d; last.words = d->common_words; last.L = d->n_sites; last.N = d->n_haps;
+        }
+        const std::vector<uint8_t> &Y = last.Y;
+        const std::vector<int32_t> &ap = last.ap, &ai = last.ai;
+        const int nhq = last.nhq;
         if (fill) {
             GH_CHECK(*ypacked_len >= (int64_t)Y.size() && *asmall_len >= (int64_t)ai.size(), GH_ERR_CAPACITY,
                      "gh_build_sparse_pbwt: output capacity");
@@ -228,6 +244,7 @@ int gh_build_sparse_pbwt(const gh_panel_desc *d, uint8_t *ypacked, int64_t *ypac
         *ypacked_len = (int64_t)Y.size();
         *asmall_len = (int64_t)ai.size();
         *n_common_hq = nhq;
+        if (fill) last = Built();
     });
 }
 

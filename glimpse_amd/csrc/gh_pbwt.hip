@@ -114,41 +114,50 @@ void build_sparse_pbwt(const gh_panel_desc *d, std::vector<uint8_t> &Y, std::vec
     const int N = H.N;
     n_hq = H.Lhq;
     std::vector<std::vector<int32_t>> asmall(H.Lhq + 1);
-    std::vector<int32_t> A(N), B(N), ref_idx(N), map_big_small(N, -1), sA, sB;
+    std::vector<int32_t> A(N), B(N), map_big_small(N, -1), sA, sB;
+    std::vector<uint8_t> in_small(N, 0);        // positions of the run's carriers (cleared after use)
+    std::vector<uint64_t> carrier((N + 63) / 64, 0ull);   // haplotype bitmap of the run's carriers
     std::iota(A.begin(), A.end(), 0);
-    std::iota(ref_idx.begin(), ref_idx.end(), 0);
     std::vector<uint8_t> small_hap;
+    std::vector<std::pair<int, int>> by_pos;
     Y.clear();
+    Y.reserve((size_t)H.Lc * 64 + (size_t)H.L * 4);
     int l_hq = 0, last_k = -1;
     for (int k = 0; k < H.L; k++) {
         if (H.lq[k]) continue;
         if (H.common[k]) {
             if (!sA.empty() && !(last_k >= 0 && H.common[last_k])) {
                 // rare run -> common: non-carriers keep their order, carriers follow in small order
-                std::vector<uint8_t> in_small(N, 0);
                 for (int p : asmall[l_hq]) in_small[p] = 1;
                 int nz = 0;
                 for (int p = 0; p < N; p++) if (!in_small[p]) B[nz++] = A[p];
                 for (size_t j = 0; j < sA.size(); j++) B[nz++] = A[asmall[l_hq][sA[j]]];
+                for (int p : asmall[l_hq]) in_small[p] = 0;
                 A.swap(B);
             }
-            const int row = H.crow[k];
-            int occ[2] = {0, (int)H.cref[k]};
-            int m = 0;
-            while (m < N) {
-                const int z = hbit(H, row, A[m]);
-                const int m0 = m;
-                while (++m < N && hbit(H, row, A[m]) == z) {}
-                for (int j = m0; j < m; j++) B[occ[z]++] = A[j];
-                p3add((uint8_t)z, (uint32_t)(m - m0), Y);
+            // one pass: the column in prefix order as runs of equal symbols, and
+            // the stable partition of the prefix array by allele
+            const uint64_t *row = H.words + (size_t)H.crow[k] * H.W64;
+            int occ0 = 0, occ1 = (int)H.cref[k];
+            int cur = -1;
+            uint32_t run = 0;
+            for (int m = 0; m < N; m++) {
+                const int h = A[m];
+                const int z = (int)((row[h >> 6] >> (h & 63)) & 1ull);
+                if (z != cur) {
+                    if (run) p3add((uint8_t)cur, run, Y);
+                    cur = z;
+                    run = 0;
+                }
+                run++;
+                if (z) B[occ1++] = h; else B[occ0++] = h;
             }
+            if (run) p3add((uint8_t)cur, run, Y);
             A.swap(B);
-            for (int p = 0; p < N; p++) ref_idx[A[p]] = p;
             l_hq++;
         } else {
             if (last_k < 0 || H.common[last_k]) {
                 // new rare run: carriers of any non-LQ rare site until the next non-LQ common site
-                if (l_hq > 0) for (int p : asmall[l_hq - 1]) map_big_small[p] = -1;
                 std::vector<int32_t> haps;
                 for (int kk = k; kk < H.L; kk++) {
                     if (H.lq[kk]) continue;
@@ -157,8 +166,14 @@ void build_sparse_pbwt(const gh_panel_desc *d, std::vector<uint8_t> &Y, std::vec
                 }
                 std::sort(haps.begin(), haps.end());
                 haps.erase(std::unique(haps.begin(), haps.end()), haps.end());
-                std::map<int, int> by_pos;
-                for (int h : haps) by_pos.emplace(ref_idx[h], h);
+                // their positions in the current prefix order, ascending
+                by_pos.clear();
+                for (int h : haps) carrier[h >> 6] |= 1ull << (h & 63);
+                for (int p = 0; p < N && by_pos.size() < haps.size(); p++) {
+                    const int h = A[p];
+                    if ((carrier[h >> 6] >> (h & 63)) & 1ull) by_pos.emplace_back(p, h);
+                }
+                for (int h : haps) carrier[h >> 6] = 0ull;
                 asmall[l_hq].clear();
                 int j = 0;
                 for (auto &e : by_pos) { map_big_small[e.second] = j++; asmall[l_hq].push_back(e.first); }
@@ -177,7 +192,7 @@ void build_sparse_pbwt(const gh_panel_desc *d, std::vector<uint8_t> &Y, std::vec
                 const int z = small_hap[sA[m]];
                 const int m0 = m;
                 while (++m < ns && small_hap[sA[m]] == z) {}
-                for (int j = m0; j < m; j++) sB[occ[z]++] = sA[j];
+                for (int jj = m0; jj < m; jj++) sB[occ[z]++] = sA[jj];
                 p3add((uint8_t)z, (uint32_t)(m - m0), Y);
             }
             sA.swap(sB);

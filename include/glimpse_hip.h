@@ -95,7 +95,8 @@ void gh_default_params(gh_params *p);
 
 /* split_reference producer step: ref_haplotype_set::build_sparsePBWT
  * (common/src/containers/ref_haplotype_set.cpp:47-79).  Two calls: first with
- * NULL outputs to get sizes, then with buffers. */
+ * NULL outputs to get sizes, then with buffers; the second call on the same
+ * thread with the same (unchanged) description reuses the first one's build. */
 int gh_build_sparse_pbwt(const gh_panel_desc *d, uint8_t *ypacked, int64_t *ypacked_len,
                          int32_t *asmall_ptr, int32_t *asmall_idx, int64_t *asmall_len,
                          int32_t *n_common_hq);
