@@ -29,6 +29,8 @@ void run_set_tuning(gh_run *G, int mode);
 void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w);
 void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss);
 void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode);
+void impute_batch_noalloc(gh_panel *P, int n, const int32_t *state_ptr, const int32_t *states, const float *HL,
+                          const uint8_t *flat, float err_imp, int ne, float *HP);
 void output_fields(int T, int L, const float *gp0, const float *gp1, const uint8_t *hds, const uint32_t *cref,
                    const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info, float *af, float *raf);
 
@@ -207,6 +209,11 @@ int gh_impute_batch(gh_panel *P, int32_t n, const int32_t *state_ptr, const int3
         GH_HIP(hipMemcpyAsync(HP, d_HP.p, sizeof(float) * (size_t)n * 2 * L, hipMemcpyDeviceToHost, s));
         GH_HIP(hipStreamSynchronize(s));
     });
+}
+
+int gh_impute_batch_noalloc(gh_panel *P, int32_t n, const int32_t *state_ptr, const int32_t *states, const float *HL,
+                            const uint8_t *flat, float err_imp, int32_t ne, float *HP) {
+    return guard([&] { impute_batch_noalloc(P, n, state_ptr, states, HL, flat, err_imp, ne, HP); });
 }
 
 int gh_build_sparse_pbwt(const gh_panel_desc *d, uint8_t *ypacked, int64_t *ypacked_len, int32_t *asmall_ptr,

@@ -77,6 +77,7 @@ struct gh_panel {
     std::vector<uint32_t> cref, calt;
     std::vector<int32_t> rare_ptr_h, rare_idx_h;   // carriers of the non-major allele per site
     int64_t device_bytes = 0;
+    int device = 0;                   // the HIP device the panel lives on
     hipStream_t stream = nullptr;
 
     gh::DBuf d_cm, d_cmpos, d_flags, d_crow, d_c2t, d_sm, d_hm, d_shap_ptr, d_shap_idx, d_rare_ptr,

@@ -48,6 +48,7 @@ void panel_create(const gh_panel_desc *d, gh_panel **out) {
         const int L = d->n_sites, N = d->n_haps;
         P->L = L; P->N = N;
         P->W64 = (N + 63) / 64;
+        GH_HIP(hipGetDevice(&P->device));
         GH_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
         P->cm.assign(d->cm, d->cm + L);
         P->cref.assign(d->cref, d->cref + L);

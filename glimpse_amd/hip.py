@@ -61,6 +61,7 @@ def lib():
         L.gh_panel_info.argtypes = [vp, vp, vp, vp]
         L.gh_condset_batch.argtypes = [vp, i32, vp, vp, i32, vp, vp, i64, vp, vp, i64, vp]
         L.gh_impute_batch.argtypes = [vp, i32, vp, vp, vp, vp, f32, i32, vp]
+        L.gh_impute_batch_noalloc.argtypes = [vp, i32, vp, vp, vp, vp, f32, i32, vp]
         L.gh_rephase_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp, f32, i32, u32, i32, vp]
         L.gh_pbwt_select.argtypes = [vp, i32, vp, vp, vp, i32, f32, vp, vp, vp, i64, i32, vp, vp]
         L.gh_phase_run.argtypes = [vp, i32, vp, vp, ctypes.POINTER(Params), vp, vp, vp, vp, vp, vp, vp,

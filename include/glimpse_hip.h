@@ -124,6 +124,19 @@ int gh_condset_batch(gh_panel *p, int32_t n_items, const int32_t *state_ptr, con
 int gh_impute_batch(gh_panel *p, int32_t n_items, const int32_t *state_ptr, const int32_t *states,
                     const float *HL, const uint8_t *flat, float err_imp, int32_t ne, float *HP);
 
+/* gh_impute_batch with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same arguments, checks, kernels and results as gh_impute_batch; every
+ * device buffer (inputs, conditioning sets, checkpoints, outputs) is carved
+ * from one file-scope device array of GH_NOALLOC_SCRATCH_BYTES per device,
+ * held until the process ends.  A batch that needs more returns
+ * GH_ERR_CAPACITY (split it).  The panel's device must be current.  Calls on
+ * one device share the array: calls that overlap on one device race. */
+#define GH_NOALLOC_SCRATCH_BYTES (256u << 20)
+int gh_impute_batch_noalloc(gh_panel *p, int32_t n_items, const int32_t *state_ptr, const int32_t *states,
+                            const float *HL, const uint8_t *flat, float err_imp, int32_t ne, float *HP);
+
 /* phasing_hmm::rephaseHaplotypes batched over diploid targets
  * (phase/src/models/phasing_hmm.cpp:247-302) with counter draws keyed by
  * (seed, it, target index tgt[i]).  h0/h1: [n_items][L] updated in place. */
