@@ -26,6 +26,10 @@ void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h
 void run_destroy(gh_run *G);
 void run_collect(gh_run *G, int from_iter);
 void run_set_tuning(gh_run *G, int mode);
+void workspace_create(gh_workspace **out);
+void workspace_destroy(gh_workspace *w);
+int64_t workspace_bytes(const gh_workspace *w);
+void run_set_workspace(gh_run *G, gh_workspace *w);
 void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w);
 void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t seed, float *loss);
 void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode);
@@ -451,6 +455,25 @@ int gh_mlp_train_run(gh_mlp *m, gh_run *r, int32_t epochs, int32_t batch, float 
                      float *loss_per_epoch) {
     return guard([&] { run_train(m, r, epochs, batch, lr, seed, loss_per_epoch); });
 }
+int gh_workspace_create(gh_workspace **out) {
+    return guard([&] { workspace_create(out); });
+}
+
+int gh_workspace_destroy(gh_workspace *w) {
+    return guard([&] { workspace_destroy(w); });
+}
+
+int gh_workspace_bytes(const gh_workspace *w, int64_t *bytes) {
+    return guard([&] {
+        GH_CHECK(w && bytes, GH_ERR_ARG, "gh_workspace_bytes: bad arguments");
+        *bytes = workspace_bytes(w);
+    });
+}
+
+int gh_run_set_workspace(gh_run *r, gh_workspace *w) {
+    return guard([&] { run_set_workspace(r, w); });
+}
+
 int gh_run_set_tuning(gh_run *r, int32_t mode) {
     return guard([&] { run_set_tuning(r, mode); });
 }

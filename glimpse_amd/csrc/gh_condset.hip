@@ -4,8 +4,9 @@
 //
 // Per item (one target's state set, sorted state ids):
 //   cond_index_kernel  : site classes (common / rare carried by a state / mono),
-//                        polymorphic list, site -> row map, float transitions,
-//                        rare rows (major fill, carriers flipped with atomics)
+//                        polymorphic list, site -> row map, float transitions
+//   cond_rare_kernel   : rare rows (major fill, carriers flipped with atomics),
+//                        once the item's rows have their place (hvar_off)
 //   cond_common_kernel : common rows of Hvar gathered from the haplotype-major
 //                        panel with a 64x64 ballot transpose per wave
 // Hvar row layout: ceil(K/64) uint64 words per polymorphic site, bit j of
@@ -53,7 +54,6 @@ __global__ __launch_bounds__(COND_NT) void cond_index_kernel(CondItem *__restric
                                                              const double *__restrict__ cm, double nrho,
                                                              int32_t *__restrict__ poly_all,
                                                              int32_t *__restrict__ pos_all, float *__restrict__ t_all,
-                                                             uint64_t *__restrict__ hvar_all,
                                                              uint8_t *__restrict__ mark_all) {
     const CondItem it = items[blockIdx.x];
     const int tid = threadIdx.x;
@@ -61,9 +61,8 @@ __global__ __launch_bounds__(COND_NT) void cond_index_kernel(CondItem *__restric
     int32_t *poly = poly_all + it.site_off;
     int32_t *pos = pos_all + it.site_off;
     float *t = t_all + it.site_off;
-    uint64_t *hv = hvar_all + it.hvar_off;
     const int32_t *st = states + it.state_off;
-    const int K = it.K, W = it.W;
+    const int K = it.K;
 
     for (int s = tid; s < L; s += COND_NT) mark[s] = flags[s] & SF_COMMON;
     __syncthreads();
@@ -91,6 +90,25 @@ __global__ __launch_bounds__(COND_NT) void cond_index_kernel(CondItem *__restric
         x = x < 1e-7 ? 1e-7 : (x > one_l ? one_l : x);
         t[q - 1] = (float)x;
     }
+}
+
+// Rare rows of Hvar (major allele everywhere, the states' carriers flipped);
+// runs once the item's row offset (hvar_off, room for Lp rows) is set.
+__global__ __launch_bounds__(COND_NT) void cond_rare_kernel(const CondItem *__restrict__ items,
+                                                            const int32_t *__restrict__ states,
+                                                            const uint8_t *__restrict__ flags,
+                                                            const int32_t *__restrict__ shap_ptr,
+                                                            const int32_t *__restrict__ shap_idx,
+                                                            const int32_t *__restrict__ poly_all,
+                                                            const int32_t *__restrict__ pos_all,
+                                                            uint64_t *__restrict__ hvar_all) {
+    const CondItem it = items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int32_t *poly = poly_all + it.site_off;
+    const int32_t *pos = pos_all + it.site_off;
+    uint64_t *hv = hvar_all + it.hvar_off;
+    const int32_t *st = states + it.state_off;
+    const int K = it.K, W = it.W, Lp = it.Lp;
     for (int64_t e = tid; e < (int64_t)Lp * W; e += COND_NT) {
         const int q = (int)(e / W);
         const uint8_t f = flags[poly[q]];
@@ -133,21 +151,35 @@ __global__ __launch_bounds__(64) void cond_common_kernel(const CondItem *__restr
     }
 }
 
-void condset_build(gh_panel *P, int n, CondItem *d_items, const CondItem *h_items, const int32_t *d_states,
-                   int32_t *d_poly, int32_t *d_pos, float *d_t, uint64_t *d_hvar, uint8_t *d_mark, int ne,
-                   int maxW, hipStream_t s) {
-    (void)h_items;
+void condset_index(gh_panel *P, int n, CondItem *d_items, const int32_t *d_states, int32_t *d_poly, int32_t *d_pos,
+                   float *d_t, uint8_t *d_mark, int ne, hipStream_t s) {
     if (n <= 0) return;
     const double nrho = (-0.04 * ne) / P->N;   // conditioning_set.cpp:34 (no state list)
     hipLaunchKernelGGL(cond_index_kernel, dim3(n), dim3(COND_NT), 0, s, d_items, d_states, P->L,
                        P->d_flags.as<uint8_t>(), P->d_shap_ptr.as<int32_t>(), P->d_shap_idx.as<int32_t>(),
-                       P->d_cm.as<double>(), nrho, d_poly, d_pos, d_t, d_hvar, d_mark);
+                       P->d_cm.as<double>(), nrho, d_poly, d_pos, d_t, d_mark);
+    GH_HIP(hipGetLastError());
+}
+
+void condset_fill(gh_panel *P, int n, const CondItem *d_items, const int32_t *d_states, const int32_t *d_poly,
+                  const int32_t *d_pos, uint64_t *d_hvar, int maxW, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(cond_rare_kernel, dim3(n), dim3(COND_NT), 0, s, d_items, d_states, P->d_flags.as<uint8_t>(),
+                       P->d_shap_ptr.as<int32_t>(), P->d_shap_idx.as<int32_t>(), d_poly, d_pos, d_hvar);
     GH_HIP(hipGetLastError());
     if (P->Lc > 0) {
         hipLaunchKernelGGL(cond_common_kernel, dim3(P->Wc, maxW, n), dim3(64), 0, s, d_items, d_states,
                            P->d_hm.as<uint64_t>(), P->Wc, P->Lc, P->d_c2t.as<int32_t>(), d_pos, d_hvar);
         GH_HIP(hipGetLastError());
     }
+}
+
+void condset_build(gh_panel *P, int n, CondItem *d_items, const CondItem *h_items, const int32_t *d_states,
+                   int32_t *d_poly, int32_t *d_pos, float *d_t, uint64_t *d_hvar, uint8_t *d_mark, int ne,
+                   int maxW, hipStream_t s) {
+    (void)h_items;
+    condset_index(P, n, d_items, d_states, d_poly, d_pos, d_t, d_mark, ne, s);
+    condset_fill(P, n, d_items, d_states, d_poly, d_pos, d_hvar, maxW, s);
 }
 
 }  // namespace gh

@@ -254,6 +254,12 @@ struct ShapeTuner {
     }
 };
 
+// Grow-only reservation with a quarter of headroom, so buffers placed by
+// per-iteration counts settle after the first iterations.
+static inline void grow(DBuf &b, size_t bytes) {
+    if (bytes > b.bytes) b.reserve(bytes + bytes / 4);
+}
+
 // A launch of at most this many items uses the latency launch shapes.
 constexpr int FEW_ITEMS = 32;
 
@@ -266,7 +272,19 @@ struct EvPair {
 
 }  // namespace gh
 
-// One prepared Gibbs run: inputs resident on the device, workspace allocated.
+// Device workspace of a run's execution: everything an execution builds and
+// discards (current haplotypes, likelihoods and posteriors, state sets,
+// conditioning sets, checkpoints, phasing loci and messages, selection
+// records, selector rows).  Buffers grow to the largest execution they serve
+// and are kept; runs executed one after another can share one workspace, so
+// only the executing runs hold this memory.
+struct gh_workspace {
+    gh::DBuf H, HL, HP0, HP1, states, poly, pos, t, mark, hvar, alpha, loci, seg, segsum, imp, impsum, of1, dip, ev,
+        evci, ins, isite, fx, fy, mass0, mass1, wlo, whi;
+};
+
+// One prepared Gibbs run: inputs, outputs and tables resident on the device;
+// the workspace is allocated by the first execution (or shared, gh_run_set_workspace).
 struct gh_run {
     gh_panel *P = nullptr;
     int T = 0, L = 0, N = 0, n_iter = 0, kinit = 0, kstride = 0, maxW = 0, npt = 0;
@@ -283,9 +301,11 @@ struct gh_run {
     std::vector<gh::PhItem> ph;
     std::vector<uint8_t> stored_h;
     std::vector<int32_t> start_h, nstates;   // nstates [T][n_iter]
-    gh::DBuf d_gl, d_flat, d_H, d_HL, d_HP0, d_HP1, d_states, d_K, d_items, d_fb, d_poly, d_pos, d_t, d_mark,
-        d_hvar, d_alpha, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_loci, d_seg, d_segsum, d_imp, d_impsum,
-        d_of1, d_dip, d_status, d_pst, d_pstart, d_pcls, d_pmod, d_ev, d_evci, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
+    gh::DBuf d_gl, d_flat, d_K, d_items, d_fb, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_status, d_pst,
+        d_pstart, d_pcls, d_pmod, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
+    gh_workspace own;
+    gh_workspace *W = &own;
+    hipStream_t stream = nullptr;   // the run's own stream (runs on one panel may overlap)
     gh::PbwtRun R;
     gh_run_stats st;
     bool done = false;
@@ -307,12 +327,12 @@ struct gh_run {
     int collect_from = -1;
     int64_t col_rows = 0;
     bool masks_ready = false;
-    gh::DBuf d_klearn, d_ins, d_fx, d_fy, d_cov, d_cnt, d_tmask, d_mass0, d_mass1, d_cx, d_cy, d_cw, d_isite,
-        d_sgrp, d_tcls, d_wlo, d_whi;
+    gh::DBuf d_klearn, d_cov, d_cnt, d_tmask, d_cx, d_cy, d_cw, d_sgrp, d_tcls;
     ~gh_run() {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (s2) (void)hipStreamDestroy(s2);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -326,7 +346,8 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
     GH_CHECK(prm->pbwt_depth > 0 && prm->pbwt_depth <= 64, GH_ERR_ARG, "gh_run_create: pbwt depth must be in 1..64");
     auto *G = new gh_run();
     try {
-        hipStream_t s = P->stream;
+        GH_HIP(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
+        hipStream_t s = G->stream;
         G->P = P; G->T = T; G->L = P->L; G->N = P->N; G->prm = *prm;
         const int L = P->L, N = P->N;
         G->kinit = std::min(prm->kinit, N);
@@ -379,32 +400,19 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         // ---------------- device workspace
         dalloc_copy(G->d_gl, gl_h, (size_t)T * L * 3, s);
         dalloc_copy(G->d_flat, flat_h, (size_t)T * L, s);
-        G->d_H.reserve((size_t)2 * T * L);
-        G->d_HL.reserve(sizeof(float) * (size_t)T * 2 * L);
-        G->d_HP0.reserve(sizeof(float) * (size_t)T * 2 * L);
-        G->d_HP1.reserve(sizeof(float) * (size_t)T * 2 * L);
-        G->d_states.reserve(sizeof(int32_t) * (size_t)T * G->kstride);
         G->d_K.reserve(sizeof(int32_t) * T);
         G->items.resize(T);
         G->fb.resize(T);
-        int64_t words = 0, aoff = 0;
         for (int t = 0; t < T; t++) G->maxW = std::max(G->maxW, (G->kmax[t] + 63) / 64);
-        const int C = fb_block_sites(G->maxW);
+        // Hvar rows, checkpoints and phasing loci are placed per iteration by
+        // each target's polymorphic-site count (offsets set after the index step)
         for (int t = 0; t < T; t++) {
             const int Wt = (G->kmax[t] + 63) / 64;
-            G->items[t] = CondItem{G->kmax[t], Wt, 0, 0, (int64_t)t * G->kstride, (int64_t)t * L, words};
-            words += (int64_t)L * Wt;
-            G->fb[t] = FbItem{0, 0, 0, t, aoff, (int64_t)t * 2 * L, (int64_t)t * L};
-            aoff += fb_ckpt_floats(L, Wt, G->maxW);
+            G->items[t] = CondItem{G->kmax[t], Wt, 0, 0, (int64_t)t * G->kstride, (int64_t)t * L, 0};
+            G->fb[t] = FbItem{0, 0, 0, t, 0, (int64_t)t * 2 * L, (int64_t)t * L};
         }
         dalloc_copy(G->d_items, G->items.data(), T, s);
         dalloc_copy(G->d_fb, G->fb.data(), T, s);
-        G->d_poly.reserve(sizeof(int32_t) * (size_t)T * L);
-        G->d_pos.reserve(sizeof(int32_t) * (size_t)T * L);
-        G->d_t.reserve(sizeof(float) * (size_t)T * L);
-        G->d_mark.reserve((size_t)T * L);
-        G->d_hvar.reserve(sizeof(uint64_t) * (size_t)words);
-        G->d_alpha.reserve(sizeof(float) * (size_t)aoff);
         G->d_gp0.reserve(sizeof(float) * (size_t)T * L);
         G->d_gp1.reserve(sizeof(float) * (size_t)T * L);
         G->d_stored.reserve((size_t)T * L);
@@ -412,7 +420,6 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         G->d_h0.reserve((size_t)T * L);
         G->d_h1.reserve((size_t)T * L);
         G->d_status.reserve(sizeof(int32_t) * 2);
-        G->d_loci.reserve(sizeof(Locus) * (size_t)T * L);
         G->ph.resize(T);
         for (int t = 0; t < T; t++) {
             std::memset(&G->ph[t], 0, sizeof(PhItem));
@@ -421,7 +428,7 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             G->ph[t].h0_off = (int64_t)(2 * t) * L;
             G->ph[t].h1_off = (int64_t)(2 * t + 1) * L;
             G->ph[t].flat_off = (int64_t)t * L;
-            G->ph[t].locus_off = (int64_t)t * L;
+            G->ph[t].locus_off = 0;   // placed per iteration by Lp
         }
         PbwtRun &R = G->R;
         std::memset(&R, 0, sizeof(R));
@@ -447,8 +454,6 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             R.sample_hap = dalloc_copy(G->d_psh, sh.data(), sh.size(), s);
             R.hap_row = dalloc_copy(G->d_hrow, hrow.data(), hrow.size(), s);
             R.sample_row = dalloc_copy(G->d_prow, srow.data(), srow.size(), s);
-            R.ev = (int32_t *)G->d_ev.reserve(sizeof(int32_t) * (size_t)2 * npt * R.cap * (2 + 2 * R.depth));
-            R.ev_ci = (int32_t *)G->d_evci.reserve(sizeof(int32_t) * (size_t)2 * npt * R.cap * 2);
             R.ev_cnt = (int32_t *)G->d_evcnt.reserve(sizeof(int32_t) * 2 * npt);
         }
         dalloc_copy(G->d_ptg, G->pbwt_targets.data(), G->pbwt_targets.size(), s);
@@ -469,37 +474,49 @@ void run_execute(gh_run *G) {
     GH_CHECK(G, GH_ERR_ARG, "gh_run_execute: null run");
     const auto wall0 = std::chrono::steady_clock::now();
     gh_panel *P = G->P;
-    hipStream_t s = P->stream;
+    hipStream_t s = G->stream;
     const int T = G->T, L = G->L, N = G->N, n_iter = G->n_iter, npt = G->npt;
     const gh_params *prm = &G->prm;
     const uint32_t seed = prm->seed;
+    gh_workspace *Wk = G->W;
+    const int kst = G->kstride, prs = G->pstride;
+    const bool learned = G->mlp != nullptr, collecting = G->collect_from >= 1;
+    GH_CHECK(!(learned && collecting), GH_ERR_ARG, "a run either records training rows or uses the learned selector");
+    // workspace of this execution (grows to the largest execution it serves)
+    Wk->H.reserve((size_t)2 * T * L);
+    Wk->HL.reserve(sizeof(float) * (size_t)T * 2 * L);
+    Wk->HP0.reserve(sizeof(float) * (size_t)T * 2 * L);
+    Wk->HP1.reserve(sizeof(float) * (size_t)T * 2 * L);
+    Wk->states.reserve(sizeof(int32_t) * (size_t)T * kst);
+    Wk->poly.reserve(sizeof(int32_t) * (size_t)T * L);
+    Wk->pos.reserve(sizeof(int32_t) * (size_t)T * L);
+    Wk->t.reserve(sizeof(float) * (size_t)T * L);
+    Wk->mark.reserve((size_t)T * L);
+    if (npt > 0) {
+        Wk->ev.reserve(sizeof(int32_t) * (size_t)2 * npt * G->R.cap * (2 + 2 * G->R.depth));
+        Wk->evci.reserve(sizeof(int32_t) * (size_t)2 * npt * G->R.cap * 2);
+    }
     GH_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_unphred), G->unph, sizeof(G->unph), 0, hipMemcpyHostToDevice, s));
-    uint8_t *H = G->d_H.as<uint8_t>();
-    float *HL = G->d_HL.as<float>(), *HP0 = G->d_HP0.as<float>(), *HP1 = G->d_HP1.as<float>();
-    int32_t *states = G->d_states.as<int32_t>(), *Kdev = G->d_K.as<int32_t>();
+    uint8_t *H = Wk->H.as<uint8_t>();
+    float *HL = Wk->HL.as<float>(), *HP0 = Wk->HP0.as<float>(), *HP1 = Wk->HP1.as<float>();
+    int32_t *states = Wk->states.as<int32_t>(), *Kdev = G->d_K.as<int32_t>();
     CondItem *items_d = G->d_items.as<CondItem>();
     FbItem *fb_d = G->d_fb.as<FbItem>();
-    int32_t *poly = G->d_poly.as<int32_t>(), *pos = G->d_pos.as<int32_t>();
-    float *tr = G->d_t.as<float>();
-    uint8_t *mark = G->d_mark.as<uint8_t>();
-    uint64_t *hvar = G->d_hvar.as<uint64_t>();
-    float *alpha = G->d_alpha.as<float>();
+    int32_t *poly = Wk->poly.as<int32_t>(), *pos = Wk->pos.as<int32_t>();
+    float *tr = Wk->t.as<float>();
+    uint8_t *mark = Wk->mark.as<uint8_t>();
     float *gp0 = G->d_gp0.as<float>(), *gp1 = G->d_gp1.as<float>();
     uint8_t *sto = G->d_stored.as<uint8_t>(), *hds = G->d_hds.as<uint8_t>();
     int32_t *status = G->d_status.as<int32_t>();
-    Locus *loci = G->d_loci.as<Locus>();
     const uint8_t *gl_d = G->d_gl.as<uint8_t>(), *flat_d = G->d_flat.as<uint8_t>();
     int32_t *ptg = G->d_ptg.as<int32_t>();
     uint8_t *pst = G->d_pst.as<uint8_t>();
     int32_t *pstart = G->d_pstart.as<int32_t>();
-    const bool learned = G->mlp != nullptr, collecting = G->collect_from >= 1;
-    GH_CHECK(!(learned && collecting), GH_ERR_ARG, "a run either records training rows or uses the learned selector");
-    const int kst = G->kstride, prs = G->pstride;
     if ((learned || collecting) && npt > 0) {
         GH_CHECK(feature_lds_bytes(P) <= FEAT_LDS_MAX, GH_ERR_CAPACITY,
                  "learned selection: chunk too large for the feature kernel");
         GH_CHECK(N < (1 << 20), GH_ERR_ARG, "learned selection supports at most 2^20 panel haplotypes");
-        G->d_ins.reserve(sizeof(int32_t) * (size_t)T * kst);
+        Wk->ins.reserve(sizeof(int32_t) * (size_t)T * kst);
         if (!G->masks_ready) {
             G->d_tmask.reserve(sizeof(uint64_t) * (size_t)T * 4 * std::max(P->Wc, 1));
             target_mask_launch(P, T, gl_d, flat_d, G->d_tmask.as<uint64_t>(), s);
@@ -524,25 +541,27 @@ void run_execute(gh_run *G) {
             for (int t = 0; t < T; t++) tc[t] = G->hap_cls[2 * t];
             dalloc_copy(G->d_sgrp, sg.data(), sg.size(), s);
             dalloc_copy(G->d_tcls, tc.data(), tc.size(), s);
-            G->d_isite.reserve(sizeof(int32_t) * (size_t)T * kst);
+            Wk->isite.reserve(sizeof(int32_t) * (size_t)T * kst);
         }
     }
     if (learned) {
-        G->d_fx.reserve(sizeof(float) * (size_t)npt * prs * MLP_FEAT);
-        G->d_fy.reserve(sizeof(float) * (size_t)npt * prs);
+        Wk->fx.reserve(sizeof(float) * (size_t)npt * prs * MLP_FEAT);
+        Wk->fy.reserve(sizeof(float) * (size_t)npt * prs);
         G->d_cov.reserve(sizeof(float) * (size_t)T);
     }
     if (collecting) {
-        G->d_mass0.reserve(sizeof(float) * (size_t)T * kst);
-        G->d_mass1.reserve(sizeof(float) * (size_t)T * kst);
-        G->d_wlo.reserve(sizeof(int32_t) * (size_t)T * kst);
-        G->d_whi.reserve(sizeof(int32_t) * (size_t)T * kst);
+        Wk->mass0.reserve(sizeof(float) * (size_t)T * kst);
+        Wk->mass1.reserve(sizeof(float) * (size_t)T * kst);
+        Wk->wlo.reserve(sizeof(int32_t) * (size_t)T * kst);
+        Wk->whi.reserve(sizeof(int32_t) * (size_t)T * kst);
     }
     G->col_rows = 0;
     PbwtRun R = G->R;
-    R.ins = (learned || collecting) ? G->d_ins.as<int32_t>() : nullptr;
-    R.ins_site = (learned || collecting) ? G->d_isite.as<int32_t>() : nullptr;
+    R.ins = (learned || collecting) ? Wk->ins.as<int32_t>() : nullptr;
+    R.ins_site = (learned || collecting) ? Wk->isite.as<int32_t>() : nullptr;
     R.haps = H;
+    R.ev = Wk->ev.as<int32_t>();
+    R.ev_ci = Wk->evci.as<int32_t>();
     R.status = status + 1;
     R.out_states = states;
     R.out_stride = G->kstride;
@@ -621,12 +640,12 @@ void run_execute(gh_run *G) {
                 fa.hm = P->d_hm.as<uint64_t>(); fa.Wc = P->Wc; fa.Lc = P->Lc; fa.N = N; fa.L = L;
                 fa.depth = prm->pbwt_depth; fa.kstride = kst; fa.rstride = prs;
                 fa.c2t = P->d_c2t.as<int32_t>(); fa.crow = P->d_crow.as<int32_t>(); fa.flags = P->d_flags.as<uint8_t>();
-                fa.ins_site = G->d_isite.as<int32_t>();
+                fa.ins_site = Wk->isite.as<int32_t>();
                 fa.rare_ptr = P->d_rare_ptr.as<int32_t>(); fa.rare_idx = P->d_rare_idx.as<int32_t>();
                 fa.H = H; fa.tmask = G->d_tmask.as<uint64_t>(); fa.targets = ptg; fa.Kpool = Kdev;
-                fa.ins = G->d_ins.as<int32_t>();
+                fa.ins = Wk->ins.as<int32_t>();
                 if (learned) {
-                    fa.X = G->d_fx.as<float>();
+                    fa.X = Wk->fx.as<float>();
                     fa.row0 = 0;
                 } else {
                     const int64_t need = G->col_rows + (int64_t)npt * prs;
@@ -644,22 +663,22 @@ void run_execute(gh_run *G) {
                 feature_launch(P, npt, fa, s);
                 if (col_it) {
                     WinArgs wa;
-                    wa.targets = ptg; wa.Kpool = Kdev; wa.states = states; wa.ins = G->d_ins.as<int32_t>();
-                    wa.ins_site = G->d_isite.as<int32_t>(); wa.c2t = P->d_c2t.as<int32_t>();
+                    wa.targets = ptg; wa.Kpool = Kdev; wa.states = states; wa.ins = Wk->ins.as<int32_t>();
+                    wa.ins_site = Wk->isite.as<int32_t>(); wa.c2t = P->d_c2t.as<int32_t>();
                     wa.crow = P->d_crow.as<int32_t>(); wa.Lc = P->Lc; wa.Wc = P->Wc; wa.kstride = kst;
-                    wa.win_lo = G->d_wlo.as<int32_t>(); wa.win_hi = G->d_whi.as<int32_t>();
+                    wa.win_lo = Wk->wlo.as<int32_t>(); wa.win_hi = Wk->whi.as<int32_t>();
                     win_launch(npt, wa, s);
                 }
                 if (learned) {
-                    mlp_forward_launch(G->mlp->w.as<float>(), G->mlp->wb.as<uint16_t>(), G->d_fx.as<float>(),
-                                       (int64_t)npt * prs, G->d_fy.as<float>(), s);
+                    mlp_forward_launch(G->mlp->w.as<float>(), G->mlp->wb.as<uint16_t>(), Wk->fx.as<float>(),
+                                       (int64_t)npt * prs, Wk->fy.as<float>(), s);
                     TopkArgs ta;
                     ta.targets = ptg; ta.Kpool = Kdev; ta.klearn = G->d_klearn.as<int32_t>();
-                    ta.y = G->d_fy.as<float>(); ta.row0 = 0; ta.kstride = kst; ta.rstride = prs;
+                    ta.y = Wk->fy.as<float>(); ta.row0 = 0; ta.kstride = kst; ta.rstride = prs;
                     ta.min_mass = G->min_mass;
-                    ta.ins = G->d_ins.as<int32_t>(); ta.states = states; ta.Kdev = Kdev; ta.items = items_d;
+                    ta.ins = Wk->ins.as<int32_t>(); ta.states = states; ta.Kdev = Kdev; ta.items = items_d;
                     ta.coverage = G->d_cov.as<float>(); ta.counters = G->d_cnt.as<int32_t>();
-                    ta.ins_site = G->sel_mode == 1 ? G->d_isite.as<int32_t>() : nullptr;
+                    ta.ins_site = G->sel_mode == 1 ? Wk->isite.as<int32_t>() : nullptr;
                     ta.site_grp = G->d_sgrp.as<int32_t>(); ta.tcls = G->d_tcls.as<int32_t>(); ta.L = L;
                     topk_launch(npt, ta, s);
                 }
@@ -667,23 +686,40 @@ void run_execute(gh_run *G) {
                 learn_ev = true;
             }
         }
-        int maxW_it = G->maxW;
-        if (learned && stage > 0) {
-            // the selection shrank the sets on the device: size this iteration's launches to them
-            GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
-            GH_HIP(hipStreamSynchronize(s));
-            maxW_it = 1;
-            for (int t = 0; t < T; t++) maxW_it = std::max(maxW_it, items[t].W);
-        }
         GH_HIP(hipEventRecord(e_sel.b, s));
         GH_HIP(hipEventRecord(e_cond.a, s));
-        condset_build(P, T, items_d, items.data(), states, poly, pos, tr, hvar, mark, prm->ne, maxW_it, s);
+        // Conditioning sets.  The index step gives each item's polymorphic-site
+        // count Lp; its Hvar rows, alpha checkpoints and phasing loci are placed
+        // by Lp (not by the chunk's L), then the rows are filled.
+        condset_index(P, T, items_d, states, poly, pos, tr, mark, prm->ne, s);
+        GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+        int maxW_it = 1;
+        {
+            int64_t words = 0, aoff = 0, loff = 0;
+            for (int t = 0; t < T; t++) {
+                maxW_it = std::max(maxW_it, items[t].W);
+                items[t].hvar_off = words;
+                words += (int64_t)items[t].Lp * items[t].W;
+                G->fb[t].alpha_off = aoff;
+                aoff += fb_ckpt_floats(items[t].Lp, items[t].W, G->maxW);
+                ph[t].locus_off = loff;
+                loff += items[t].Lp;
+            }
+            grow(Wk->hvar, sizeof(uint64_t) * (size_t)std::max<int64_t>(words, 1));
+            grow(Wk->alpha, sizeof(float) * (size_t)std::max<int64_t>(aoff, 1));
+            grow(Wk->loci, sizeof(Locus) * (size_t)std::max<int64_t>(loff, 1));
+        }
+        uint64_t *hvar = Wk->hvar.as<uint64_t>();
+        float *alpha = Wk->alpha.as<float>();
+        Locus *loci = Wk->loci.as<Locus>();
+        GH_HIP(hipMemcpyAsync(items_d, items.data(), sizeof(CondItem) * T, hipMemcpyHostToDevice, s));
+        GH_HIP(hipMemcpyAsync(fb_d, G->fb.data(), sizeof(FbItem) * T, hipMemcpyHostToDevice, s));
+        condset_fill(P, T, items_d, states, poly, pos, hvar, maxW_it, s);
         // K-class batching (not while tuning): items of the largest word-count
         // class first in G->ford, the rest after them
         int fbA = T, fbB = 0, fbWA = maxW_it, fbWB = 0;
         if (!G->tune) {
-            GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
-            GH_HIP(hipStreamSynchronize(s));
             auto capw = [](int w) { return w <= 4 ? 4 : w <= 8 ? 8 : w <= 16 ? 16 : 32; };
             int top = 0;
             for (int t = 0; t < T; t++) top = std::max(top, capw(items[t].W));
@@ -706,8 +742,8 @@ void run_execute(gh_run *G) {
                            : (G->latency || fewA) ? fb_default_pick(maxW_it, true)
                                                   : FbPick{0, 0};
         const bool label_it = collecting && stage > 0 && npt > 0 && it >= G->collect_from;
-        float *mass0 = label_it ? G->d_mass0.as<float>() : nullptr, *mass1 = label_it ? G->d_mass1.as<float>() : nullptr;
-        const int32_t *wlo = label_it ? G->d_wlo.as<int32_t>() : nullptr, *whi = label_it ? G->d_whi.as<int32_t>() : nullptr;
+        float *mass0 = label_it ? Wk->mass0.as<float>() : nullptr, *mass1 = label_it ? Wk->mass1.as<float>() : nullptr;
+        const int32_t *wlo = label_it ? Wk->wlo.as<int32_t>() : nullptr, *whi = label_it ? Wk->whi.as<int32_t>() : nullptr;
         GH_HIP(hipEventRecord(e_cond.b, s));
         GH_HIP(hipEventRecord(e_fb.a, s));
         // one forward-backward pass over every item: the top class on s, the rest
@@ -756,7 +792,7 @@ void run_execute(gh_run *G) {
         GH_HIP(hipEventRecord(e_fb.b, s));
         if (label_it) {
             LabelArgs la;
-            la.targets = ptg; la.items = items_d; la.states = states; la.ins = G->d_ins.as<int32_t>();
+            la.targets = ptg; la.items = items_d; la.states = states; la.ins = Wk->ins.as<int32_t>();
             la.poly = poly; la.win_lo = wlo; la.win_hi = whi;
             la.mass0 = mass0; la.mass1 = mass1; la.kstride = kst; la.rstride = prs;
             la.X = G->d_cx.as<float>(); la.Y = G->d_cy.as<float>(); la.W = G->d_cw.as<float>();
@@ -808,12 +844,18 @@ void run_execute(gh_run *G) {
         }
         GH_HIP(hipMemcpyAsync(ph_d, ph.data(), sizeof(PhItem) * T, hipMemcpyHostToDevice, s));
         PhaseBufs PB;
-        PB.seg = (float *)G->d_seg.reserve(sizeof(float) * std::max<int64_t>(so, 1));
-        PB.segsum = (float *)G->d_segsum.reserve(sizeof(float) * std::max<int64_t>(ss, 1));
-        PB.imp = (float *)G->d_imp.reserve(sizeof(float) * std::max<int64_t>(io, 1));
-        PB.impsum = (float *)G->d_impsum.reserve(sizeof(float) * std::max<int64_t>(is, 1));
-        PB.of1 = (float *)G->d_of1.reserve(sizeof(float) * std::max<int64_t>(is, 1));
-        PB.dip = (int32_t *)G->d_dip.reserve(sizeof(int32_t) * std::max<int64_t>(dd, 1));
+        grow(Wk->seg, sizeof(float) * std::max<int64_t>(so, 1));
+        grow(Wk->segsum, sizeof(float) * std::max<int64_t>(ss, 1));
+        grow(Wk->imp, sizeof(float) * std::max<int64_t>(io, 1));
+        grow(Wk->impsum, sizeof(float) * std::max<int64_t>(is, 1));
+        grow(Wk->of1, sizeof(float) * std::max<int64_t>(is, 1));
+        grow(Wk->dip, sizeof(int32_t) * std::max<int64_t>(dd, 1));
+        PB.seg = Wk->seg.as<float>();
+        PB.segsum = Wk->segsum.as<float>();
+        PB.imp = Wk->imp.as<float>();
+        PB.impsum = Wk->impsum.as<float>();
+        PB.of1 = Wk->of1.as<float>();
+        PB.dip = Wk->dip.as<int32_t>();
         GH_HIP(hipEventRecord(e_pk.a, s));
         if (phB == 0) {
             phase_launch(P, T, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
@@ -892,7 +934,7 @@ void run_collect(gh_run *G, int from_iter) {
 void run_collected(gh_run *G, int64_t *n_rows, float *X, float *y, float *w) {
     GH_CHECK(G && n_rows, GH_ERR_ARG, "gh_run_collected: bad arguments");
     *n_rows = G->col_rows;
-    hipStream_t s = G->P->stream;
+    hipStream_t s = G->stream;
     const size_t n = (size_t)G->col_rows;
     if (n == 0) return;
     if (X) GH_HIP(hipMemcpyAsync(X, G->d_cx.p, sizeof(float) * n * MLP_FEAT, hipMemcpyDeviceToHost, s));
@@ -906,7 +948,7 @@ void run_train(gh_mlp *M, gh_run *G, int epochs, int batch, float lr, uint32_t s
     GH_CHECK(G->col_rows > 0, GH_ERR_STATE, "gh_mlp_train_run: the run recorded no training rows");
     GH_CHECK(G->mlp != M, GH_ERR_ARG, "gh_mlp_train_run: the network is this run's selector");
     mlp_train_device(M, G->d_cx.as<float>(), G->d_cy.as<float>(), G->d_cw.as<float>(), G->col_rows, epochs, batch,
-                     lr, seed, G->P->stream, loss);
+                     lr, seed, G->stream, loss);
 }
 
 void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mass, int mode) {
@@ -918,10 +960,32 @@ void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mas
     GH_CHECK(G->collect_from < 1, GH_ERR_ARG, "gh_run_set_selector: the run records training rows");
     GH_CHECK(min_mass >= 0.0f && min_mass <= 1.0f, GH_ERR_ARG, "gh_run_set_selector: min_mass must be in [0, 1]");
     for (int t = 0; t < G->T; t++) GH_CHECK(klearn[t] > 0, GH_ERR_ARG, "gh_run_set_selector: k_learn must be > 0");
-    dalloc_copy(G->d_klearn, klearn, (size_t)G->T, G->P->stream);
-    GH_HIP(hipStreamSynchronize(G->P->stream));
+    dalloc_copy(G->d_klearn, klearn, (size_t)G->T, G->stream);
+    GH_HIP(hipStreamSynchronize(G->stream));
     G->mlp = M;
     G->min_mass = min_mass;
+}
+
+void workspace_create(gh_workspace **out) {
+    GH_CHECK(out, GH_ERR_ARG, "gh_workspace_create: null output");
+    *out = new gh_workspace();
+}
+
+void workspace_destroy(gh_workspace *w) { delete w; }
+
+int64_t workspace_bytes(const gh_workspace *w) {
+    const gh::DBuf *b[] = {&w->H, &w->HL, &w->HP0, &w->HP1, &w->states, &w->poly, &w->pos, &w->t, &w->mark,
+                           &w->hvar, &w->alpha, &w->loci, &w->seg, &w->segsum, &w->imp, &w->impsum, &w->of1,
+                           &w->dip, &w->ev, &w->evci, &w->ins, &w->isite, &w->fx, &w->fy, &w->mass0, &w->mass1,
+                           &w->wlo, &w->whi};
+    int64_t n = 0;
+    for (const gh::DBuf *x : b) n += (int64_t)x->bytes;
+    return n;
+}
+
+void run_set_workspace(gh_run *G, gh_workspace *w) {
+    GH_CHECK(G, GH_ERR_ARG, "gh_run_set_workspace: null run");
+    G->W = w ? w : &G->own;
 }
 
 void run_set_tuning(gh_run *G, int mode) {
@@ -934,7 +998,7 @@ void run_set_tuning(gh_run *G, int mode) {
 void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h0_h, uint8_t *h1_h,
                int32_t *nstates_h, gh_run_stats *stats) {
     GH_CHECK(G && G->done, GH_ERR_ARG, "gh_run_fetch: the run has not been executed");
-    hipStream_t s = G->P->stream;
+    hipStream_t s = G->stream;
     const size_t TL = (size_t)G->T * G->L;
     if (gp0_h) GH_HIP(hipMemcpyAsync(gp0_h, G->d_gp0.p, sizeof(float) * TL, hipMemcpyDeviceToHost, s));
     if (gp1_h) GH_HIP(hipMemcpyAsync(gp1_h, G->d_gp1.p, sizeof(float) * TL, hipMemcpyDeviceToHost, s));

@@ -106,6 +106,13 @@ struct CondItem {
 // items/poly/hvar/t are device arrays sized by the caller:
 //   poly, pos: L per item; t: L per item; hvar: L*W per item (words).
 // Returns with Lp of each item written to items[i].Lp (device).
+// The same in two steps, for callers that place each item's rows by its Lp:
+// condset_index writes Lp (and poly, pos, t), condset_fill the Hvar rows at
+// the items' hvar_off.
+void condset_index(gh_panel *P, int n, CondItem *d_items, const int32_t *d_states, int32_t *d_poly, int32_t *d_pos,
+                   float *d_t, uint8_t *d_mark, int ne, hipStream_t s);
+void condset_fill(gh_panel *P, int n, const CondItem *d_items, const int32_t *d_states, const int32_t *d_poly,
+                  const int32_t *d_pos, uint64_t *d_hvar, int maxW, hipStream_t s);
 void condset_build(gh_panel *P, int n, CondItem *d_items, const CondItem *h_items, const int32_t *d_states,
                    int32_t *d_poly, int32_t *d_pos, float *d_t, uint64_t *d_hvar, uint8_t *d_mark,
                    int ne, int maxW, hipStream_t s);

@@ -70,6 +70,10 @@ def lib():
         L.gh_run_execute.argtypes = [vp]
         L.gh_run_fetch.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(RunStats)]
         L.gh_run_destroy.argtypes = [vp]
+        L.gh_workspace_create.argtypes = [ctypes.POINTER(vp)]
+        L.gh_workspace_destroy.argtypes = [vp]
+        L.gh_workspace_bytes.argtypes = [vp, vp]
+        L.gh_run_set_workspace.argtypes = [vp, vp]
         L.gh_mlp_create.argtypes = [u32, ctypes.POINTER(vp)]
         L.gh_mlp_destroy.argtypes = [vp]
         L.gh_mlp_get_params.argtypes = [vp, vp]
@@ -307,6 +311,32 @@ def select_topk(scores, pool, K, k_learn, states, min_mass=0.95):
     return st, Ko, cov
 
 
+class Workspace:
+    """Device workspace shared by runs that execute one after another (gh_workspace_*)."""
+
+    def __init__(self):
+        h = vp()
+        _check(lib().gh_workspace_create(ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def bytes(self):
+        n = ctypes.c_int64()
+        _check(lib().gh_workspace_bytes(self.h, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().gh_workspace_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Run:
     """A prepared Gibbs run (gh_run_*): inputs resident on the device.
     execute() runs the whole INIT + burn-in + main schedule on the GPU and
@@ -336,6 +366,12 @@ class Run:
         defaults for the run on a batch's critical path."""
         m = 2 if mode == "latency" else int(mode)
         _check(lib().gh_run_set_tuning(self.h, m))
+
+    def set_workspace(self, ws):
+        """Execute in a shared Workspace (None: the run's own); runs sharing one
+        must not execute at the same time."""
+        _check(lib().gh_run_set_workspace(self.h, ws.h if ws is not None else None))
+        self._ws = ws
 
     def collect(self, from_iter=1):
         """Record (features, label, weight) rows for every PBWT pool candidate at

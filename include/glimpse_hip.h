@@ -168,7 +168,8 @@ int gh_phase_run(gh_panel *p, int32_t T, const uint8_t *gl, const uint8_t *flat,
 
 /* The same run split in three so a caller can time or repeat the device part
  * with its inputs resident: create uploads the inputs and allocates the
- * workspace (everything gh_phase_run allocates), execute runs the full Gibbs
+ * outputs (the workspace comes with the first execute, see gh_workspace
+ * below), execute runs the full Gibbs
  * schedule on the device (a fresh chain each call, same seed, same result),
  * fetch copies the outputs back.  These have no single reference counterpart:
  * together they are caller::phase_loop split at its I/O boundaries
@@ -180,6 +181,22 @@ int gh_run_execute(gh_run *r);
 int gh_run_fetch(gh_run *r, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1, int32_t *nstates,
                  gh_run_stats *stats);
 int gh_run_destroy(gh_run *r);
+/* Execution workspace (no reference counterpart: memory management for
+ * batches).  A run's execution builds and discards device buffers: the
+ * current haplotypes, likelihoods and posteriors, state sets, conditioning
+ * sets (placed by each target's polymorphic-site count), alpha checkpoints,
+ * phasing loci and messages, selection records.  By default each run keeps
+ * its own, allocated by its first execution and grown as needed.  Runs that
+ * execute one after another (never at the same time) can share one
+ * workspace, so a batch of many runs holds only the workspaces of the runs
+ * executing at once; the run's inputs and outputs stay its own.  The
+ * workspace must outlive the runs that use it. */
+typedef struct gh_workspace gh_workspace;
+int gh_workspace_create(gh_workspace **out);
+int gh_workspace_destroy(gh_workspace *w);
+int gh_workspace_bytes(const gh_workspace *w, int64_t *bytes);
+/* NULL: the run's own workspace again. */
+int gh_run_set_workspace(gh_run *r, gh_workspace *w);
 /* Launch shapes (waves per target and work per lane of the forward-backward
  * and phasing kernels).  mode 0 (the default): throughput defaults, the
  * fewest waves per target that fill a CU, right when many runs share the GPU.

@@ -78,3 +78,30 @@ def test_mixed_k_run_matches_oracle(case, mode):
     assert np.array_equal(got["nstates"], ref["nstates"]), "conditioning-set sizes differ"
     d = np.abs(got["gp0"] - ref["gp0"]) + np.abs(got["gp1"] - ref["gp1"])
     assert np.mean(d < 1e-4) > 0.995, "GP differs at %.4f of sites" % np.mean(d >= 1e-4)
+
+
+def test_runs_sharing_a_workspace(case):
+    """Runs executed one after another in one shared workspace (a larger run on
+    another panel first, then the case's run, then the larger one again) give
+    exactly the results of runs with their own workspaces."""
+    from glimpse_amd import hip
+    P, gl, flat, OP, GP = case
+    kw = dict(burnin=1, main_iter=2, kpbwt=300, kinit=600, seed=5)
+    P2 = _with_lq(synth.make_panel(1500, 4000, 10, length_mb=0.8, seed=33), 7, 3)
+    gl2, flat2 = synth.make_reads(P2["tgt_haps"], 1.5, seed=34)
+    GP2 = hip.Panel(P2)
+    own1 = GP.phase_run(gl, flat, hip.default_params(**kw))
+    own2 = GP2.phase_run(gl2, flat2, hip.default_params(**kw))
+    ws = hip.Workspace()
+    runs = [(hip.Run(GP2, gl2, flat2, hip.default_params(**kw)), own2),
+            (hip.Run(GP, gl, flat, hip.default_params(**kw)), own1)]
+    for r, _ in runs:
+        r.set_workspace(ws)
+    for r, own in runs + runs[:1]:
+        r.execute()
+        got = r.fetch()
+        for k in ("gp0", "gp1", "h0", "h1", "nstates"):
+            assert np.array_equal(got[k], own[k]), k
+    assert ws.bytes > 0
+    for r, _ in runs:
+        r.close()
