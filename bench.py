@@ -1,54 +1,69 @@
 #!/usr/bin/env python
 """bench.py -- GLIMPSE2_phase compute core on one MI355X.
 
-Workload (the batched configuration): a synthetic panel of 10,000 haplotypes x
-60,000 biallelic SNPs (5,000 SNPs per Mb at 1 cM/Mb, 50-founder Li-Stephens
-mosaics, 1/x frequency spectrum, mutation 1e-3), 200 diploid targets with read
-coverage U(0.1, 2)x, and 5 chunks of U[20k, 60k] sites (windows of the panel,
-each with its own sparse PBWT); every chunk is phased for all 200 targets,
-Kpbwt per target U[500, 2000], default Gibbs schedule (1 INIT + 5 burn-in + 15
-main).  One step = one complete phase run of every chunk (all targets, all
-iterations) with the inputs already resident in HBM; chunks run concurrently
-on their own HIP streams.
+Default workload: the mixed production batch (C5).  A reference panel of
+100,000 haplotypes split into 10 chunks of U[40k, 150k] biallelic SNPs
+(about 6,700 SNPs per cM, 75% of candidate sites injected rare and the 1/x
+spectrum adding more, so about 80% of sites have MAF < 0.1%; 50-founder
+Li-Stephens mosaics, mutation 1e-3), 1,000 diploid targets with read coverage
+log-uniform in 0.1-4x; every chunk is phased for all 1,000 targets with the
+default Gibbs schedule (1 INIT + 5 burn-in + 15 main), Kpbwt 1,000.  The
+chunks are independent synthetic regions (one panel, one sparse PBWT and one
+set of held-out target haplotypes each).
 
-Two paths over the same inputs:
+One step = the whole job: every (chunk, block of 250 targets) unit runs one
+complete phase run (all iterations) with its inputs already resident in HBM.
+The 40 units are served longest chunk first by 4 host threads, each executing
+its units one after another in its own device workspace, so 4 runs (1,000
+Gibbs chains) are in flight at a time and only their workspaces are held.
+Before the timed step each worker executes its largest unit once on both
+paths (outputs discarded) so the workspaces are allocated; --warmup adds full
+untimed steps.
+
+Two paths over the same resident inputs:
   plain PBWT  -- every target conditions on its Kpbwt PBWT states;
   learned     -- from iteration 1 on, the selector network keeps
-                 min(round(0.75 Kpbwt), 1024) states of each target's PBWT
-                 pool, falling back to the whole pool when the kept states'
-                 predicted mass is below 0.95 (INIT is unchanged).
-The network is first trained on the GPU (outside the timed region) on rows a
-plain run records: a separate panel of 4,000 haplotypes x 10,000 SNPs (2 Mb),
-48 targets at 1x, Kpbwt 2,000, every PBWT iteration; labels are the HMM's own
-posterior masses.
+                 round(keep x Kpbwt) states of each target's PBWT pool, or the
+                 whole pool when the kept states' predicted mass is below
+                 0.95 (INIT is unchanged).
+The network is trained on the GPU (outside the timed region) on rows a plain
+run records: 4,000 haplotypes x 10,000 SNPs (2 Mb), 48 targets at 1x, Kpbwt
+2,000; labels are the HMM's own posterior masses.
 
 metric value: target-site imputations/s = sum over chunks of targets x chunk
 sites / wall time of one step, for the learned path when its dosage r2 is
-within 0.001 of the plain path's (the "equal r2" condition), else for the
-plain path (config.path says which).  Both paths' figures, dosage r2 / NRC
-against the simulated truth, and the learned path's mean |DS - DS(plain)| (its
-dosage error measured against the plain run at Kpbwt) are in the same line.
-The roofline object reports the dominant kernel (haploid forward-backward)
-against HBM: algorithmic bytes per launch / average launch duration (HIP
-events on the launch stream); `traffic` is the HBM bytes per launch measured
-with rocprofv3 counters for this workload
-(profiles/fb_kernel_batch200x5_<path>_traffic.md).
-cpu_baseline: the CPU oracle on a bounded sample of the same workload (targets
-of the smallest chunk, INIT + 1 burn-in + 1 main iteration each, scaled to the
-21 iterations of a full run), one target per thread on up to 16 host threads,
-with the one-core figure beside it.
+within 0.001 of the plain path's (equal r2), else for the plain path
+(config.path says which).  Both paths' figures, per-chunk throughput, dosage
+r2 / NRC against the simulated truth, and the learned path's distance to the
+plain run are in the same line.
+
+roofline: the dominant kernel (haploid forward-backward) against HBM,
+algorithmic bytes per dispatch / average dispatch duration (HIP events on the
+launch stream), measured on one unit running alone (the largest chunk's first
+block, plain path: 500 haplotypes per dispatch fill the CUs), so no other
+dispatch shares its time; `step_level_frac` is the whole step's algorithmic
+forward-backward bytes over the step time; `traffic` is the measured HBM bytes
+per dispatch from the committed rocprofv3 counter summary of the same probe.
+cpu_baseline: the CPU oracle (a C++ restatement of the reference path) timed
+on this host on a bounded sample of the same workload: full 1+5+15 runs of as
+many targets as threads (up to 16) on the smallest chunk, the PBWT sweep of
+an iteration serial and the targets spread over the threads as the
+reference's --threads does.
+
+--workload c2 runs the batched configuration instead (10,000 haplotypes x
+60,000 SNPs, 200 targets at U(0.1, 2)x, 5 chunks of U[20k, 60k] sites, Kpbwt
+U[500, 2000], every chunk one unit, all 5 in flight).
 """
 import argparse
 import json
 import os
+import queue
 import sys
 import threading
 import time
 
-# One hardware queue per concurrently running stream (HIP's default is 4; with
-# 5 chunks, each with a second stream for its smaller set-size class, streams
-# would share queues and run one after the other).
-# Must be set before the HIP runtime starts.
+# Two HIP streams per run in flight (K-class batching); HIP's default of 4
+# hardware queues would serialise them.  Must be set before the runtime starts.
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
     os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
@@ -61,6 +76,35 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "target-site imputations/s at equal dosage r² (learned vs PBWT K); HBM %"
 
 
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ workloads
+def build_c5(quick, seed):
+    """C5 chunks: independent synthetic regions of one 100k-haplotype panel."""
+    from glimpse_amd import synth
+    if quick:
+        N, T, n_chunks, lo, hi, block = 20000, 200, 3, 10000, 30000, 100
+    else:
+        N, T, n_chunks, lo, hi, block = 100000, 1000, 10, 40000, 150000, 250
+    rng = np.random.default_rng(seed)
+    lens = [int(x) for x in rng.integers(lo, hi + 1, n_chunks)]
+    cov = np.exp(rng.uniform(np.log(0.1), np.log(4.0), T))
+    chunks = []
+    for c, L in enumerate(lens):
+        P = synth.make_panel(N, L, T, length_mb=L / 4167.0, rare_frac=0.75, seed=seed + 101 * (c + 1), fast=True)
+        gl, flat = synth.make_reads(P["tgt_haps"], cov, seed=seed + 103 * (c + 1), threads=0)
+        truth = synth.truth_genotypes(P["tgt_haps"])
+        del P["tgt_haps"]
+        chunks.append(dict(panel=P, gl=gl, flat=flat, truth=truth, K=np.full(T, 1000, np.int32), L=L))
+        log("chunk %d: %d sites, %.1f cM, common %.3f" % (c, L, P["cm"][-1], P["common"].mean()))
+    desc = ("C5 mixed production batch: %d haplotypes, %d chunks of U[%d,%d] sites (rare-heavy, ~80%% MAF<0.1%%), "
+            "%d targets at log-uniform 0.1-4x, Kpbwt 1000, 1+5+15 Gibbs iterations" % (N, n_chunks, lo, hi, T))
+    meta = dict(N=N, targets=T, chunks=n_chunks, chunk_sites=lens, kpbwt=1000, block=block)
+    return chunks, desc, meta
+
+
 def slice_panel(P, o, l):
     """Window [o, o+l) of a panel dict as its own chunk panel (cM rebased)."""
     s = slice(o, o + l)
@@ -68,15 +112,14 @@ def slice_panel(P, o, l):
     crow0 = int(P["common"][:o].sum())
     nc = int(common.sum())
     rp = P["rare_ptr"][o:o + l + 1]
-    sub = dict(L=l, N=P["N"], bp=P["bp"][s].copy(), cm=P["cm"][s] - P["cm"][o],
-               cref=P["cref"][s].copy(), calt=P["calt"][s].copy(), common=common.copy(),
-               major=P["major"][s].copy(), common_words=P["common_words"][crow0:crow0 + nc].copy(),
-               rare_ptr=(rp - rp[0]).astype(np.int32), rare_idx=P["rare_idx"][rp[0]:rp[-1]].copy(),
-               lq=np.zeros(l, np.uint8))
-    return sub
+    return dict(L=l, N=P["N"], bp=P["bp"][s].copy(), cm=P["cm"][s] - P["cm"][o],
+                cref=P["cref"][s].copy(), calt=P["calt"][s].copy(), common=common.copy(),
+                major=P["major"][s].copy(), common_words=P["common_words"][crow0:crow0 + nc].copy(),
+                rare_ptr=(rp - rp[0]).astype(np.int32), rare_idx=P["rare_idx"][rp[0]:rp[-1]].copy(),
+                lq=np.zeros(l, np.uint8))
 
 
-def build_workload(quick, seed=15052011):
+def build_c2(quick, seed):
     from glimpse_amd import synth
     if quick:
         N, Lp, T, n_chunks, lo, hi, kmin, kmax = 2000, 12000, 24, 3, 4000, 12000, 200, 600
@@ -85,77 +128,87 @@ def build_workload(quick, seed=15052011):
     P = synth.make_panel(N, Lp, T, length_mb=Lp / 5000.0, seed=seed)
     rng = np.random.default_rng(seed)
     cov = rng.uniform(0.1, 2.0, T)
-    K = rng.integers(kmin, kmax + 1, T)
-    lens = rng.integers(lo, hi + 1, n_chunks)
+    K = rng.integers(kmin, kmax + 1, T).astype(np.int32)
+    lens = [int(x) for x in rng.integers(lo, hi + 1, n_chunks)]
     offs = [int(rng.integers(0, Lp - ln + 1)) for ln in lens]
     gl, flat = synth.make_reads(P["tgt_haps"], cov, seed=seed + 1)
     truth = synth.truth_genotypes(P["tgt_haps"])
     chunks = []
-    for c in range(n_chunks):
-        tg = np.arange(T)   # every chunk (a genomic region) is phased for every target sample
-        o, ln = offs[c], int(lens[c])
-        chunks.append(dict(panel=slice_panel(P, o, ln), targets=tg,
-                           gl=np.ascontiguousarray(gl[tg][:, o:o + ln]),
-                           flat=np.ascontiguousarray(flat[tg][:, o:o + ln]),
-                           K=K[tg].astype(np.int32), truth=truth[tg][:, o:o + ln]))
-    desc = ("batched plain PBWT: %d haps x %d-site panel, %d targets at U(0.1,2)x, each phased on %d chunks of "
-            "U[%d,%d] sites, Kpbwt U[%d,%d] per target, 1+5+15 Gibbs iterations" % (N, Lp, T, n_chunks, lo, hi, kmin, kmax))
-    return chunks, desc, dict(N=N, panel_sites=Lp, targets=T, chunks=n_chunks, chunk_sites=[int(x) for x in lens])
+    for o, ln in zip(offs, lens):
+        chunks.append(dict(panel=slice_panel(P, o, ln), gl=np.ascontiguousarray(gl[:, o:o + ln]),
+                           flat=np.ascontiguousarray(flat[:, o:o + ln]), K=K, truth=truth[:, o:o + ln], L=ln))
+    desc = ("C2 batched: %d haps x %d-site panel, %d targets at U(0.1,2)x, each phased on %d chunks of "
+            "U[%d,%d] sites, Kpbwt U[%d,%d] per target, 1+5+15 Gibbs iterations" % (N, Lp, T, n_chunks, lo, hi,
+                                                                                      kmin, kmax))
+    meta = dict(N=N, targets=T, chunks=n_chunks, chunk_sites=lens, kpbwt="U[%d,%d]" % (kmin, kmax), block=T)
+    return chunks, desc, meta
 
 
+def make_panels(chunks):
+    """Device panels (sparse-PBWT build + upload), one host thread per chunk."""
+    from glimpse_amd import hip
+    panels = [None] * len(chunks)
+    errs = []
+
+    def one(i):
+        try:
+            panels[i] = hip.Panel(chunks[i]["panel"])
+        except Exception as e:   # surfaced below
+            errs.append(e)
+    th = [threading.Thread(target=one, args=(i,)) for i in range(len(chunks))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    if errs:
+        raise errs[0]
+    return panels
+
+
+# ------------------------------------------------------------------ accuracy
 def dosage(gp0, gp1):
     return gp1 + 2.0 * np.clip(1.0 - gp0 - gp1, 0.0, 1.0)
 
 
 def accuracy(outs, chunks):
+    """Dosage r2 (basic_stats.h:61-125) and NRC = 1 - NRD
+    (concordance/src/containers/call_set_writing.cpp:46-55) against the truth,
+    over every chunk, target and site."""
     ds = np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in outs])
     tr = np.concatenate([c["truth"].ravel() for c in chunks]).astype(np.float64)
     r = np.corrcoef(ds, tr)[0, 1]
     g = np.concatenate([(o["h0"].astype(int) + o["h1"]).ravel() for o in outs])
-    # non-reference concordance (concordance/src/containers/call_set_writing.cpp:46-55)
-    mism = np.sum((g != tr) & ((tr > 0) | (g > 0)))
+    mism = np.sum(g != tr)
     match_nonref = np.sum((g == tr) & (tr > 0))
     nrc = 1.0 - mism / max(1, mism + match_nonref)
     return dict(dosage_r2=float(r * r), nrc=float(nrc))
 
 
+# ------------------------------------------------------------------ CPU baseline
 def cpu_baseline(chunks):
-    """The CPU oracle timed on the host: one target alone (one core), then one
-    target per thread on as many threads as the host share allows (up to
-    16), each an INIT + 1 burn-in + 1 main run of the smallest chunk,
-    scaled by 21/3 to the full 1 + 5 + 15 schedule.  The oracle's C calls
-    release the interpreter lock, so the threads run in parallel."""
+    """The oracle on this host: full 1 + 5 + 15 runs of `cores` targets on the
+    smallest chunk, the targets of an iteration spread over `cores` threads
+    after the serial PBWT sweep (the reference's --threads model; counter
+    draws, so the threaded run is reproducible)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
-    c = min(chunks, key=lambda x: x["panel"]["L"])
+    c = min(chunks, key=lambda x: x["L"])
+    cores = max(1, min(16, os.cpu_count() or 1, c["gl"].shape[0]))
+    t0 = time.perf_counter()
     OP = orc.Panel(c["panel"])
-    L = c["panel"]["L"]
-
-    def one(t):
-        prm = orc.default_params(burnin=1, main_iter=1, kpbwt=int(c["K"][t]), rng_mode=orc.RNG_REFERENCE)
-        OP.phase_run(c["gl"][t:t + 1], c["flat"][t:t + 1], prm)
-
+    prep = time.perf_counter() - t0
+    kp = np.ascontiguousarray(c["K"][:cores], np.int32)
+    prm = orc.default_params(rng_mode=orc.RNG_COUNTER, n_threads=cores)
     t0 = time.perf_counter()
-    one(0)
-    dt1 = time.perf_counter() - t0
-    n = max(1, min(16, os.cpu_count() or 1, c["gl"].shape[0]))
-    th = [threading.Thread(target=one, args=(t,)) for t in range(n)]
-    t0 = time.perf_counter()
-    for x in th:
-        x.start()
-    for x in th:
-        x.join()
-    dtn = time.perf_counter() - t0
-    v1 = L / (dt1 * 21.0 / 3.0)
-    vn = n * L / (dtn * 21.0 / 3.0)
-    return dict(value=vn, unit="target-site imputations/s", cores=n, kind="port",
-                one_core=v1,
-                sample="oracle (C++ restatement) on %d targets x %d-site chunk, Kpbwt per target (%d..%d), INIT+1 "
-                       "burn-in+1 main iterations, one target per thread on %d threads in %.2f s (one target on one "
-                       "thread: %.2f s), scaled x21/3 to a full run"
-                       % (n, L, int(c["K"][:n].min()), int(c["K"][:n].max()), n, dtn, dt1))
+    OP.phase_run(c["gl"][:cores], c["flat"][:cores], prm, kpbwt_per_target=kp)
+    dt = time.perf_counter() - t0
+    return dict(value=cores * c["L"] / dt, unit="target-site imputations/s", cores=cores, kind="port",
+                sample="oracle (C++ restatement of the reference path) on %d targets x the %d-site chunk "
+                       "(N=%d, Kpbwt %d): one full 1+5+15 run, %.1f s on %d threads (panel and PBWT build %.1f s "
+                       "not counted)" % (cores, c["L"], c["panel"]["N"], int(kp[0]), dt, cores, prep))
 
 
+# ------------------------------------------------------------------ selector training
 def train_selector(seed=31337, epochs=8):
     """Trains the selector network on rows a plain K=2000 run records."""
     from glimpse_amd import hip, synth
@@ -172,109 +225,138 @@ def train_selector(seed=31337, epochs=8):
     losses = net.train_run(run, epochs=epochs, batch=16384, lr=2e-3, seed=seed)
     t2 = time.perf_counter()
     run.close()
-    info = dict(rows=rows, epochs=epochs, label_run_s=round(t1 - t0, 2), train_s=round(t2 - t1, 3),
-                loss_first=float(losses[0]), loss_last=float(losses[-1]),
-                data="4000 haps x 10000 SNPs (2 Mb), 48 targets at 1x, Kpbwt 2000, rows from iterations 1-20")
-    return net, info
+    return net, dict(rows=rows, epochs=epochs, label_run_s=round(t1 - t0, 2), train_s=round(t2 - t1, 3),
+                     loss_first=float(losses[0]), loss_last=float(losses[-1]),
+                     data="4000 haps x 10000 SNPs (2 Mb), 48 targets at 1x, Kpbwt 2000, rows from iterations 1-20")
 
 
-def timed(runs, a, label):
-    import torch
+# ------------------------------------------------------------------ units and workers
+class Unit:
+    def __init__(self, chunk, ci, lo, hi, panel):
+        from glimpse_amd import hip
+        self.ci, self.lo, self.hi, self.L = ci, lo, hi, chunk["L"]
+        self.run = hip.Run(panel, chunk["gl"][lo:hi], chunk["flat"][lo:hi], hip.default_params(),
+                           kpbwt_per_target=chunk["K"][lo:hi])
+        self.t0 = self.t1 = 0.0
 
-    def step():
-        if a.serial:
-            for r in runs:
-                r.execute()
-            return
-        errs = []
+    @property
+    def work(self):
+        return self.L * (self.hi - self.lo)
 
-        def go(r):
+
+def run_units(units, workspaces, label):
+    """Serves the units (longest first) to one host thread per workspace;
+    returns the wall time.  Each thread executes its units one after another."""
+    q = queue.Queue()
+    for u in sorted(units, key=lambda u: -u.work):
+        q.put(u)
+    errs = []
+    T0 = time.perf_counter()
+
+    def worker(ws):
+        while True:
             try:
-                r.execute()
-            except Exception as e:  # surfaced below
+                u = q.get_nowait()
+            except queue.Empty:
+                return
+            try:
+                u.run.set_workspace(ws)
+                u.t0 = time.perf_counter() - T0
+                u.run.execute()
+                u.t1 = time.perf_counter() - T0
+            except Exception as e:   # surfaced below
                 errs.append(e)
-        th = [threading.Thread(target=go, args=(r,)) for r in runs]
-        for t in th:
-            t.start()
-        for t in th:
-            t.join()
-        if errs:
-            raise errs[0]
-
-    for i in range(a.warmup):
-        step()
-        print("[bench] %s warmup step %d done" % (label, i), file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / max(1, a.steps)
-    print("[bench] %s: %.2f s per step" % (label, dt), file=sys.stderr, flush=True)
-    return dt, [r.fetch() for r in runs]
+                return
+    th = [threading.Thread(target=worker, args=(ws,)) for ws in workspaces]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - T0
+    if errs:
+        raise errs[0]
+    log("%s: %.2f s for %d units" % (label, dt, len(units)))
+    return dt
 
 
-# workload tag of the committed rocprofv3 summaries (200 targets x 5 chunks, full size)
-PROFILE_WORKLOAD = "batch200x5"
+def set_path(units, path, net, keep, min_mass, chunks):
+    for u in units:
+        if path == "learned":
+            kl = np.maximum(1, np.round(chunks[u.ci]["K"][u.lo:u.hi] * keep)).astype(np.int32)
+            u.run.set_selector(net, kl, min_mass)
+        else:
+            u.run.set_selector(None, None)
 
 
-def profiled_traffic(path):
-    """HBM bytes per forward-backward launch from the committed rocprofv3 counter
-    summary of this workload and path (FETCH_SIZE and WRITE_SIZE runs, scaled by
-    a 1 GiB copy's counter readings), or None when there is none."""
-    f = os.path.join(ROOT, "profiles", "fb_kernel_%s_%s_traffic.md" % (PROFILE_WORKLOAD, path))
-    if not os.path.exists(f):
-        return None
-    for line in open(f):
-        if line.startswith("traffic_bytes_per_launch"):
-            return float(line.split("=")[1])
-    return None
+def gather(units, chunks):
+    """Outputs per chunk, targets in order."""
+    outs = []
+    for ci in range(len(chunks)):
+        us = sorted([u for u in units if u.ci == ci], key=lambda u: u.lo)
+        fs = [u.run.fetch() for u in us]
+        o = {k: np.concatenate([f[k] for f in fs]) for k in ("gp0", "gp1", "h0", "h1", "nstates")}
+        o["stats"] = [u.run.stats() for u in us]
+        outs.append(o)
+    return outs
 
 
-def fb_roofline(outs, path=None):
-    fb_bytes = sum(o["stats"]["fb_bytes"] for o in outs)
-    fb_ms = sum(o["stats"]["fb_kernel_ms"] for o in outs)
-    fb_n = sum(o["stats"]["fb_launches"] for o in outs)
+def fb_roofline(stats_list):
+    fb_bytes = sum(s["fb_bytes"] for s in stats_list)
+    fb_ms = sum(s["fb_kernel_ms"] for s in stats_list)
+    fb_n = sum(s["fb_launches"] for s in stats_list)
     per_launch = fb_bytes / fb_n
     avg_ms = fb_ms / fb_n
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(path) if path else None,
-            "kernel": "fb_kernel (haploid forward-backward)",
-            "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches_per_step": fb_n}
+    return dict(achieved=achieved, bytes_per_launch=per_launch, avg_launch_ms=avg_ms, launches=fb_n,
+                fb_bytes=fb_bytes)
 
 
-def stage_ms(outs):
-    return {k: round(sum(o["stats"][k] for o in outs), 1)
-            for k in ("ms_select", "ms_learn", "ms_condset", "ms_fb", "ms_phase", "ms_other")}
+def profile_value(name, key):
+    """A figure from a committed rocprofv3 summary (profiles/<name>): the line `key = value`."""
+    f = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith(key):
+            try:
+                return float(line.split("=")[1].split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def per_chunk(units, chunks):
+    rows = []
+    for ci, c in enumerate(chunks):
+        us = [u for u in units if u.ci == ci]
+        t0, t1 = min(u.t0 for u in us), max(u.t1 for u in us)
+        n = sum(u.hi - u.lo for u in us)
+        rows.append(dict(sites=c["L"], window_s=round(t1 - t0, 2),
+                         unit_s=round(sum(u.t1 - u.t0 for u in us), 2),
+                         imputations_per_s=round(n * c["L"] / (t1 - t0))))
+    return rows
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--quick", action="store_true", help="small workload (profiling / smoke)")
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", choices=("c5", "c2"), default="c5")
+    ap.add_argument("--quick", action="store_true", help="small workload (smoke / profiling of the plumbing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serial", action="store_true", help="run chunks one after another")
     ap.add_argument("--path", choices=("both", "plain", "learned"), default="both")
-    ap.add_argument("--k-frac", type=float, default=0.75,
-                    help="learned path: kept fraction of the PBWT pool (0.65 was the smallest tried whose dosage r2 "
-                         "is within 0.001 of the plain path's on this workload; 0.75 keeps a margin at equal speed)")
-    ap.add_argument("--k-max", type=int, default=1024,
-                    help="learned path: at most this many kept states per target (0: no cap); 1024 keeps the "
-                         "learned sets in the 1024-state launch class")
+    ap.add_argument("--workers", type=int, default=0, help="units in flight (default: 4 for C5, all chunks for C2)")
+    ap.add_argument("--keep", type=float, default=0.6,
+                    help="learned path: kept fraction of each target's PBWT pool")
     ap.add_argument("--min-mass", type=float, default=0.95,
                     help="learned path: a target whose kept states hold less than this share of the predicted mass "
                          "keeps its whole PBWT pool (the plain selection) for that iteration")
-    ap.add_argument("--latency-runs", type=int, default=0,
-                    help="the N chunks with the most work (sites x states) use latency launch shapes (more waves "
-                         "per target); with every chunk in flight the throughput defaults were faster for all, so 0")
-    ap.add_argument("--tune", action="store_true",
-                    help="autotune the launch shapes during the warm-up steps (each chunk's tuner times its own "
-                         "launches; with all chunks in flight the measured defaults are faster)")
+    ap.add_argument("--probe", action="store_true",
+                    help="only the roofline probe: the largest unit alone on the plain path (for rocprofv3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB three times (HBM counter calibration)")
+    ap.add_argument("--seed", type=int, default=15052011)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on exactly one GPU (the Gibbs chains are independent per target and "
@@ -295,48 +377,84 @@ def main():
     from glimpse_amd import hip
 
     t_setup = time.perf_counter()
-    chunks, desc, meta = build_workload(a.quick)
-    panels = [hip.Panel(c["panel"]) for c in chunks]
-    prm = hip.default_params()
-    runs = [hip.Run(p, c["gl"], c["flat"], prm, kpbwt_per_target=c["K"]) for p, c in zip(panels, chunks)]
-    if a.tune:
-        for r in runs:
-            r.set_tuning(1)   # the warm-up executions time each launch shape and keep the fastest
-    else:
-        work = [c["panel"]["L"] * int(np.sum(c["K"])) for c in chunks]
-        for ci in sorted(range(len(chunks)), key=lambda i: -work[i])[:max(0, a.latency_runs)]:
-            runs[ci].set_tuning("latency")
+    build = build_c5 if a.workload == "c5" else build_c2
+    chunks, desc, meta = build(a.quick, a.seed)
+    log("inputs %.1f s" % (time.perf_counter() - t_setup))
+    panels = make_panels(chunks)
+    log("panels %.1f s" % (time.perf_counter() - t_setup))
+    block = meta["block"]
+    units = [Unit(c, ci, lo, min(lo + block, c["gl"].shape[0]), panels[ci])
+             for ci, c in enumerate(chunks) for lo in range(0, c["gl"].shape[0], block)]
+    n_workers = a.workers or (4 if a.workload == "c5" else len(chunks))
+    workspaces = [hip.Workspace() for _ in range(n_workers)]
+    tsites = sum(c["gl"].shape[0] * c["L"] for c in chunks)
     setup_s = time.perf_counter() - t_setup
-    print("[bench] setup %.1f s (%s)" % (setup_s, desc), file=sys.stderr, flush=True)
-    tsites = sum(len(c["targets"]) * c["panel"]["L"] for c in chunks)
+    log("setup %.1f s: %s; %d units, %d in flight" % (setup_s, desc, len(units), n_workers))
+
+    if a.probe:
+        u = max(units, key=lambda u: u.work)
+        set_path([u], "plain", None, 0, 0, chunks)
+        u.run.set_workspace(workspaces[0])
+        u.run.execute()
+        rf = fb_roofline([u.run.stats()])
+        print(json.dumps(dict(probe="largest unit alone, plain path", chunk_sites=u.L, targets=u.hi - u.lo, **rf)),
+              flush=True)
+        return
+
+    paths = ["learned", "plain"] if a.path == "both" else [a.path]
+    net, train = (None, None)
+    if "learned" in paths:
+        net, train = train_selector()
+        log("selector trained: %s" % json.dumps(train))
+    # workspace sizing: each worker executes its largest unit once per path (outputs discarded)
+    big = sorted(units, key=lambda u: -u.work)[:n_workers]
+    for p in paths:
+        set_path(big, p, net, a.keep, a.min_mass, chunks)
+        run_units(big, workspaces, "workspace sizing (%s)" % p)
 
     res = {}
-    if a.path in ("both", "plain"):
-        dt, outs = timed(runs, a, "plain")
-        res["plain"] = dict(value=tsites / dt, ms_per_step=dt * 1e3, outs=outs)
+    for p in paths:
+        set_path(units, p, net, a.keep, a.min_mass, chunks)
+        for i in range(a.warmup):
+            run_units(units, workspaces, "%s warmup step %d" % (p, i))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            run_units(units, workspaces, "%s step %d" % (p, i))
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / max(1, a.steps)
+        outs = gather(units, chunks)
+        stats = [s for o in outs for s in o["stats"]]
+        rf = fb_roofline(stats)
+        res[p] = dict(value=tsites / dt, ms_per_step=dt * 1e3, accuracy=accuracy(outs, chunks),
+                      per_chunk=per_chunk(units, chunks), step_fb_bytes=rf["fb_bytes"],
+                      step_level_frac=rf["fb_bytes"] / dt / 1e9 / HBM_PEAK_GBS,
+                      mean_states=float(np.mean(np.concatenate([o["nstates"][:, 1:].ravel() for o in outs]))),
+                      stage_ms_summed=({k: round(sum(s[k] for s in stats), 1)
+                                        for k in ("ms_select", "ms_learn", "ms_condset", "ms_fb", "ms_phase",
+                                                  "ms_other")}),
+                      ds=np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in outs]))
+        if p == "learned":
+            res[p]["target_iterations_learned"] = sum(s["learned_sel"] for s in stats)
+            res[p]["target_iterations_pool_kept"] = sum(s["pool_kept"] for s in stats)
+        log("%s: %.3fM target-site imputations/s, r2 %.5f, NRC %.5f" % (
+            p, res[p]["value"] / 1e6, res[p]["accuracy"]["dosage_r2"], res[p]["accuracy"]["nrc"]))
     free_b, total_b = torch.cuda.mem_get_info()
-    hbm_used_gb = (total_b - free_b) / 1e9   # device memory in use with every chunk's run resident
-    train = None
-    if a.path in ("both", "learned"):
-        net, train = train_selector()
-        print("[bench] selector trained: %s" % json.dumps(train), file=sys.stderr, flush=True)
-        for r, c in zip(runs, chunks):
-            kl = np.maximum(1, np.round(c["K"] * a.k_frac))
-            if a.k_max > 0:
-                kl = np.minimum(kl, a.k_max)
-            r.set_selector(net, kl.astype(np.int32), a.min_mass)
-        dt, outs = timed(runs, a, "learned")
-        res["learned"] = dict(value=tsites / dt, ms_per_step=dt * 1e3, outs=outs)
-    for k, v in res.items():
-        v["accuracy"] = accuracy(v["outs"], chunks)
-        v["stage_ms_summed_over_chunks"] = stage_ms(v["outs"])
-    # the headline path: the learned selector when its dosage r2 is within 0.001
-    # of the plain path's, else the plain path
-    main_path = "learned" if "learned" in res else "plain"
-    if "learned" in res and "plain" in res and \
-            abs(res["learned"]["accuracy"]["dosage_r2"] - res["plain"]["accuracy"]["dosage_r2"]) > 0.001:
+    hbm_used_gb = (total_b - free_b) / 1e9
+
+    # roofline probe: the largest unit alone on the plain path
+    u = max(units, key=lambda u: u.work)
+    set_path([u], "plain", None, 0, 0, chunks)
+    u.run.set_workspace(workspaces[0])
+    u.run.execute()
+    probe = fb_roofline([u.run.stats()])
+
+    main_path = paths[0]
+    if len(paths) == 2 and abs(res["learned"]["accuracy"]["dosage_r2"] - res["plain"]["accuracy"]["dosage_r2"]) > 0.001:
         main_path = "plain"
     M = res[main_path]
+    wl = a.workload + ("-quick" if a.quick else "")
+    traffic = profile_value("fb_kernel_%s_probe_traffic.md" % wl, "traffic_bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": M["value"],
@@ -350,42 +468,37 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded simulator: founder mosaics + Poisson reads)",
-        "config": dict(workload=desc, path=main_path,
-                       concurrency="serial" if a.serial else "chunks on concurrent streams", **meta),
-        "roofline": fb_roofline(M["outs"], main_path if a.quick is False else None),
+        "config": dict(workload=desc, path=main_path, units_in_flight=n_workers, **meta),
+        "roofline": {"bound": "hbm", "achieved": probe["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": probe["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "fb_kernel (haploid forward-backward), largest unit alone",
+                     "bytes_per_launch": probe["bytes_per_launch"], "avg_launch_ms": probe["avg_launch_ms"],
+                     "step_level_frac": M["step_level_frac"],
+                     "valu_issue_frac": profile_value("fb_kernel_%s_probe_valu.md" % wl, "valu_issue_frac"),
+                     "valu_lane_insts_per_state_site": profile_value("fb_kernel_%s_probe_valu.md" % wl,
+                                                                     "valu_lane_insts_per_state_site")},
         "accuracy": M["accuracy"],
-        "stage_ms_summed_over_chunks": M["stage_ms_summed_over_chunks"],
-        "launch_shapes": [dict(fb="%dx%d" % (o["stats"]["fb_nw"], o["stats"]["fb_mwl"]),
-                               phase="%dx%d" % (o["stats"]["ph_nw"], o["stats"]["ph_kpt"])) for o in M["outs"]],
-        "setup_s": setup_s,
+        "per_chunk": M["per_chunk"],
+        "setup_s": round(setup_s, 1),
         "hbm_used_gb": round(hbm_used_gb, 1),
-        "chunk_wall_ms": [round(o["stats"]["ms_total"], 1) for o in M["outs"]],
     }
-    if "learned" in res:
-        L_ = res["learned"]
-        st = [o["stats"] for o in L_["outs"]]
-        line["learned"] = dict(value=L_["value"], ms_per_step=L_["ms_per_step"], accuracy=L_["accuracy"],
-                               roofline=fb_roofline(L_["outs"], None if a.quick else "learned"),
-                               k_frac=a.k_frac, k_max=a.k_max, min_mass=a.min_mass, training=train,
-                               target_iterations_learned=sum(s["learned_sel"] for s in st),
-                               target_iterations_pool_kept=sum(s["pool_kept"] for s in st),
-                               mean_states=float(np.mean(np.concatenate(
-                                   [o["nstates"][:, 1:].ravel() for o in L_["outs"]]))))
-    if "plain" in res:
-        Pn = res["plain"]
-        line["plain_pbwt"] = dict(value=Pn["value"], ms_per_step=Pn["ms_per_step"], accuracy=Pn["accuracy"],
-                                  roofline=fb_roofline(Pn["outs"], None if a.quick else "plain"),
-                                  stage_ms_summed_over_chunks=Pn["stage_ms_summed_over_chunks"],
-                                  mean_states=float(np.mean(np.concatenate(
-                                      [o["nstates"][:, 1:].ravel() for o in Pn["outs"]]))))
-        if "learned" in res:
-            dsl = np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in res["learned"]["outs"]])
-            dsp = np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in Pn["outs"]])
-            line["learned"]["mean_abs_ds_vs_plain"] = float(np.mean(np.abs(dsl - dsp)))
-            line["learned"]["r2_minus_plain"] = res["learned"]["accuracy"]["dosage_r2"] - Pn["accuracy"]["dosage_r2"]
-            line["learned"]["speedup_vs_plain"] = res["learned"]["value"] / Pn["value"]
+    for p in paths:
+        R = res[p]
+        d = {k: R[k] for k in ("value", "ms_per_step", "accuracy", "mean_states", "step_level_frac",
+                               "stage_ms_summed")}
+        if p == "learned":
+            d.update(keep=a.keep, min_mass=a.min_mass, training=train,
+                     target_iterations_learned=R["target_iterations_learned"],
+                     target_iterations_pool_kept=R["target_iterations_pool_kept"])
+        line["learned" if p == "learned" else "plain_pbwt"] = d
+    if len(paths) == 2:
+        line["learned"]["mean_abs_ds_vs_plain"] = float(np.mean(np.abs(res["learned"]["ds"] - res["plain"]["ds"])))
+        line["learned"]["r2_minus_plain"] = (res["learned"]["accuracy"]["dosage_r2"] -
+                                             res["plain"]["accuracy"]["dosage_r2"])
+        line["learned"]["speedup_vs_plain"] = res["learned"]["value"] / res["plain"]["value"]
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(chunks)
+        log("cpu baseline: %s" % json.dumps(line["cpu_baseline"]))
     print(json.dumps(line), flush=True)
 
 

@@ -20,6 +20,10 @@
 #include <map>
 #include <numeric>
 #include <set>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <thread>
 
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
@@ -260,6 +264,18 @@ static inline void grow(DBuf &b, size_t bytes) {
     if (bytes > b.bytes) b.reserve(bytes + bytes / 4);
 }
 
+// f(t) for t in [0, n) on up to 16 host threads (f must be thread safe).
+template <typename F>
+static void parallel_targets(int n, F &&f) {
+    const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+    if (nt == 1) { for (int t = 0; t < n; t++) f(t); return; }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; i++)
+        th.emplace_back([&]() { for (int t = next++; t < n; t = next++) f(t); });
+    for (auto &x : th) x.join();
+}
+
 // A launch of at most this many items uses the latency launch shapes.
 constexpr int FEW_ITEMS = 32;
 
@@ -300,6 +316,7 @@ struct gh_run {
     std::vector<gh::FbItem> fb;
     std::vector<gh::PhItem> ph;
     std::vector<uint8_t> stored_h;
+    std::vector<int32_t> init_buf;   // host state sets of the INIT iteration, [T][kstride]
     std::vector<int32_t> start_h, nstates;   // nstates [T][n_iter]
     gh::DBuf d_gl, d_flat, d_K, d_items, d_fb, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_status, d_pst,
         d_pstart, d_pcls, d_pmod, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
@@ -590,24 +607,44 @@ void run_execute(gh_run *G) {
         bool learn_ev = false;
         GH_HIP(hipEventRecord(e_sel.a, s));
         if (stage == 0 || it == 1) {
-            // host-side state sets: INIT selection, and full-panel targets (fixed iota)
-            bool any = false;
-            for (int t = 0; t < T; t++) {
+            // host-side state sets: INIT selection, and full-panel targets (fixed
+            // iota); targets spread over host threads, every set in one buffer
+            // laid out with the run's stride, one copy
+            std::vector<int32_t> &buf = G->init_buf;
+            buf.resize((size_t)T * G->kstride);
+            std::vector<int32_t> kset(T, 0);
+            std::vector<std::string> errs;
+            std::mutex emu;
+            parallel_targets(T, [&](int t) {
                 std::vector<int32_t> S;
                 if (stage == 0 && G->kinit > 0)
                     init_states(P, t, G->gl_h.data() + (size_t)t * L * 3, G->flat_h.data() + (size_t)t * L,
                                 G->kinit, seed, S, G->unph);
                 else if (G->kp[t] >= N) { S.resize(N); std::iota(S.begin(), S.end(), 0); }
-                else continue;
-                GH_CHECK(!S.empty(), GH_ERR_STATE, "target " + std::to_string(t) + ": zero conditioning states");
-                GH_HIP(hipMemcpyAsync(states + (size_t)t * G->kstride, S.data(), sizeof(int32_t) * S.size(),
-                                      hipMemcpyHostToDevice, s));
-                items[t].K = (int)S.size();
-                items[t].W = ((int)S.size() + 63) / 64;
-                GH_HIP(hipStreamSynchronize(s));   // S is a temporary
-                any = true;
+                else return;
+                if (S.empty() || (int)S.size() > G->kstride) {
+                    std::lock_guard<std::mutex> lk(emu);
+                    errs.push_back("target " + std::to_string(t) + ": " +
+                                   (S.empty() ? "zero conditioning states" : "state set larger than the run's stride"));
+                    return;
+                }
+                std::memcpy(buf.data() + (size_t)t * G->kstride, S.data(), sizeof(int32_t) * S.size());
+                kset[t] = (int)S.size();
+            });
+            GH_CHECK(errs.empty(), GH_ERR_STATE, errs.empty() ? std::string() : errs[0]);
+            bool any = false;
+            for (int t = 0; t < T; t++)
+                if (kset[t] > 0) {
+                    items[t].K = kset[t];
+                    items[t].W = (kset[t] + 63) / 64;
+                    any = true;
+                }
+            if (any) {
+                // targets without a host set (PBWT targets at it == 1) are rewritten by the walk
+                GH_HIP(hipMemcpyAsync(states, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice, s));
+                GH_HIP(hipMemcpyAsync(items_d, items.data(), sizeof(CondItem) * T, hipMemcpyHostToDevice, s));
+                GH_HIP(hipStreamSynchronize(s));   // buf is reused by the next execution
             }
-            if (any) GH_HIP(hipMemcpyAsync(items_d, items.data(), sizeof(CondItem) * T, hipMemcpyHostToDevice, s));
         }
         if (stage > 0 && npt > 0) {
             // haplotype_set.cpp:335-348: one stored site per group, random start per haplotype

@@ -53,7 +53,11 @@ constexpr int WALK_LOADERS = 192;
 constexpr int WALK_NT = WALK_WALKERS + WALK_LOADERS;
 constexpr int TILE_WORDS = 4096;
 constexpr int TILE_OPS = 64;
-constexpr size_t WALK_BUF = (size_t)TILE_WORDS * 12 + sizeof(SiteOp) * TILE_OPS + (size_t)TILE_OPS * WALK_WALKERS;
+// carrier positions (A_small_idx) of the tile's init ops, staged so the walk's
+// lower-bound scans over them read LDS (at large N these sets hold hundreds)
+constexpr int TILE_SI = 4096;
+constexpr size_t WALK_BUF = (size_t)TILE_WORDS * 12 + sizeof(SiteOp) * TILE_OPS + (size_t)TILE_OPS * WALK_WALKERS +
+                            (size_t)TILE_SI * 4;
 constexpr size_t WALK_LDS = 2 * WALK_BUF;
 static_assert(WALK_BUF % 16 == 0, "tile buffers keep 16-byte alignment");
 
@@ -226,8 +230,8 @@ void pbwt_decode(gh_panel *P) {
     auto *D = new gh_panel::PbwtDev();
     P->pbwt = D;
     std::vector<SiteOp> ops;
-    std::vector<uint64_t> bits;
-    std::vector<uint32_t> cnt;
+    std::vector<uint64_t> bits, cbits;   // column words of rare ops / of common sites
+    std::vector<uint32_t> cnt, ccnt;
     const int Lhq = P->Lhq;
     const int S = pbwt_ckpt_every(Lhq, N);
     D->ck_every = S;
@@ -241,7 +245,8 @@ void pbwt_decode(gh_panel *P) {
     std::iota(A.begin(), A.end(), 0);
     const uint8_t *y = P->ypacked.data(), *yend = y + P->ypacked.size();
     int l_hq = 0, last_k = -1, prev_ref = 0, n_runs = 0;
-    auto decode_col = [&](int n, int ref, std::vector<int32_t> &a, std::vector<int32_t> &b) {
+    auto decode_col = [&](int n, int ref, std::vector<int32_t> &a, std::vector<int32_t> &b,
+                          std::vector<uint64_t> &bits, std::vector<uint32_t> &cnt) {
         // appends column bits (prefix order) + popcount prefixes, permutes a -> b -> a
         const int W = (n + 63) / 64;
         const size_t base = bits.size();
@@ -308,7 +313,7 @@ void pbwt_decode(gh_panel *P) {
             op.prev_ref = prev_ref;
             op.nbits = ns;
             op.a_off = P->asmall_ptr[l_hq];
-            op.roff = decode_col(N, op.ref, A, B);
+            op.roff = decode_col(N, op.ref, A, B, cbits, ccnt);
             ci.roff = op.roff;
             ci.ref = op.ref;
             if ((l_hq + 1) % S == 0) {
@@ -335,35 +340,70 @@ void pbwt_decode(gh_panel *P) {
             op.ref = op.major ? (int)P->cref[k] : ns - (int)P->calt[k];
             op.nbits = ns;
             op.a_off = P->asmall_ptr[l_hq];
-            op.roff = decode_col(ns, op.ref, sA, sB);
+            op.roff = decode_col(ns, op.ref, sA, sB, bits, cnt);
         }
         ops.push_back(op);
         last_k = k;
     }
     GH_CHECK(y == yend, GH_ERR_ARG, "sparse-PBWT stream longer than the panel");
     D->n_ops = (int)ops.size();
+    for (int o = (int)ops.size() - 1, nxt = (int)ops.size(); o >= 0; o--) {
+        ops[o].next_common = nxt;
+        if (ops[o].kind == 0) nxt = o;
+    }
     D->n_runs = n_runs;
-    // LDS tiles for the walk: greedy runs of ops whose rank data fits TILE_WORDS
+    // LDS tiles for the walk: greedy runs of at most TILE_OPS ops whose rank
+    // data fits TILE_WORDS.  A tile holds the rare ops' columns, and the common
+    // sites' columns too when one is small (N <= 32,768); larger common columns
+    // stay in global memory and the walkers look them up there (a tile of two
+    // 100,000-haplotype columns would cost a full copy per two sites).
     D->total_words = (int64_t)bits.size();
+    D->total_cwords = (int64_t)cbits.size();
+    const int64_t ccol = (N + 63) / 64 + 1;   // words of one common column (with its pad)
+    D->stage_common = ccol <= TILE_WORDS / 8;
     {
-        std::vector<int32_t> tiles;
+        std::vector<int64_t> t_op, t_r, t_c, t_s;
+        std::vector<int32_t> tsi;   // each tile's staged carrier sets, concatenated
         const int n = (int)ops.size();
-        auto end_of = [&](int o) { return o < n ? ops[o].roff : (int64_t)bits.size(); };
+        auto si_of = [&](const SiteOp &o) -> int64_t { return (o.init && o.nbits <= TILE_SI) ? o.nbits : 0; };
+        auto words_of = [&](const SiteOp &o) -> int64_t {
+            return o.kind == 0 ? (D->stage_common ? ccol : 0) : (int64_t)((o.nbits + 63) / 64 + 1);
+        };
         bool ok = true;
+        int64_t r = 0, c = 0;
         for (int o = 0; o < n && ok;) {
             const int o0 = o;
-            if (end_of(o0 + 1) - ops[o0].roff > TILE_WORDS) { ok = false; break; }
-            int o1 = o0 + 1;
-            while (o1 < n && o1 - o0 < TILE_OPS && end_of(o1 + 1) - ops[o0].roff <= TILE_WORDS) o1++;
-            tiles.push_back(o0);
-            o = o1;
+            if (words_of(ops[o0]) > TILE_WORDS) { ok = false; break; }
+            t_op.push_back(o0); t_r.push_back(r); t_c.push_back(c); t_s.push_back((int64_t)tsi.size());
+            int64_t wsum = 0, ssum = 0;
+            while (o < n && o - o0 < TILE_OPS && wsum + words_of(ops[o]) <= TILE_WORDS &&
+                   ssum + si_of(ops[o]) <= TILE_SI) {
+                wsum += words_of(ops[o]);
+                SiteOp &op = ops[o];
+                op.si_loc = -1;
+                if (si_of(op) > 0) {
+                    op.si_loc = (int32_t)ssum;
+                    tsi.insert(tsi.end(), P->asmall_idx.begin() + op.a_off, P->asmall_idx.begin() + op.a_off + op.nbits);
+                    ssum += op.nbits;
+                }
+                if (op.kind == 0) c += ccol; else r += (op.nbits + 63) / 64 + 1;
+                o++;
+            }
         }
         if (ok && n > 0) {
-            tiles.push_back(n);
-            D->n_tiles = (int)tiles.size() - 1;
+            t_op.push_back(n); t_r.push_back(r); t_c.push_back(c); t_s.push_back((int64_t)tsi.size());
+            GH_CHECK(r == D->total_words && c == D->total_cwords, GH_ERR_ARG, "pbwt tiles: rank data out of step");
+            D->n_tiles = (int)t_op.size() - 1;
+            std::vector<int64_t> tiles;
+            tiles.insert(tiles.end(), t_op.begin(), t_op.end());
+            tiles.insert(tiles.end(), t_r.begin(), t_r.end());
+            tiles.insert(tiles.end(), t_c.begin(), t_c.end());
+            tiles.insert(tiles.end(), t_s.begin(), t_s.end());
             dalloc_copy(D->tiles, tiles.data(), tiles.size(), P->stream);
+            dalloc_copy(D->tsi, tsi.data(), tsi.size(), P->stream);
         } else {
-            D->n_tiles = 0;   // a single site's rank data exceeds a tile: global lookups
+            D->n_tiles = 0;   // a single op's rank data exceeds a tile: global lookups
+            for (auto &op : ops) op.si_loc = -1;
         }
     }
     D->n_arrays = n_ck;
@@ -371,12 +411,15 @@ void pbwt_decode(gh_panel *P) {
     dalloc_copy(D->ops, ops.data(), ops.size(), s);
     dalloc_copy(D->bits, bits.data(), bits.size(), s);
     dalloc_copy(D->cnt, cnt.data(), cnt.size(), s);
+    dalloc_copy(D->cbits, cbits.data(), cbits.size(), s);
+    dalloc_copy(D->ccnt, ccnt.data(), ccnt.size(), s);
     dalloc_copy(D->A, Astore.data(), Astore.size(), s);
     dalloc_copy(D->aidx, P->asmall_idx.data(), P->asmall_idx.size(), s);
     dalloc_copy(D->sfin, sfin.data(), sfin.size(), s);
     dalloc_copy(D->cols, cols.data(), cols.size(), s);
     GH_HIP(hipStreamSynchronize(s));
-    P->device_bytes += (int64_t)(D->ops.bytes + D->bits.bytes + D->cnt.bytes + D->A.bytes + D->aidx.bytes +
+    P->device_bytes += (int64_t)(D->ops.bytes + D->bits.bytes + D->cnt.bytes + D->cbits.bytes + D->ccnt.bytes +
+                                 D->A.bytes + D->aidx.bytes +
                                  D->sfin.bytes + D->cols.bytes);
 }
 
@@ -384,8 +427,10 @@ void pbwt_decode(gh_panel *P) {
 struct WalkArgs {
     const SiteOp *ops;
     int n_ops;
-    const uint64_t *bits;
+    const uint64_t *bits;    // rare ops' columns
     const uint32_t *cnt;
+    const uint64_t *cbits;   // common sites' columns
+    const uint32_t *ccnt;
     int N;
     const int32_t *aidx;
     float cm0;             // cM of site 0 (the walk's first reset position)
@@ -404,9 +449,12 @@ struct WalkArgs {
     int cap;
     int32_t *ev_cnt;       // [n_haps]
     int32_t *status;
-    const int32_t *tile_op;  // [n_tiles + 1] first op of each LDS tile
+    const int64_t *tile_op;  // [n_tiles + 1] first op of each LDS tile
+    const int64_t *tile_r;   // [n_tiles + 1] its first rare column word
+    const int64_t *tile_c;   // [n_tiles + 1] its first common column word
+    const int64_t *tile_s;   // [n_tiles + 1] its first staged carrier position in tsi
+    const int32_t *tsi;
     int n_tiles;
-    int64_t total_words;     // length of bits / cnt
 };
 
 __device__ __forceinline__ int rank1(const uint64_t *bits, const uint32_t *cnt, int64_t off, int x) {
@@ -416,21 +464,25 @@ __device__ __forceinline__ int rank1(const uint64_t *bits, const uint32_t *cnt, 
     return r;
 }
 // Counts of the entries of the sorted list a[0, n) below v0, v1 and v2 (three
-// lower bounds at once).  The lists are the short carrier sets of rare runs
-// (tens of entries), which the lanes of a wave read at the same addresses, so
-// batches of independent loads beat dependent binary-search steps.
+// lower bounds at once): branchless binary searches whose step count depends
+// only on n, so the lanes of a wave, which search the same carrier list of a
+// rare run, stay together; the three searches' loads overlap.
 __device__ __forceinline__ void lower_count3(const int32_t *a, int n, int v0, int v1, int v2, int &c0, int &c1,
                                              int &c2) {
-    c0 = c1 = c2 = 0;
-    const int vmax = max(v0, max(v1, v2));
-    for (int j0 = 0; j0 < n; j0 += 8) {
-        int x[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) x[u] = j0 + u < n ? a[j0 + u] : INT_MAX;
-#pragma unroll
-        for (int u = 0; u < 8; u++) { c0 += x[u] < v0; c1 += x[u] < v1; c2 += x[u] < v2; }
-        if (x[7] >= vmax) break;
+    if (n <= 0) { c0 = c1 = c2 = 0; return; }
+    int b0 = 0, b1 = 0, b2 = 0;
+    int m = n;
+    while (m > 1) {
+        const int half = m >> 1;
+        const int x0 = a[b0 + half], x1 = a[b1 + half], x2 = a[b2 + half];
+        b0 = x0 < v0 ? b0 + half : b0;
+        b1 = x1 < v1 ? b1 + half : b1;
+        b2 = x2 < v2 ? b2 + half : b2;
+        m -= half;
     }
+    c0 = b0 + (a[b0] < v0);
+    c1 = b1 + (a[b1] < v1);
+    c2 = b2 + (a[b2] < v2);
 }
 
 // STAGED: the rank data, site ops and this tile's target alleles / stored
@@ -438,7 +490,9 @@ __device__ __forceinline__ void lower_count3(const int32_t *a, int n, int v0, in
 // lookup -> new insertion index -> next lookup) runs at LDS latency.  Without
 // STAGED every lookup goes to global memory (panels whose single site exceeds
 // a tile).
-template <bool STAGED>
+// CGLOBAL (with STAGED): the common sites' columns are not staged; their rank
+// lookups read global memory (large panels).
+template <bool STAGED, bool CGLOBAL>
 __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkArgs w) {
     extern __shared__ uint64_t wsm[];
     constexpr int NWALK = STAGED ? WALK_WALKERS : 64;
@@ -499,17 +553,37 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
         uint32_t *sc = (uint32_t *)(sb + TILE_WORDS);
         SiteOp *so = (SiteOp *)(sc + TILE_WORDS);
         uint8_t *sa = (uint8_t *)(so + TILE_OPS);
-        const int o0 = w.tile_op[ti], o1 = w.tile_op[ti + 1], no = o1 - o0;
-        const int64_t r0 = w.ops[o0].roff;
-        const int64_t r1 = o1 < w.n_ops ? w.ops[o1].roff : w.total_words;
-        const int nwd = (int)(r1 - r0);
+        int32_t *ss = (int32_t *)(sa + TILE_OPS * NWALK);
+        const int o0 = (int)w.tile_op[ti], o1 = (int)w.tile_op[ti + 1], no = o1 - o0;
+        {
+            const int64_t s0 = w.tile_s[ti];
+            const int nsi = (int)(w.tile_s[ti + 1] - s0);
+            for (int i0 = t0; i0 < nsi; i0 += nt * 8) {
+                int32_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int i = i0 + u * nt;
+                    if (i < nsi) v[u] = w.tsi[s0 + i];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int i = i0 + u * nt;
+                    if (i < nsi) ss[i] = v[u];
+                }
+            }
+        }
+        // rare columns first, then (when staged) the common ones
+        const int64_t r0 = w.tile_r[ti], c0 = w.tile_c[ti];
+        const int nr = (int)(w.tile_r[ti + 1] - r0), nc = CGLOBAL ? 0 : (int)(w.tile_c[ti + 1] - c0);
+        const int nwd = nr + nc;
         for (int i0 = t0; i0 < nwd; i0 += nt * 8) {
             uint64_t vb[8];
             uint32_t vc[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const int i = i0 + u * nt;
-                if (i < nwd) { vb[u] = w.bits[r0 + i]; vc[u] = w.cnt[r0 + i]; }
+                if (i < nr) { vb[u] = w.bits[r0 + i]; vc[u] = w.cnt[r0 + i]; }
+                else if (i < nwd) { vb[u] = w.cbits[c0 + i - nr]; vc[u] = w.ccnt[c0 + i - nr]; }
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
@@ -554,20 +628,24 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
     }
     for (int ti = 0; ti < n_tiles; ti++) {
     int o0 = 0, o1 = w.n_ops;
-    int64_t r0 = 0;
+    int64_t r0 = 0, c0 = 0, nr = 0;
     const uint64_t *Rb = w.bits;
     const uint32_t *Rc = w.cnt;
     const SiteOp *s_ops = nullptr;
     const uint8_t *s_a = nullptr;
+    const int32_t *s_si = nullptr;
     if (STAGED) {
-        o0 = w.tile_op[ti];
-        o1 = w.tile_op[ti + 1];
-        r0 = w.ops[o0].roff;
+        o0 = (int)w.tile_op[ti];
+        o1 = (int)w.tile_op[ti + 1];
+        r0 = w.tile_r[ti];
+        c0 = w.tile_c[ti];
+        nr = w.tile_r[ti + 1] - r0;
         const int b = ti & 1;
         Rb = (const uint64_t *)((const uint8_t *)wsm + (size_t)b * WALK_BUF);
         Rc = (const uint32_t *)(Rb + TILE_WORDS);
         s_ops = (const SiteOp *)(Rc + TILE_WORDS);
         s_a = (const uint8_t *)(s_ops + TILE_OPS);
+        s_si = (const int32_t *)(s_a + TILE_OPS * NWALK);
         // buffer b ^ 1 was last read during tile ti - 1, before the barrier
         if (loader && ti + 1 < n_tiles) stage(ti + 1, b ^ 1, (int)threadIdx.x - NWALK, WALK_LOADERS);
     }
@@ -575,14 +653,17 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
     for (int o = o0; o < o1; o++) {
         const SiteOp op = STAGED ? s_ops[o - o0] : w.ops[o];
         const int k = op.k;
-        const int64_t roff = op.roff - r0;
+        // where this op's column is: the tile (rare, then staged common), or global memory
+        const int64_t roff = !STAGED ? op.roff : op.kind ? op.roff - r0 : CGLOBAL ? op.roff : nr + (op.roff - c0);
+        const uint64_t *Cb = (STAGED && !CGLOBAL) ? Rb : w.cbits;
+        const uint32_t *Cc = (STAGED && !CGLOBAL) ? Rc : w.ccnt;
         const int sa = STAGED ? s_a[(o - o0) * NWALK + threadIdx.x] : 0;
         const int a_here = STAGED ? (sa & 1) : hap[k];
         if (op.kind == 0) {
             if (op.init) {
                 // haplotype_set::init_common (haplotype_set.cpp:562-630)
                 const int ns = op.nbits;
-                const int32_t *si = w.aidx + op.a_off;
+                const int32_t *si = (STAGED && op.si_loc >= 0) ? s_si + op.si_loc : w.aidx + op.a_off;
                 const int Aprev = code_after(op.l - 1);   // the order after the previous HQ common site
                 int id, fd, gd;
                 if (in_rare) {
@@ -615,9 +696,9 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
             const int prev = tar;
             tar = a;
             const int ref = op.ref;
-            const int vi = rank1(Rb, Rc, roff, idx);
-            const int vf = rank1(Rb, Rc, roff, f);
-            const int vg = rank1(Rb, Rc, roff, g);
+            const int vi = rank1(Cb, Cc, roff, idx);
+            const int vf = rank1(Cb, Cc, roff, f);
+            const int vg = rank1(Cb, Cc, roff, g);
             const int id = a ? ref + vi : idx - vi;
             int fd = a ? ref + vf : f - vf;
             int gd = a ? ref + vg : g - vg;
@@ -662,7 +743,7 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                     }
                 }
                 if (in_rare) {
-                    const int32_t *si = w.aidx + op.a_off;
+                    const int32_t *si = (STAGED && op.si_loc >= 0) ? s_si + op.si_loc : w.aidx + op.a_off;
                     int fd, gd, c2;
                     lower_count3(si, ns, f, g, idx, fd, gd, c2);
                     if (gd <= fd) { fd = 0; gd = ns; last_reset = k; cm_last = op.cm_k; }
@@ -686,6 +767,9 @@ __global__ __launch_bounds__(STAGED ? WALK_NT : 64) void pbwt_walk_kernel(WalkAr
                 }
                 sidx = si; sf = fd; sg = gd;
             }
+            // no walker of this wave is inside the rare run: its remaining ops
+            // change nothing, go to the run's end (or the tile's)
+            if (!__any(in_rare)) o = min(op.next_common, o1) - 1;
         }
     }
     if (STAGED) __syncthreads();   // tile ti + 1 is in place and tile ti is no longer read
@@ -891,6 +975,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     WalkArgs w;
     w.ops = D->ops.as<SiteOp>(); w.n_ops = D->n_ops;
     w.bits = D->bits.as<uint64_t>(); w.cnt = D->cnt.as<uint32_t>();
+    w.cbits = D->cbits.as<uint64_t>(); w.ccnt = D->ccnt.as<uint32_t>();
     w.N = P->N;
     w.aidx = D->aidx.as<int32_t>();
     w.cm0 = P->cm_pos[0];
@@ -898,24 +983,39 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     w.stored = R.stored; w.hap_class = R.hap_class; w.modulo = R.modulo; w.Lhq = P->Lhq;
     GH_CHECK(R.ev_ci, GH_ERR_ARG, "pbwt selection: no record index buffer");
     w.depth = R.depth; w.ev = R.ev; w.ev_ci = R.ev_ci; w.cap = R.cap; w.ev_cnt = R.ev_cnt; w.status = R.status;
-    w.tile_op = D->tiles.as<int32_t>(); w.n_tiles = D->n_tiles; w.total_words = D->total_words;
+    w.n_tiles = D->n_tiles;
     if (D->n_tiles > 0) {
+        const int64_t *t = D->tiles.as<int64_t>();
+        w.tile_op = t;
+        w.tile_r = t + D->n_tiles + 1;
+        w.tile_c = t + 2 * (D->n_tiles + 1);
+        w.tile_s = t + 3 * (D->n_tiles + 1);
+        w.tsi = D->tsi.as<int32_t>();
+    }
+    const dim3 gw((R.n_haps + WALK_WALKERS - 1) / WALK_WALKERS);
+    if (D->n_tiles > 0 && D->stage_common) {
         static const bool attr = [] {
-            GH_HIP(hipFuncSetAttribute((const void *)pbwt_walk_kernel<true>,
+            GH_HIP(hipFuncSetAttribute((const void *)pbwt_walk_kernel<true, false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)WALK_LDS));
             return true;
         }();
         (void)attr;
-        hipLaunchKernelGGL(pbwt_walk_kernel<true>, dim3((R.n_haps + WALK_WALKERS - 1) / WALK_WALKERS), dim3(WALK_NT),
-                           WALK_LDS,
-                           s, w);
+        hipLaunchKernelGGL((pbwt_walk_kernel<true, false>), gw, dim3(WALK_NT), WALK_LDS, s, w);
+    } else if (D->n_tiles > 0) {
+        static const bool attr = [] {
+            GH_HIP(hipFuncSetAttribute((const void *)pbwt_walk_kernel<true, true>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)WALK_LDS));
+            return true;
+        }();
+        (void)attr;
+        hipLaunchKernelGGL((pbwt_walk_kernel<true, true>), gw, dim3(WALK_NT), WALK_LDS, s, w);
     } else {
-        hipLaunchKernelGGL(pbwt_walk_kernel<false>, dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
+        hipLaunchKernelGGL((pbwt_walk_kernel<false, false>), dim3((R.n_haps + 63) / 64), dim3(64), 0, s, w);
     }
     GH_HIP(hipGetLastError());
     {
         ResolveArgs ra;
-        ra.cols = D->cols.as<ColInfo>(); ra.bits = D->bits.as<uint64_t>(); ra.cnt = D->cnt.as<uint32_t>();
+        ra.cols = D->cols.as<ColInfo>(); ra.bits = D->cbits.as<uint64_t>(); ra.cnt = D->ccnt.as<uint32_t>();
         ra.A = D->A.as<int32_t>(); ra.aidx = D->aidx.as<int32_t>(); ra.sfin = D->sfin.as<int32_t>();
         ra.N = P->N; ra.W = (P->N + 63) / 64;
         ra.ev = R.ev; ra.ev_ci = R.ev_ci; ra.ev_cnt = R.ev_cnt; ra.cap = R.cap; ra.depth = R.depth;

@@ -15,8 +15,10 @@ struct SiteOp {
     int32_t ref;       // ref_rac_l: count of 0 symbols (full or small PBWT)
     int32_t prev_ref;  // ref of the previous HQ common site
     int32_t nbits;     // size of the small set of A_small_idx at l: of the rare run ending here (common), of this run (rare)
-    int32_t pad1[3];
-    int64_t roff;      // offset of this site's column words / popcount prefixes
+    int32_t si_loc;    // init ops: offset of that small set in its walk tile's copy, -1 when not staged
+    int32_t next_common;   // rare ops: index of the next common op (the end of the run), n_ops if none
+    int32_t pad1;
+    int64_t roff;      // offset of this site's column words / popcount prefixes (common or rare arrays)
     uint8_t kind;      // 0 common, 1 rare
     uint8_t init;      // common: rare run just ended (init_common); rare: first site of a run (init_rare)
     uint8_t major;     // rare: major allele
@@ -41,7 +43,7 @@ __host__ __device__ inline int32_t code_entering(int l) { return 2 * (l + 1) + 1
 
 // Per HQ common site: where its column lives and how its entering order was made.
 struct ColInfo {
-    int64_t roff;    // column words / popcount prefixes (prefix order entering the site)
+    int64_t roff;    // common column words / popcount prefixes (prefix order entering the site)
     int32_t ref;     // zeros in the column (cref)
     int32_t ns;      // carriers of the rare run that ends here (0: none)
     int32_t a_off;   // their positions in A_small_idx and their final small order in sfin
@@ -85,10 +87,11 @@ float pbwt_groups(gh_panel *P, int depth, float modulo_in, int kpbwt, std::vecto
 }  // namespace gh
 
 struct gh_panel::PbwtDev {
-    gh::DBuf ops, bits, cnt, A, aidx, sfin, cols, tiles;
+    gh::DBuf ops, bits, cnt, cbits, ccnt, A, aidx, sfin, cols, tiles, tsi;
     int n_ops = 0, n_runs = 0, n_arrays = 0, n_tiles = 0;
     int ck_every = 1;            // S: one stored prefix array every S HQ common sites
-    int64_t total_words = 0;
+    bool stage_common = true;    // common columns staged in the walk's LDS tiles (else read globally)
+    int64_t total_words = 0, total_cwords = 0;   // rare / common column words
 };
 
 namespace gh {

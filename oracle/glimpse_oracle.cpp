@@ -25,11 +25,14 @@
 #include <iterator>
 #include <limits>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <random>
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <atomic>
 #include <vector>
 
 namespace orc {
@@ -1558,10 +1561,19 @@ static void phase_run(const Panel &P, int T, const uint8_t *gl, const uint8_t *f
     }
     { int cid = 0; for (auto &c : classes) class_id[c.first] = cid++; }
 
-    CondSet C(P, prm.ne, err_imp, prm.err_phase);
-    ImputeHMM HMM(&C);
-    PhaseHMM DMM(&C);
-    std::vector<float> HL(2 * L, 0.0f), HP0(2 * L, 0.0f), HP1(2 * L, 0.0f);
+    const int n_threads = prm.rng_mode == ORC_RNG_COUNTER ? std::max(1, std::min(prm.n_threads, T)) : 1;
+    /* per-worker models and buffers (caller_initialise.cpp:143-161) */
+    struct Worker {
+        CondSet C;
+        ImputeHMM HMM;
+        PhaseHMM DMM;
+        std::vector<float> HL, HP0, HP1;
+        Worker(const Panel &P, int ne, float ei, float ep, int L)
+            : C(P, ne, ei, ep), HMM(&C), DMM(&C), HL(2 * L, 0.0f), HP0(2 * L, 0.0f), HP1(2 * L, 0.0f) {}
+    };
+    std::vector<std::unique_ptr<Worker>> workers;
+    for (int w = 0; w < n_threads; w++)
+        workers.emplace_back(new Worker(P, prm.ne, err_imp, prm.err_phase, L));
     const int n_iter = 1 + prm.burnin + prm.main_iter;
     std::vector<std::vector<std::vector<int>>> pbwt_states(T);
 
@@ -1598,23 +1610,43 @@ static void phase_run(const Panel &P, int T, const uint8_t *gl, const uint8_t *f
                 for (size_t i = 0; i < tg.size(); i++) pbwt_states[tg[i]] = S.states[i];
             }
         }
-        for (int t = 0; t < T; t++) {
+        /* caller::phase_individual per target (caller_algorithm.cpp:53-84) */
+        auto individual = [&](int t, Worker &w, Rng &r) {
             Target &g = G[t];
             std::vector<int> states;
             if (stage == 0 && kinit > 0) states.assign(g.init_states.begin(), g.init_states.end());
             else if (kp[t] > 0 && kp[t] < N) states = compact_states(pbwt_states[t], kp[t]);
             else if (kp[t] >= N) { states.resize(N); std::iota(states.begin(), states.end(), 0); }
-            C.build(states);
-            if (nstates_out) nstates_out[t * n_iter + it] = (int32_t)C.n_states;
-            if (stage == 0) hl_init(L, g.gl, g.flat, prm.min_gl, HL);
-            else hl_make(L, g.gl, g.flat, g.H1, prm.min_gl, HL);
-            HMM.run(HL, g.flat, HP0);
-            for (int l = 0; l < L; l++) g.H0[l] = rng.getFloat(P_H0, (uint32_t)t, (uint32_t)l) > HP0[2 * l];
-            hl_make(L, g.gl, g.flat, g.H0, prm.min_gl, HL);
-            HMM.run(HL, g.flat, HP1);
-            for (int l = 0; l < L; l++) g.H1[l] = rng.getFloat(P_H1, (uint32_t)t, (uint32_t)l) > HP1[2 * l];
-            DMM.rephase(g.H0, g.H1, g.flat, rng, t);
-            if (stage == 2) g.store(HP0, HP1);
+            w.C.build(states);
+            if (nstates_out) nstates_out[t * n_iter + it] = (int32_t)w.C.n_states;
+            if (stage == 0) hl_init(L, g.gl, g.flat, prm.min_gl, w.HL);
+            else hl_make(L, g.gl, g.flat, g.H1, prm.min_gl, w.HL);
+            w.HMM.run(w.HL, g.flat, w.HP0);
+            for (int l = 0; l < L; l++) g.H0[l] = r.getFloat(P_H0, (uint32_t)t, (uint32_t)l) > w.HP0[2 * l];
+            hl_make(L, g.gl, g.flat, g.H0, prm.min_gl, w.HL);
+            w.HMM.run(w.HL, g.flat, w.HP1);
+            for (int l = 0; l < L; l++) g.H1[l] = r.getFloat(P_H1, (uint32_t)t, (uint32_t)l) > w.HP1[2 * l];
+            w.DMM.rephase(g.H0, g.H1, g.flat, r, t);
+            if (stage == 2) g.store(w.HP0, w.HP1);
+        };
+        if (n_threads == 1) {
+            for (int t = 0; t < T; t++) individual(t, *workers[0], rng);
+        } else {
+            /* counter draws are keyed by (iteration, target, site): any thread order gives the same run */
+            std::atomic<int> next(0);
+            std::vector<std::thread> th;
+            std::vector<std::string> errs(n_threads);
+            for (int w = 0; w < n_threads; w++)
+                th.emplace_back([&, w]() {
+                    Rng r = rng;
+                    try {
+                        for (int t = next++; t < T; t = next++) individual(t, *workers[w], r);
+                    } catch (const std::exception &e) {
+                        errs[w] = e.what();
+                    }
+                });
+            for (auto &x : th) x.join();
+            for (auto &e : errs) if (!e.empty()) throw OrcError(e);
         }
         if (stage == 0) for (auto &g : G) g.init_states.clear();
     }
@@ -1656,7 +1688,7 @@ void orc_default_params(orc_params *p) {
     p->min_gl = 1e-10f; p->err_imp = 1e-12f; p->err_phase = 0.0001f;
     p->pbwt_depth = 12; p->pbwt_modulo_cm = 0.1f;
     p->kinit = 1000; p->kpbwt = 2000;
-    p->sparse_maf = 0.001f; p->seed = 15052011u; p->rng_mode = ORC_RNG_REFERENCE;
+    p->sparse_maf = 0.001f; p->seed = 15052011u; p->rng_mode = ORC_RNG_REFERENCE; p->n_threads = 1;
 }
 
 const char *orc_last_error(void) { return g_err.c_str(); }

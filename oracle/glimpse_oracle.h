@@ -49,6 +49,9 @@ typedef struct {
     float sparse_maf;
     uint32_t seed;
     int32_t rng_mode;
+    int32_t n_threads;          /* --threads: targets of an iteration spread over threads after the
+                                   serial PBWT sweep (caller_algorithm.cpp:94-116); counter mode only,
+                                   the reference's shared mt19937 makes threaded runs irreproducible */
 } orc_params;
 
 void orc_default_params(orc_params *p);

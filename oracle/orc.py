@@ -24,7 +24,7 @@ class Params(ctypes.Structure):
                 ("min_gl", ctypes.c_float), ("err_imp", ctypes.c_float), ("err_phase", ctypes.c_float),
                 ("pbwt_depth", ctypes.c_int32), ("pbwt_modulo_cm", ctypes.c_float),
                 ("kinit", ctypes.c_int32), ("kpbwt", ctypes.c_int32), ("sparse_maf", ctypes.c_float),
-                ("seed", ctypes.c_uint32), ("rng_mode", ctypes.c_int32)]
+                ("seed", ctypes.c_uint32), ("rng_mode", ctypes.c_int32), ("n_threads", ctypes.c_int32)]
 
 
 def build():
