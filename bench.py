@@ -396,9 +396,10 @@ def main():
         set_path([u], "plain", None, 0, 0, chunks)
         u.run.set_workspace(workspaces[0])
         u.run.execute()
-        rf = fb_roofline([u.run.stats()])
-        print(json.dumps(dict(probe="largest unit alone, plain path", chunk_sites=u.L, targets=u.hi - u.lo, **rf)),
-              flush=True)
+        st = u.run.stats()
+        rf = fb_roofline([st])
+        print(json.dumps(dict(probe="largest unit alone, plain path", chunk_sites=u.L, targets=u.hi - u.lo,
+                              fb_state_sites_per_launch=st["fb_state_sites"] / st["fb_launches"], **rf)), flush=True)
         return
 
     paths = ["learned", "plain"] if a.path == "both" else [a.path]

@@ -107,10 +107,11 @@ inline std::vector<FbPick> fb_shapes(int maxW) {
 inline int fb_block_sites(int maxW) { return fb_shape_pinned(maxW, FbPick{0, 0}, false, nullptr).C; }
 
 // Checkpoint floats of one haplotype item with up to Lp polymorphic sites and W
-// words: every C-th alpha vector (64 W floats).
+// words: every C-th alpha vector (64 W floats), then the forward pass's
+// scaling sum of every site (Lp floats).
 inline int64_t fb_ckpt_floats(int Lp, int W, int maxW) {
     const int C = fb_block_sites(maxW);
-    return (int64_t)((Lp + C - 1) / C) * 64 * W;
+    return (int64_t)((Lp + C - 1) / C) * 64 * W + Lp;
 }
 
 // The host-side guard of a launch: a launch of up to maxW words with shape sh
@@ -119,7 +120,8 @@ inline bool fb_launch_fits(int maxW, int ckpt_maxW, const FbShape &sh) {
     return maxW <= ckpt_maxW && sh.C >= fb_block_sites(ckpt_maxW);
 }
 
-// Floats a launch with block length C writes for an item of Lp sites and W words.
-inline int64_t fb_ckpt_written(int Lp, int W, int C) { return (int64_t)((Lp + C - 1) / C) * 64 * W; }
+// Floats a launch with block length C writes for an item of Lp sites and W
+// words (its checkpoints, then its Lp scaling sums right after them).
+inline int64_t fb_ckpt_written(int Lp, int W, int C) { return (int64_t)((Lp + C - 1) / C) * 64 * W + Lp; }
 
 }  // namespace gh

@@ -80,6 +80,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     const float ee = A.ee, ed = A.ed;
     const float Kf = (float)K, invK = 1.0f / Kf;
     const size_t vstride = (size_t)64 * W;   // one alpha vector (checkpoint) in floats
+    // the forward pass's per-site scaling sums, after the checkpoints: the
+    // recomputation reads them instead of reducing across the waves again
+    float *ssum = ck + (size_t)((Lp + C - 1) / C) * vstride;
 
     // this wave owns words wv, wv + NW, wv + 2 NW, ... of the item, nI of them;
     // slots i >= nI hold no word of this item and are skipped (a wave-uniform
@@ -265,6 +268,26 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
         }
     };
 
+    // the same update at site l > 0 without the scaling sum (the recomputation
+    // takes it from ssum): identical alpha, no cross-wave reduction or barrier
+    auto fwd_site_nosum = [&](int j, float sp) {
+        const float4 mf = smeta[3 * j];
+        uint32_t rw[MWL];
+        rwords(j, rw);
+        const float e0 = mf.x, e1 = mf.y;
+        const float e0p = pv ? e0 : 0.0f, e1p = pv ? e1 : 0.0f;
+        const float f1 = mf.z, f2 = mf.w * __builtin_amdgcn_rcpf(sp);
+        by_fill([&](auto FULLc) {
+            constexpr bool full = decltype(FULLc)::value;
+#pragma unroll
+            for (int i = 0; i < MWL; i++) {
+                if (!full && i >= nI) continue;
+                const int m = rmask(rw, i);
+                a[i] = fmaf(a[i], f2, f1) * (i > 0 ? sel(m, e1, e0) : sel(m, e1p, e0p));
+            }
+        });
+    };
+
     // ---------------- forward: checkpoints every C sites
     {
         float sp = 1.0f;
@@ -278,6 +301,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                 const int l = l0 + j;
                 if (more && j == FB_TILE / 2) prefetch2();
                 fwd_site(j, l, sp);
+                if (tid == 0) ssum[l] = sp;
                 if ((l & (C - 1)) == 0) {
                     float *dst = ck + (size_t)(l / C) * vstride + lane;
 #pragma unroll
@@ -332,24 +356,16 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
 #pragma unroll
             for (int i = 0; i < MWL; i++) a[i] = nck[i];
             if (bs > 0) fetch_ck(bs - C);
-            float sp;
-            {
-                const bool fl0 = __builtin_amdgcn_readfirstlane(__float_as_int(smeta[3 * (bs - l0) + 2].y)) & 1;
-                float s = 0.0f;
-#pragma unroll
-                for (int i = 0; i < MWL; i++) {
-                    if (i >= nI) continue;
-                    s += a[i];
-                }
-                sp = (bs == 0 && fl0) ? 1.0f : xsum1(s);
-            }
+            float sp = ssum[bs];
 #pragma unroll
             for (int i = 0; i < MWL; i++) {
                 if (i >= nI) continue;
                 ablk[((size_t)0 * NW * MWL + i * NW + wv) * 64 + lane] = a[i];
             }
             for (int l = bs + 1; l < be; l++) {
-                fwd_site(l - l0, l, sp);
+                const float spn = ssum[l];
+                fwd_site_nosum(l - l0, sp);
+                sp = spn;
                 by_fill([&](auto FULLc) {
                     constexpr bool full = decltype(FULLc)::value;
 #pragma unroll
