@@ -1484,15 +1484,14 @@ static void counter_sample(const std::vector<int> &pop, int n, std::vector<int> 
     }
 }
 
-/* haplotype_set.cpp:775-833 */
-static void init_selection(const Panel &P, std::vector<Target> &G, int kinit, Rng &rng) {
+/* haplotype_set.cpp:775-833, one target (index ind keys its counter draws) */
+static void init_selection_one(const Panel &P, Target &T, size_t ind, int kinit, Rng &rng) {
     std::vector<int> iota_all(P.N);
     std::iota(iota_all.begin(), iota_all.end(), 0);
     const int k_init = kinit;
     const int k_init8 = (int)std::floor(k_init * 0.8f);
     std::vector<int> sampled;
-    for (size_t ind = 0; ind < G.size(); ++ind) {
-        Target &T = G[ind];
+    {
         std::set<int> &S = T.init_states;
         uint32_t ctr = 0;
         auto do_sample = [&](const std::vector<int> &pop, int n) {
@@ -1525,6 +1524,27 @@ static void init_selection(const Panel &P, std::vector<Target> &G, int kinit, Rn
             do_sample(iota_all, k_init);
             for (size_t i = 0; i < sampled.size() && (int)S.size() < k_init; ++i) S.insert(sampled[i]);
         }
+    }
+}
+
+static void init_selection(const Panel &P, std::vector<Target> &G, int kinit, Rng &rng) {
+    for (size_t ind = 0; ind < G.size(); ++ind) init_selection_one(P, G[ind], ind, kinit, rng);
+}
+
+/* genotype::sortAndNormAndInferGenotype (genotype.cpp:211-244) for one target */
+static void finalize_target(Target &g, int L, float *gp0, float *gp1, uint8_t *hds, uint8_t *h0, uint8_t *h1) {
+    std::sort(g.stored.begin(), g.stored.end(), [](const Stored &a, const Stored &b) { return a.idx < b.idx; });
+    for (auto &e : g.stored) { e.gp0 /= g.stored_cnt; e.gp1 /= g.stored_cnt; }
+    for (int l = 0; l < L; l++) { gp0[l] = 1.0f; gp1[l] = 0.0f; hds[l] = 0; h0[l] = 0; h1[l] = 0; }
+    for (auto &e : g.stored) {
+        const size_t o = e.idx;
+        gp0[o] = e.gp0; gp1[o] = e.gp1; hds[o] = e.hds;
+        const float g2 = 1.0f - e.gp1 - e.gp0;
+        int gt = 0;
+        if (e.gp0 > e.gp1 && e.gp0 > g2) gt = 0;
+        else if (e.gp1 > e.gp0 && e.gp1 > g2) gt = 1;
+        else if (g2 > e.gp0 && g2 > e.gp1) gt = 2;
+        h0[o] = gt == 2; h1[o] = gt >= 1;
     }
 }
 
@@ -1650,25 +1670,9 @@ static void phase_run(const Panel &P, int T, const uint8_t *gl, const uint8_t *f
         }
         if (stage == 0) for (auto &g : G) g.init_states.clear();
     }
-    /* genotype::sortAndNormAndInferGenotype (genotype.cpp:211-244) */
     for (int t = 0; t < T; t++) {
-        Target &g = G[t];
-        std::sort(g.stored.begin(), g.stored.end(), [](const Stored &a, const Stored &b) { return a.idx < b.idx; });
-        for (auto &e : g.stored) { e.gp0 /= g.stored_cnt; e.gp1 /= g.stored_cnt; }
-        for (int l = 0; l < L; l++) {
-            gp0[(size_t)t * L + l] = 1.0f; gp1[(size_t)t * L + l] = 0.0f; hds[(size_t)t * L + l] = 0;
-            h0[(size_t)t * L + l] = 0; h1[(size_t)t * L + l] = 0;
-        }
-        for (auto &e : g.stored) {
-            size_t o = (size_t)t * L + e.idx;
-            gp0[o] = e.gp0; gp1[o] = e.gp1; hds[o] = e.hds;
-            const float g2 = 1.0f - e.gp1 - e.gp0;
-            int gt = 0;
-            if (e.gp0 > e.gp1 && e.gp0 > g2) gt = 0;
-            else if (e.gp1 > e.gp0 && e.gp1 > g2) gt = 1;
-            else if (g2 > e.gp0 && g2 > e.gp1) gt = 2;
-            h0[o] = gt == 2; h1[o] = gt >= 1;
-        }
+        const size_t o = (size_t)t * L;
+        finalize_target(G[t], L, gp0 + o, gp1 + o, hds + o, h0 + o, h1 + o);
     }
 }
 
@@ -1896,6 +1900,63 @@ uint32_t orc_mt19937_nth(uint32_t seed, int64_t n) {
 }
 
 float orc_unphred(int32_t i) { return UNPHRED[i & 255]; }
+
+/* One sampling call of the INIT selection over the population 0..n_pop-1:
+ * std::sample driven by std::mt19937(seed) (reference mode) or Algorithm S
+ * with counter draws keyed (seed, iteration 0, INIT purpose, tgt, 0, 1, ...). */
+int orc_sample(int32_t n_pop, int32_t k, int32_t rng_mode, uint32_t seed, int32_t tgt, int32_t *out, int32_t *n_out) {
+    ORC_TRY({
+        Rng rng; rng.mode = rng_mode; rng.set_seed(seed); rng.it = 0;
+        std::vector<int> pop(n_pop), sampled;
+        std::iota(pop.begin(), pop.end(), 0);
+        if (rng_mode == ORC_RNG_REFERENCE) std::sample(pop.begin(), pop.end(), std::back_inserter(sampled), k, rng.eng);
+        else { uint32_t ctr = 0; counter_sample(pop, k, sampled, rng, (uint32_t)tgt, ctr); }
+        *n_out = (int32_t)sampled.size();
+        std::copy(sampled.begin(), sampled.end(), out);
+    });
+}
+
+/* The INIT-iteration state set of one target (haplotype_set.cpp:108-169, 775-833). */
+int orc_init_states(void *panel, const uint8_t *gl, const uint8_t *flat, int32_t kinit, int32_t rng_mode,
+                    uint32_t seed, int32_t tgt, int32_t *out, int32_t cap, int32_t *n_out) {
+    ORC_TRY({
+        const Panel &P = *(const Panel *)panel;
+        Target g;
+        g.L = P.L; g.gl = gl;
+        g.flat.assign(flat, flat + P.L);
+        Rng rng; rng.mode = rng_mode; rng.set_seed(seed); rng.it = 0;
+        init_rare_tar(P, g);
+        init_selection_one(P, g, (size_t)tgt, std::min(kinit, P.N), rng);
+        if ((int)g.init_states.size() > cap) throw OrcError("init_states: capacity");
+        *n_out = (int32_t)g.init_states.size();
+        std::copy(g.init_states.begin(), g.init_states.end(), out);
+    });
+}
+
+/* GP storage over n_iter main iterations of one target, then the final
+ * normalisation and best guess (genotype.cpp:168-244): HP0/HP1 [n_iter][2L]. */
+int orc_gp_run(int32_t L, int32_t n_iter, const float *HP0, const float *HP1, float *gp0, float *gp1, uint8_t *hds,
+               uint8_t *h0, uint8_t *h1) {
+    ORC_TRY({
+        Target g;
+        g.L = L;
+        for (int i = 0; i < n_iter; i++) {
+            std::vector<float> a(HP0 + (size_t)i * 2 * L, HP0 + (size_t)(i + 1) * 2 * L);
+            std::vector<float> b(HP1 + (size_t)i * 2 * L, HP1 + (size_t)(i + 1) * 2 * L);
+            g.store(a, b);
+        }
+        finalize_target(g, L, gp0, gp1, hds, h0, h1);
+    });
+}
+
+/* getFloat draws: counter mode, n draws keyed (seed, it, purpose, a, b = 0..n-1);
+ * reference mode, the first n draws of std::mt19937(seed). */
+int orc_draws(uint32_t seed, int32_t it, int32_t purpose, int32_t a, int32_t n, int32_t rng_mode, float *out) {
+    ORC_TRY({
+        Rng rng; rng.mode = rng_mode; rng.set_seed(seed); rng.it = it;
+        for (int i = 0; i < n; i++) out[i] = (float)rng.getFloat(purpose, (uint32_t)a, (uint32_t)i);
+    });
+}
 
 /* genotype_writer::writeGenotypes per-site fields for diploid targets
  * (phase/src/io/genotype_writer.cpp:109-190): DS, GP with the writer's

@@ -77,6 +77,11 @@ def lib():
         L.orc_unphred.restype = ctypes.c_float
         L.orc_unphred.argtypes = [ctypes.c_int32]
         L.orc_output_fields.argtypes = [ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 11
+        i32, u32 = ctypes.c_int32, ctypes.c_uint32
+        L.orc_sample.argtypes = [i32, i32, i32, u32, i32, vp, vp]
+        L.orc_init_states.argtypes = [vp, vp, vp, i32, i32, u32, i32, vp, i32, vp]
+        L.orc_gp_run.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_draws.argtypes = [u32, i32, i32, i32, i32, i32, vp]
         _lib = L
     return _lib
 
@@ -256,6 +261,14 @@ class Panel:
         n, m = ns.value, nm.value
         return seg[:n], hp[:max(n - 1, 0)], of1[:m]
 
+    def init_states(self, gl, flat, kinit, rng_mode, seed, tgt=0):
+        out = np.zeros(self.N, np.int32)
+        n = ctypes.c_int32()
+        _check(lib().orc_init_states(self.h, _p(np.ascontiguousarray(gl, np.uint8)),
+                                     _p(np.ascontiguousarray(flat, np.uint8)), kinit, rng_mode, seed, tgt, _p(out),
+                                     self.N, ctypes.byref(n)))
+        return out[:n.value]
+
     def phase_run(self, gl, flat, params, kpbwt_per_target=None):
         gl = np.ascontiguousarray(gl, np.uint8)
         flat = np.ascontiguousarray(flat, np.uint8)
@@ -315,4 +328,31 @@ def output_fields(gp0, gp1, hds, cref, calt):
                                  p(out["gt"]), p(out["info"]), p(out["af"]), p(out["raf"]))
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())
+    return out
+
+
+def sample(n_pop, k, rng_mode, seed, tgt=0):
+    """One INIT sampling call over 0..n_pop-1 (std::sample or Algorithm S)."""
+    out = np.zeros(max(k, 1), np.int32)
+    n = ctypes.c_int32()
+    _check(lib().orc_sample(n_pop, k, rng_mode, seed, tgt, _p(out), ctypes.byref(n)))
+    return out[:n.value]
+
+
+def draws(seed, it, purpose, a, n, rng_mode):
+    out = np.zeros(n, np.float32)
+    _check(lib().orc_draws(seed, it, purpose, a, n, rng_mode, _p(out)))
+    return out
+
+
+def gp_run(HP0, HP1):
+    """GP storage over the main iterations of one target: HP0/HP1 [n_iter][2L]."""
+    HP0 = np.ascontiguousarray(HP0, np.float32)
+    HP1 = np.ascontiguousarray(HP1, np.float32)
+    n_iter, L2 = HP0.shape
+    L = L2 // 2
+    out = dict(gp0=np.zeros(L, np.float32), gp1=np.zeros(L, np.float32), hds=np.zeros(L, np.uint8),
+               h0=np.zeros(L, np.uint8), h1=np.zeros(L, np.uint8))
+    _check(lib().orc_gp_run(L, n_iter, _p(HP0), _p(HP1), _p(out["gp0"]), _p(out["gp1"]), _p(out["hds"]),
+                            _p(out["h0"]), _p(out["h1"])))
     return out
