@@ -23,8 +23,9 @@ constexpr int FB_TILE = 32;               // sites per metadata / Hvar tile
 constexpr int FB_LDS_BYTES = 156 * 1024;  // LDS a workgroup may use
 constexpr int FB_WG_PER_CU = 3;           // workgroups per CU the block length leaves room for
 // LDS words before the Hvar tile: the row-word reads of word slots a wave does
-// not fill may point up to NW*MWL (<= 32) words below a row
-constexpr int FB_ROW_PAD = 32;
+// not fill may point up to NW*MWL (<= 64) words below a row
+constexpr int FB_ROW_PAD = 64;
+constexpr int FB_MAX_WORDS = 64;          // largest conditioning set: 64 words, 4096 states
 
 // Launch shape: NW waves per haplotype, MWL 64-state words per lane.  {0, 0}: default.
 struct FbPick {
@@ -42,6 +43,7 @@ static const int kFbShapes[][2] = {
     {1, 8}, {2, 4}, {4, 2}, {8, 1},             // W <= 8
     {1, 16}, {2, 8}, {4, 4}, {8, 2}, {16, 1},   // W <= 16
     {2, 16}, {4, 8}, {8, 4}, {16, 2},           // W <= 32 (one wave of 32 words spills)
+    {8, 8}, {16, 4},                            // W <= 64
 };
 constexpr int kNumFbShapes = (int)(sizeof(kFbShapes) / sizeof(kFbShapes[0]));
 
@@ -63,7 +65,7 @@ inline FbShape fb_shape_of(int maxW, int NW, int MWL) {
     return s;
 }
 
-inline int fb_cap_words(int maxW) { return maxW <= 4 ? 4 : maxW <= 8 ? 8 : maxW <= 16 ? 16 : 32; }
+inline int fb_cap_words(int maxW) { return maxW <= 4 ? 4 : maxW <= 8 ? 8 : maxW <= 16 ? 16 : maxW <= 32 ? 32 : 64; }
 
 // The shape of a launch whose largest set has maxW words: an explicit pick of
 // the capacity class, else a pinned "<NW>x<MWL>" string (pin, may be NULL) of
@@ -82,7 +84,8 @@ inline FbShape fb_shape_pinned(int maxW, FbPick pick, bool latency, const char *
     if (cap == 4) return fb_shape_of(maxW, 1, 4);
     if (cap == 8) return fb_shape_of(maxW, 2, 4);
     if (cap == 16) return fb_shape_of(maxW, 4, 4);
-    return latency ? fb_shape_of(maxW, 8, 4) : fb_shape_of(maxW, 4, 8);
+    if (cap == 32) return latency ? fb_shape_of(maxW, 8, 4) : fb_shape_of(maxW, 4, 8);
+    return latency ? fb_shape_of(maxW, 16, 4) : fb_shape_of(maxW, 8, 8);
 }
 
 // GH_FB_SHAPE=<NW>x<MWL> pins a shape wherever it fits (testing, reproducing a tuned run).

@@ -412,7 +412,7 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             G->kmax[t] = std::max(G->kinit, G->kp[t]);
             kmax_t = std::max(kmax_t, G->kmax[t]);
         }
-        GH_CHECK(kmax_t <= 2048, GH_ERR_ARG, "more than 2048 conditioning states is not supported");
+        GH_CHECK(kmax_t <= 4096, GH_ERR_ARG, "more than 4096 conditioning states is not supported");
         G->kstride = kmax_t;
         // ---------------- device workspace
         dalloc_copy(G->d_gl, gl_h, (size_t)T * L * 3, s);
@@ -530,6 +530,7 @@ void run_execute(gh_run *G) {
     uint8_t *pst = G->d_pst.as<uint8_t>();
     int32_t *pstart = G->d_pstart.as<int32_t>();
     if ((learned || collecting) && npt > 0) {
+        GH_CHECK(prs <= TOPK_MAX, GH_ERR_ARG, "learned selection: PBWT pools of at most 2048 states");
         GH_CHECK(feature_lds_bytes(P) <= FEAT_LDS_MAX, GH_ERR_CAPACITY,
                  "learned selection: chunk too large for the feature kernel");
         GH_CHECK(N < (1 << 20), GH_ERR_ARG, "learned selection supports at most 2^20 panel haplotypes");
@@ -757,7 +758,7 @@ void run_execute(gh_run *G) {
         // class first in G->ford, the rest after them
         int fbA = T, fbB = 0, fbWA = maxW_it, fbWB = 0;
         if (!G->tune) {
-            auto capw = [](int w) { return w <= 4 ? 4 : w <= 8 ? 8 : w <= 16 ? 16 : 32; };
+            auto capw = [](int w) { return fb_cap_words(w); };
             int top = 0;
             for (int t = 0; t < T; t++) top = std::max(top, capw(items[t].W));
             G->ford.clear();
@@ -771,7 +772,7 @@ void run_execute(gh_run *G) {
             maxW_it = fbWA;
             if (fbB > 0) dalloc_copy(G->d_ford, G->ford.data(), (size_t)T, s);
         }
-        const int wcls = maxW_it <= 4 ? 0 : maxW_it <= 8 ? 1 : maxW_it <= 16 ? 2 : 3;
+        const int wcls = maxW_it <= 4 ? 0 : maxW_it <= 8 ? 1 : maxW_it <= 16 ? 2 : maxW_it <= 32 ? 3 : 4;
         // a launch of a few items cannot fill the GPU: its chains are the
         // critical path, so it takes the latency shapes
         const bool fewA = fbA <= FEW_ITEMS, fewB = fbB <= FEW_ITEMS;
@@ -849,7 +850,7 @@ void run_execute(gh_run *G) {
         // K-class batching of the phasing launch, as for forward-backward
         int phA = T, phB = 0, phKB = 1;
         if (!G->tune) {
-            auto capk = [](int k) { return k <= 256 ? 256 : k <= 512 ? 512 : k <= 1024 ? 1024 : 2048; };
+            auto capk = [](int k) { return k <= 256 ? 256 : k <= 512 ? 512 : k <= 1024 ? 1024 : k <= 2048 ? 2048 : 4096; };
             const int top = capk(maxK);
             G->pord.clear();
             for (int t = 0; t < T; t++)
@@ -860,7 +861,7 @@ void run_execute(gh_run *G) {
             phB = T - phA;
             if (phB > 0) dalloc_copy(G->d_pord, G->pord.data(), (size_t)T, s);
         }
-        const int kcls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
+        const int kcls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : maxK <= 2048 ? 3 : 4;
         const PhaseShape phs = (G->tune && stage > 0)
                                    ? G->ph_tuner.pick(kcls, phase_shapes(maxK), phase_shape(maxK, G->latency))
                                    : phase_shape(maxK, G->latency || phA <= FEW_ITEMS);

@@ -495,7 +495,7 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass, const int32_t *d_win_lo,
                const int32_t *d_win_hi, FbPick pick, const int32_t *d_order) {
     if (n <= 0) return;
-    GH_CHECK(maxW <= 32, GH_ERR_ARG, "fb: more than 2048 conditioning states per haplotype is not supported");
+    GH_CHECK(maxW <= FB_MAX_WORDS, GH_ERR_ARG, "fb: more than 4096 conditioning states per haplotype is not supported");
     const FbShape sh = fb_shape(maxW, pick);
     // the launch writes a checkpoint every sh.C sites into buffers sized for
     // blocks of fb_block_sites(ckpt_maxW): longer blocks write fewer, shorter
@@ -515,7 +515,7 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
     }
     GH_FB(1, 4) GH_FB(2, 2) GH_FB(4, 1) GH_FB(1, 8) GH_FB(2, 4) GH_FB(4, 2) GH_FB(8, 1)
     GH_FB(1, 16) GH_FB(2, 8) GH_FB(4, 4) GH_FB(8, 2) GH_FB(16, 1)
-    GH_FB(2, 16) GH_FB(4, 8) GH_FB(8, 4) GH_FB(16, 2)
+    GH_FB(2, 16) GH_FB(4, 8) GH_FB(8, 4) GH_FB(16, 2) GH_FB(8, 8) GH_FB(16, 4)
 #undef GH_FB
     throw Error(GH_ERR_ARG, "fb: no kernel for the requested launch shape");
 }

@@ -60,7 +60,7 @@ void impute_batch_noalloc(gh_panel *P, int n, const int32_t *state_ptr, const in
     for (int i = 0; i < n; i++) {
         const int K = state_ptr[i + 1] - state_ptr[i];
         GH_CHECK(K > 0, GH_ERR_STATE, "conditioning set " + std::to_string(i) + " has zero states");
-        GH_CHECK(K <= P->N && K <= 2048, GH_ERR_ARG, "gh_impute_batch_noalloc: at most min(N, 2048) states per set");
+        GH_CHECK(K <= P->N && K <= 4096, GH_ERR_ARG, "gh_impute_batch_noalloc: at most min(N, 4096) states per set");
         for (int k = state_ptr[i]; k < state_ptr[i + 1]; k++) {
             GH_CHECK(states[k] >= 0 && states[k] < P->N, GH_ERR_ARG, "state id out of range");
             GH_CHECK(k == state_ptr[i] || states[k] > states[k - 1], GH_ERR_ARG, "states must be sorted and unique");

@@ -619,6 +619,8 @@ static const PhaseShape kPhaseShapes[] = {
     {1, 8}, {2, 4}, {4, 2}, {8, 1},                                 // K <= 512
     {2, 8}, {4, 4}, {8, 2}, {16, 1},                                // K <= 1024
     {4, 8}, {8, 4}, {16, 2},                                        // K <= 2048
+    {8, 8},                                                         // K <= 4096 (16 x 4 would spill:
+                                                                    // 1024 threads leave 128 VGPRs)
 };
 
 std::vector<PhaseShape> phase_shapes(int maxK) {
@@ -631,7 +633,7 @@ std::vector<PhaseShape> phase_shapes(int maxK) {
 }
 
 PhaseShape phase_shape(int maxK, bool latency) {
-    GH_CHECK(maxK <= 2048, GH_ERR_ARG, "phasing: more than 2048 conditioning states is not supported");
+    GH_CHECK(maxK <= 4096, GH_ERR_ARG, "phasing: more than 4096 conditioning states is not supported");
     // GH_PHASE_SHAPE=<NW>x<KPT> pins a shape wherever it fits (testing, reproducing a tuned run)
     if (const char *e = std::getenv("GH_PHASE_SHAPE")) {
         int nw = 0, kpt = 0;
@@ -643,9 +645,9 @@ PhaseShape phase_shape(int maxK, bool latency) {
     // fewest waves per target whose register budget still lets two or more
     // targets share a CU (4 states per thread below 2048 states, 8 at 2048).
     // Latency defaults (a run on the critical path): more waves per target.
-    const int cls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : 3;
-    static const PhaseShape dflt[4] = {{1, 4}, {2, 4}, {4, 4}, {4, 8}};
-    static const PhaseShape fast[4] = {{4, 1}, {4, 2}, {8, 2}, {8, 4}};
+    const int cls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : maxK <= 2048 ? 3 : 4;
+    static const PhaseShape dflt[5] = {{1, 4}, {2, 4}, {4, 4}, {4, 8}, {8, 8}};
+    static const PhaseShape fast[5] = {{4, 1}, {4, 2}, {8, 2}, {8, 4}, {8, 8}};
     return latency ? fast[cls] : dflt[cls];
 }
 
@@ -667,6 +669,7 @@ void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const Con
     }
     GH_PH(1, 1) GH_PH(1, 2) GH_PH(1, 4) GH_PH(2, 1) GH_PH(2, 2) GH_PH(4, 1) GH_PH(2, 4) GH_PH(4, 2) GH_PH(8, 1)
     GH_PH(1, 8) GH_PH(2, 8) GH_PH(4, 4) GH_PH(8, 2) GH_PH(16, 1) GH_PH(4, 8) GH_PH(8, 4) GH_PH(16, 2)
+    GH_PH(8, 8)
 #undef GH_PH
     throw Error(GH_ERR_ARG, "phasing: no kernel for the requested launch shape");
 }

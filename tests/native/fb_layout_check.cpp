@@ -27,7 +27,7 @@ int main() {
     for (int i = 0; i < kNumFbShapes; i++)
         pins.push_back(std::to_string(kFbShapes[i][0]) + "x" + std::to_string(kFbShapes[i][1]));
     const int Lps[] = {1, 2, 3, 4, 5, 7, 8, 9, 31, 32, 33, 63, 64, 65, 1000, 58604};
-    for (int ckW = 1; ckW <= 32; ckW++) {
+    for (int ckW = 1; ckW <= FB_MAX_WORDS; ckW++) {
         const int Cl = fb_block_sites(ckW);
         CHECK(Cl >= 1 && FB_TILE % Cl == 0, "layout block %d of ckpt_maxW %d does not divide the tile\n", Cl, ckW);
         for (int maxW = 1; maxW <= ckW; maxW++) {
@@ -56,7 +56,7 @@ int main() {
         }
         // launches the guard must refuse: more words than the buffers, or a
         // shorter block than the layout (a shape built for a smaller LDS share)
-        for (int maxW = ckW + 1; maxW <= 32; maxW++)
+        for (int maxW = ckW + 1; maxW <= FB_MAX_WORDS; maxW++)
             CHECK(!fb_launch_fits(maxW, ckW, fb_shape_pinned(maxW, FbPick{0, 0}, false, nullptr)),
                   "guard accepts maxW %d > ckpt_maxW %d\n", maxW, ckW);
         if (Cl > 1) {

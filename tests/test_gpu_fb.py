@@ -113,3 +113,27 @@ def test_fb_mixed_sizes_one_launch(panels, monkeypatch, shape):
         ref = OP.impute_hap(s, HLs[i], flat[i % T])
         err = np.abs(HP[i] - ref).max()
         assert err <= FB_TOL, "shape %s K=%d: max |HP - oracle| = %g" % (shape, len(s), err)
+
+
+@pytest.fixture(scope="module")
+def big_panels():
+    """5,000 haplotypes, so conditioning sets of up to 4,096 states fit."""
+    from glimpse_amd import hip, synth
+    P = synth.make_panel(5000, 1500, 2, length_mb=0.5, rare_frac=0.2, flip_frac=0.05, seed=17)
+    gl, flat = synth.make_reads(P["tgt_haps"], 1.0, seed=18)
+    return P, gl, flat, orc.Panel(P), hip.Panel(P, with_pbwt=False)
+
+
+@pytest.mark.parametrize("shape,K", [("8x8", 4096), ("16x4", 3000), ("8x8", 2049)])
+def test_fb_sets_above_2048_states(big_panels, monkeypatch, shape, K):
+    """Conditioning sets of up to 4,096 states (64 words): the 8x8 and 16x4 shapes."""
+    monkeypatch.setenv("GH_FB_SHAPE", shape)
+    P, gl, flat, OP, GP = big_panels
+    rng = np.random.default_rng(K)
+    sets = [np.sort(rng.choice(P["N"], K, replace=False)), np.sort(rng.choice(P["N"], 700, replace=False))]
+    HLs = [orc.hap_likelihoods(gl[i], flat[i]) for i in range(2)]
+    HP = GP.impute_batch(sets, np.stack(HLs), np.stack([flat[0], flat[1]]))
+    for i, s in enumerate(sets):
+        ref = OP.impute_hap(s, HLs[i], flat[i])
+        err = np.abs(HP[i] - ref).max()
+        assert err <= FB_TOL, "shape %s K=%d: max |HP - oracle| = %g" % (shape, len(s), err)

@@ -105,3 +105,19 @@ def test_runs_sharing_a_workspace(case):
     assert ws.bytes > 0
     for r, _ in runs:
         r.close()
+
+
+def test_run_with_sets_above_2048_states():
+    """A Gibbs run whose PBWT sets reach 3,000 states (and a full-panel target of
+    3,500 haplotypes) against the counter-mode oracle."""
+    from glimpse_amd import hip
+    P = synth.make_panel(3500, 2500, 3, length_mb=0.5, seed=41)
+    gl, flat = synth.make_reads(P["tgt_haps"], 1.0, seed=42)
+    kpt = np.array([3000, 1200, 3500], np.int32)
+    kw = dict(burnin=0, main_iter=1, kpbwt=3500, kinit=1000, seed=43)
+    ref = orc.Panel(P).phase_run(gl, flat, orc.default_params(rng_mode=orc.RNG_COUNTER, **kw), kpbwt_per_target=kpt)
+    got = hip.Panel(P).phase_run(gl, flat, hip.default_params(**kw), kpbwt_per_target=kpt)
+    assert np.array_equal(got["nstates"], ref["nstates"]), "conditioning-set sizes differ"
+    assert got["nstates"][2, 1] == 3500
+    d = np.abs(got["gp0"] - ref["gp0"]) + np.abs(got["gp1"] - ref["gp1"])
+    assert np.mean(d < 1e-4) > 0.995, "GP differs at %.4f of sites" % np.mean(d >= 1e-4)

@@ -85,3 +85,21 @@ def test_rephase_every_launch_shape(panels, monkeypatch, nw, kpt, K):
     for t in range(T):
         r0, r1 = OP.rephase(sets[t], flat[t], h0s[t], h1s[t], 99, 2, 7 + t)
         assert np.array_equal(g0[t], r0) and np.array_equal(g1[t], r1), "shape %dx%d K=%d target %d" % (nw, kpt, K, t)
+
+
+@pytest.mark.parametrize("K", [2500, 4096])
+def test_rephase_sets_above_2048_states(K):
+    """Phasing with conditioning sets of up to 4,096 states (the 8 x 8 shape)."""
+    from glimpse_amd import hip, synth
+    P = synth.make_panel(5000, 1500, 2, length_mb=0.5, rare_frac=0.2, flip_frac=0.05, seed=17)
+    gl, flat = synth.make_reads(P["tgt_haps"], 1.0, seed=18)
+    OP, GP = orc.Panel(P), hip.Panel(P, with_pbwt=False)
+    rng = np.random.default_rng(K)
+    T = 2
+    sets = [np.sort(rng.choice(P["N"], K, replace=False)) for _ in range(T)]
+    h0s = np.stack([_perturb(P["tgt_haps"][2 * t], 0.02, rng) for t in range(T)])
+    h1s = np.stack([_perturb(P["tgt_haps"][2 * t + 1], 0.02, rng) for t in range(T)])
+    g0, g1 = GP.rephase_batch(sets, flat, h0s, h1s, 5, 1, [0, 1])
+    for t in range(T):
+        r0, r1 = OP.rephase(sets[t], flat[t], h0s[t], h1s[t], 5, 1, t)
+        assert np.array_equal(g0[t], r0) and np.array_equal(g1[t], r1), "K=%d target %d" % (K, t)
