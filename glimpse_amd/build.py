@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = [
     "gh_capi.hip", "gh_panel.hip", "gh_condset.hip", "gh_impute.hip", "gh_phase.hip", "gh_pbwt.hip", "gh_gibbs.hip",
-    "gh_mlp.hip", "gh_select.hip", "gh_output.hip", "gh_noalloc.hip",
+    "gh_mlp.hip", "gh_select.hip", "gh_output.hip", "gh_noalloc.hip", "gh_concord.hip", "gh_ligate.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -37,6 +37,12 @@ void impute_batch_noalloc(gh_panel *P, int n, const int32_t *state_ptr, const in
                           const uint8_t *flat, float err_imp, int ne, float *HP);
 void output_fields(int T, int L, const float *gp0, const float *gp1, const uint8_t *hds, const uint32_t *cref,
                    const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info, float *af, float *raf);
+void concordance(const gh_conc_params *prm, const gh_conc_input *in, gh_conc_output *out);
+double conc_r2(uint64_t n, double sx, double sxx, double sxy, double sy, double syy, bool xc, bool yc);
+void ligate_scan(int S, int N, const int32_t *gt_a, const int32_t *gt_b, const uint8_t *swap_a, int32_t *nmatch,
+                 int32_t *nmism, uint8_t *swap_b, float *phaseq);
+void ligate(int n_chunks, int N, const gh_lig_chunk *chunks, int64_t cap, int32_t *out_chunk, int32_t *out_site,
+            int64_t *n_out, uint8_t *swap, gh_lig_overlap *ov);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -499,6 +505,22 @@ int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *
 int gh_select_topk(int32_t T, const float *scores, const int32_t *pool, const int32_t *K, const int32_t *k_learn,
                    int32_t kstride, float min_mass, int32_t *states, int32_t *K_out, float *coverage) {
     return guard([&] { select_topk_batch(T, scores, pool, K, k_learn, kstride, min_mass, states, K_out, coverage); });
+}
+
+int gh_concordance(const gh_conc_params *prm, const gh_conc_input *in, gh_conc_output *out) {
+    return guard([&] { concordance(prm, in, out); });
+}
+double gh_conc_r2(uint64_t n, double sx, double sxx, double sxy, double sy, double syy, int32_t x_constant,
+                  int32_t y_constant) {
+    return conc_r2(n, sx, sxx, sxy, sy, syy, x_constant != 0, y_constant != 0);
+}
+int gh_ligate_scan(int32_t n_sites, int32_t n_samples, const int32_t *gt_a, const int32_t *gt_b,
+                   const uint8_t *swap_a, int32_t *nmatch, int32_t *nmism, uint8_t *swap_b, float *phaseq) {
+    return guard([&] { ligate_scan(n_sites, n_samples, gt_a, gt_b, swap_a, nmatch, nmism, swap_b, phaseq); });
+}
+int gh_ligate(int32_t n_chunks, int32_t n_samples, const gh_lig_chunk *chunks, int64_t cap, int32_t *out_chunk,
+              int32_t *out_site, int64_t *n_out, uint8_t *swap, gh_lig_overlap *ov) {
+    return guard([&] { ligate(n_chunks, n_samples, chunks, cap, out_chunk, out_site, n_out, swap, ov); });
 }
 
 }  // extern "C"

@@ -296,6 +296,126 @@ int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *
 int gh_select_topk(int32_t T, const float *scores, const int32_t *pool, const int32_t *K, const int32_t *k_learn,
                    int32_t kstride, float min_mass, int32_t *states, int32_t *K_out, float *coverage);
 
+/* ---------------------------------------------------------------------------
+ * Concordance: the genotype metrics of the concordance tool's call_set
+ * (concordance/src/containers/call_set_reading.cpp:476-760 per site and
+ * sample, call_set_header.h:162-259 for the truth / estimate rules), for
+ * n_sites biallelic records of n_samples samples of one ploidy (1 or 2).
+ *
+ * Per site: the truth and estimate of every sample are classified as the
+ * reference does (getTruth with the probability threshold min_val_gl and depth
+ * threshold min_val_dp, getMostLikely with min_tar_gp; alleles flipped at
+ * af > 0.5 unless use_alt_af), and every validated genotype is counted into
+ * its sample, its frequency bin or site group, and its calibration bin.
+ *
+ * A record of counts holds, for truth t and estimate e in 0..ploidy:
+ *   [3 t + e]   genotypes with truth t estimated as e (a 3x3 table; errors of
+ *               truth t = its row minus the diagonal),
+ *   [9 + t]     genotypes with a usable truth t (#val_gt columns),
+ *   [12]        genotypes with a usable truth whose estimate failed min_tar_gp.
+ * A record of sums holds, over the estimated genotypes: [0] sum DS, [1] sum
+ * DS^2, [2] sum DS * truth, [3] min DS, [4] max DS (with the counts they give
+ * the squared correlations the reference's stats2D computes; gh_conc_r2).
+ * Records come per type: index 0 SNPs, 1 indels (the reference's "all" rows
+ * are their sums).
+ */
+#define GH_CONC_NCNT 13
+#define GH_CONC_NSUM 5
+#define GH_CONC_NCAL 100   /* calibration bins: trunc(max GP * 99) (N_BIN_CAL, call_set_header.h:30) */
+#define GH_CONC_NREJ 11
+typedef struct {
+    int32_t ploidy;          /* 1 or 2 for every sample */
+    int32_t n_bins;          /* > 0: frequency bins, edges bins[0..n_bins]; bin b holds bins[b] < maf <= bins[b+1] */
+    const double *bins;
+    int32_t n_groups;        /* n_bins == 0: site groups; site_group[s] in [0, n_groups) or -1 (site not used) */
+    const int32_t *site_group;
+    float min_val_gl;        /* T: a truth class needs probability >= T */
+    int32_t min_val_dp;      /* D: truth depth threshold; 0: depth not used */
+    float min_tar_gp;        /* estimates whose largest GP is below this are filtered */
+    int32_t use_alt_af;      /* 1: no allele flip, maf = af */
+    int32_t truth_from_gt;   /* 1: truth from truth_gt (--gt-val), else from truth_pl */
+    int32_t esti_from_gt;    /* 1: estimates from esti_gt alone (--gt-tar), else from esti_gp and esti_ds */
+} gh_conc_params;
+
+typedef struct {
+    int32_t n_sites, n_samples;
+    const float *af;         /* [S] allele frequency of the site (AF, or AC/AN); NaN: none, the site is skipped */
+    const uint8_t *type;     /* [S] 1 SNP, 2 indel, anything else: skipped */
+    const int32_t *truth_pl; /* [S][N][ploidy + 1] PL; negative (htslib's missing / vector end): missing */
+    const int8_t *truth_gt;  /* [S][N] alt allele count of the truth GT, -1 missing */
+    const int32_t *truth_dp; /* [S][N] DP (htslib's missing value INT32_MIN allowed), NULL when min_val_dp == 0 */
+    const float *esti_gp;    /* [S][N][ploidy + 1] GP (NaN: missing) */
+    const float *esti_ds;    /* [S][N] DS */
+    const int8_t *esti_gt;   /* [S][N] alt allele count of the estimated GT, -1 missing; optional with GP input */
+} gh_conc_input;
+
+typedef struct {
+    uint64_t *spl_cnt;   /* [N][2][GH_CONC_NCNT] per sample and type */
+    double *spl_sum;     /* [N][2][GH_CONC_NSUM] */
+    uint64_t *grp_cnt;   /* [G][2][GH_CONC_NCNT] per frequency bin / site group (G = n_bins or n_groups) */
+    double *grp_sum;     /* [G][2][GH_CONC_NSUM + 1]: + sum of the site maf over the estimated genotypes */
+    uint64_t *cal_cnt;   /* [GH_CONC_NCAL][2][9] truth x estimate per calibration bin */
+    uint64_t *rejected;  /* [GH_CONC_NREJ]: truth codes -1..-5, estimate codes -1..-4 (of usable truths),
+                            sites without AF or of another type, sites in no bin */
+    uint32_t *site_cnt;  /* [S][GH_CONC_NCNT] per site, optional (NULL) */
+    double *site_sum;    /* [S][GH_CONC_NSUM] per site (r2 per variant), optional */
+    int32_t *site_tag;   /* [S] 2 * group + type index, -1 for a skipped site, optional */
+} gh_conc_output;
+
+int gh_concordance(const gh_conc_params *prm, const gh_conc_input *in, gh_conc_output *out);
+
+/* Squared correlation of n (x, y) pairs as stats2D::corrXY gives it
+ * (concordance/src/utils/basic_stats.h:110-115): 0 for n = 0, NaN for n = 1,
+ * 0 when x or y is constant, else (n sxy - sx sy)^2 / ((n sxx - sx^2)(n syy - sy^2)). */
+double gh_conc_r2(uint64_t n, double sx, double sxx, double sxy, double sy, double syy, int32_t x_constant,
+                  int32_t y_constant);
+
+/* ---------------------------------------------------------------------------
+ * Ligation of overlapping phased chunks (ligate/src/ligater/ligater_algorithm.cpp).
+ * GT values are htslib's (bcf_get_genotypes): (allele + 1) << 1 | phased,
+ * 0 missing, INT32_MIN + 1 vector end (haploid), two per sample.
+ *
+ * The overlap of chunk f and f + 1 (same contig, f + 1 starting at or before
+ * f's last site) is scanned as scan_overlap / update_distances do (:108-127,
+ * :160-216): over the records present in both (same contig, position and
+ * allele key) and biallelic in f, each sample's phased heterozygous genotypes
+ * count as matches or mismatches against chunk f's phase as already swapped;
+ * chunk f + 1 is swapped for the samples with fewer matches than mismatches.
+ * A chunk without an overlap keeps the previous chunk's swaps (the reference's
+ * swap state carries over).  The first half of the overlap's common records is
+ * emitted from chunk f, the rest from f + 1 (:434-444); records in one chunk
+ * only are emitted from it.  phase_update (:90-106) is applied in place to
+ * every row of a swapped chunk.
+ * Outputs: the emission order (out_chunk, out_site), swap [n_chunks][N],
+ * and per overlap (n_chunks - 1 entries) its statistics. */
+typedef struct {
+    int32_t n_sites;
+    const int32_t *contig;    /* [n_sites] contig id */
+    const int32_t *pos;       /* [n_sites] position, ascending within a contig */
+    const uint64_t *key;      /* [n_sites] allele key (records match on contig, position and key) */
+    const uint8_t *biallelic; /* [n_sites] or NULL (all biallelic) */
+    int32_t *gt;              /* [n_sites][N][2], updated in place */
+} gh_lig_chunk;
+
+typedef struct {
+    int32_t n_common;      /* records present in both chunks (0: no overlap) */
+    int32_t n_scanned;     /* of them biallelic in the first chunk (L_isec) */
+    int32_t n_swapped;     /* samples whose phase was swapped */
+    float mean_hets;       /* mean over samples of the phased hets compared (Avg #hets) */
+    float mean_phaseq;     /* mean phase quality (Avg phaseQ) */
+} gh_lig_overlap;
+
+int gh_ligate(int32_t n_chunks, int32_t n_samples, const gh_lig_chunk *chunks, int64_t cap, int32_t *out_chunk,
+              int32_t *out_site, int64_t *n_out, uint8_t *swap, gh_lig_overlap *ov);
+
+/* The overlap scan alone: n_sites aligned records of the two chunks
+ * (gt_a [n][N][2], gt_b [n][N][2]) and chunk A's swaps; per sample the match
+ * and mismatch counts, the swap of chunk B and the phase quality
+ * 99 (0.7 + f ln f + (1 - f) ln(1 - f)) / 0.7, f = matches / (matches + mismatches)
+ * (99 when either count is 0). */
+int gh_ligate_scan(int32_t n_sites, int32_t n_samples, const int32_t *gt_a, const int32_t *gt_b,
+                   const uint8_t *swap_a, int32_t *nmatch, int32_t *nmism, uint8_t *swap_b, float *phaseq);
+
 #ifdef __cplusplus
 }
 #endif
