@@ -1,7 +1,8 @@
 """Builds the package's native libraries in-tree.
 
 libgh_synth.so   -- host C++ synthetic panel / read generator (csrc/synth.cpp)
-libglimpse_hip.so -- HIP/CDNA4 kernels + C-ABI (csrc/*.hip), gfx950 only
+libglimpse_hip.so -- HIP/CDNA4 kernels + C-ABI (csrc/*.hip), gfx950 only, and the
+                     host-only VCF/BCF I/O (csrc/gh_vcf.cpp, zlib)
 
 All HIP code is compiled with packed-FP32 instruction selection disabled.
 """
@@ -17,6 +18,9 @@ HIP_SOURCES = [
     "gh_capi.hip", "gh_panel.hip", "gh_condset.hip", "gh_impute.hip", "gh_phase.hip", "gh_pbwt.hip", "gh_gibbs.hip",
     "gh_mlp.hip", "gh_select.hip", "gh_output.hip", "gh_noalloc.hip", "gh_concord.hip", "gh_ligate.hip",
 ]
+
+# host-only C++ (file I/O on zlib), built with g++ into the same library
+CXX_SOURCES = ["gh_vcf.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -51,10 +55,17 @@ def build_hip():
         print("[build]", " ".join(cmd), flush=True)
         jobs.append(subprocess.Popen(cmd))
         objs.append(obj)
-    bad = [src for src, j in zip(HIP_SOURCES, jobs) if j.wait() != 0]
+    for src in CXX_SOURCES:
+        obj = os.path.join(objdir, src.replace(".cpp", ".o"))
+        cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-I", os.path.join(ROOT, "include"), "-c", "-o", obj,
+               os.path.join(CSRC, src)]
+        print("[build]", " ".join(cmd), flush=True)
+        jobs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+    bad = [src for src, j in zip(HIP_SOURCES + CXX_SOURCES, jobs) if j.wait() != 0]
     if bad:
-        raise RuntimeError("hipcc failed on " + ", ".join(bad))
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs)
+        raise RuntimeError("compile failed on " + ", ".join(bad))
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs + ["-lz"])
     return out
 
 

@@ -10,25 +10,15 @@
 #include "glimpse_hip.h"
 #include "gh_fb_layout.hpp"
 
+#include "gh_error.hpp"
+
 namespace gh {
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-void set_error(const std::string &m);
 
 #define GH_HIP(x)                                                                               \
     do {                                                                                        \
         hipError_t _e = (x);                                                                    \
         if (_e != hipSuccess)                                                                   \
             throw ::gh::Error(GH_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(_e));      \
-    } while (0)
-
-#define GH_CHECK(cond, code, msg)                                                               \
-    do {                                                                                        \
-        if (!(cond)) throw ::gh::Error((code), (msg));                                          \
     } while (0)
 
 // Device buffer with RAII; grows, never shrinks.

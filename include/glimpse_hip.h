@@ -26,7 +26,8 @@ enum {
     GH_ERR_HIP = -2,      /* HIP runtime error (message holds hipGetErrorString) */
     GH_ERR_NUMERIC = -3,  /* underflow the reference asserts on (phasing_hmm.cpp:174-175) */
     GH_ERR_CAPACITY = -4, /* an output buffer is too small */
-    GH_ERR_STATE = -5     /* zero conditioning states (conditioning_set.cpp:125) */
+    GH_ERR_STATE = -5,    /* zero conditioning states (conditioning_set.cpp:125) */
+    GH_ERR_IO = -6        /* unreadable, malformed or unwritable file */
 };
 
 typedef struct gh_panel gh_panel;
@@ -415,6 +416,74 @@ int gh_ligate(int32_t n_chunks, int32_t n_samples, const gh_lig_chunk *chunks, i
  * (99 when either count is 0). */
 int gh_ligate_scan(int32_t n_sites, int32_t n_samples, const int32_t *gt_a, const int32_t *gt_b,
                    const uint8_t *swap_a, int32_t *nmatch, int32_t *nmism, uint8_t *swap_b, float *phaseq);
+
+/* ---------------------------------------------------------------------------
+ * VCF / BCF input and output (host code; zlib for BGZF).  Reads VCF text
+ * (plain or BGZF-compressed) and BCF 2.2; values are handed out as htslib's
+ * bcf_get_format_* / bcf_get_info_* / bcf_get_genotypes do (per-sample
+ * vectors padded with the vector-end value, htslib's missing sentinels,
+ * GT = (allele + 1) << 1 | phased).
+ */
+typedef struct gh_vcf_reader gh_vcf_reader;
+typedef struct {
+    const char *chrom;   /* valid until the next gh_vcf_next */
+    int32_t pos;         /* 1-based */
+    const char *id;
+    int32_t n_allele;
+    const char *ref, *alt;   /* first ALT ("." when none) */
+    int32_t is_snp;      /* biallelic, one base each */
+} gh_vcf_record;
+int gh_vcf_open(const char *path, gh_vcf_reader **out);
+int gh_vcf_close(gh_vcf_reader *r);
+/* Sample names, NUL-separated into names (cap bytes); names NULL: count only. */
+int gh_vcf_samples(gh_vcf_reader *r, int32_t *n_samples, char *names, int64_t cap);
+/* Next record: 1, 0 at the end of the file, negative on error. */
+int gh_vcf_next(gh_vcf_reader *r, gh_vcf_record *rec);
+/* A FORMAT / INFO field of the current record into iv (Integer) or fv (Float):
+ * returns the number of values (n_samples x per-sample count for FORMAT; the
+ * per-sample count is the return value / n_samples), 0 when absent. */
+int gh_vcf_format(gh_vcf_reader *r, const char *key, int32_t *iv, float *fv, int64_t cap);
+int gh_vcf_info(gh_vcf_reader *r, const char *key, int32_t *iv, float *fv, int64_t cap);
+
+/* Target genotype likelihoods for the panel's sites
+ * (genotype_reader::scanTarGenotypes + parseTarGenotypes,
+ * phase/src/io/genotype_reader.cpp:352-521), for diploid targets: reads the
+ * remaining records of r; a biallelic record counts for the first panel site
+ * with the same position, REF and ALT (SNPs only unless use_indels), with
+ * FORMAT/PL, or FORMAT/GL as lroundf(-10 GL), capped at 255; records whose
+ * field does not hold 3 values per sample are skipped, as are samples with a
+ * missing value.  gl [T][L][3] (0 where nothing was read), flat [T][L] (0
+ * where the sample's three values differ).  A negative PL or positive GL is
+ * an error (GH_ERR_IO), as in the reference. */
+typedef struct {
+    int32_t n_sites;
+    const int32_t *pos;            /* 1-based */
+    const char *const *ref, *const *alt;
+    int32_t use_gl;                /* 1: FORMAT/GL, 0: FORMAT/PL */
+    int32_t use_indels;
+} gh_gl_query;
+int gh_vcf_read_gls(gh_vcf_reader *r, const gh_gl_query *q, uint8_t *gl, uint8_t *flat, int64_t *n_matched);
+
+/* The phase tool's output file (genotype_writer::writeGenotypes,
+ * phase/src/io/genotype_writer.cpp:43-190): the writer's header lines (with
+ * ##NMAIN, ##FPLOIDY), per site ID, REF, ALT, INFO/RAF, AF and INFO, and per
+ * sample GT (both alleles phased), DS and GP, from gh_output_fields' arrays
+ * (gt [T][L][2], ds [T][L], gp [T][L][3], info / af / raf [L]).  BCF 2.2 in
+ * BGZF blocks for a ".bcf" path, BGZF-compressed VCF for ".gz", else VCF. */
+typedef struct {
+    const char *contig;
+    int32_t n_samples;
+    const char *const *sample_names;
+    int32_t n_sites;
+    const int32_t *pos;
+    const char *const *id;         /* NULL: "." */
+    const char *const *ref, *const *alt;
+    const uint8_t *gt;
+    const float *ds, *gp, *info, *af, *raf;
+    int32_t n_main, fploidy;
+    const char *source, *file_date;
+} gh_vcf_out;
+int gh_write_genotypes(const char *path, const gh_vcf_out *d);
 
 #ifdef __cplusplus
 }
