@@ -43,6 +43,10 @@ void ligate_scan(int S, int N, const int32_t *gt_a, const int32_t *gt_b, const u
                  int32_t *nmism, uint8_t *swap_b, float *phaseq);
 void ligate(int n_chunks, int N, const gh_lig_chunk *chunks, int64_t cap, int32_t *out_chunk, int32_t *out_site,
             int64_t *n_out, uint8_t *swap, gh_lig_overlap *ov);
+void bam_default_params(gh_bam_params *p);
+void bam_tables_out(float *t);
+void bam_call_gls(const char *path, int32_t ploidy, const gh_bam_region *rg, const gh_bam_params *prm, uint8_t *gl,
+                  uint8_t *flat, int32_t *depth, gh_bam_stats *st);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -521,6 +525,17 @@ int gh_ligate_scan(int32_t n_sites, int32_t n_samples, const int32_t *gt_a, cons
 int gh_ligate(int32_t n_chunks, int32_t n_samples, const gh_lig_chunk *chunks, int64_t cap, int32_t *out_chunk,
               int32_t *out_site, int64_t *n_out, uint8_t *swap, gh_lig_overlap *ov) {
     return guard([&] { ligate(n_chunks, n_samples, chunks, cap, out_chunk, out_site, n_out, swap, ov); });
+}
+
+void gh_bam_default_params(gh_bam_params *p) {
+    if (p) bam_default_params(p);
+}
+int gh_bam_tables(float *t) {
+    return guard([&] { bam_tables_out(t); });
+}
+int gh_bam_call_gls(const char *path, int32_t ploidy, const gh_bam_region *r, const gh_bam_params *p, uint8_t *gl,
+                    uint8_t *flat, int32_t *depth, gh_bam_stats *stats) {
+    return guard([&] { bam_call_gls(path, ploidy, r, p, gl, flat, depth, stats); });
 }
 
 }  // extern "C"

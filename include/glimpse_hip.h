@@ -485,6 +485,46 @@ typedef struct {
 } gh_vcf_out;
 int gh_write_genotypes(const char *path, const gh_vcf_out *d);
 
+/* ---------------------------------------------------------------------------
+ * Genotype likelihoods from sequencing reads: genotype_bam_caller with the
+ * "standard" model (phase/src/io/genotype_bam_caller.cpp:225-488), for one
+ * sample's coordinate-sorted BAM file over one contig.  Reads pass read_bam's
+ * filters (:30-59); at every SNP site of the panel inside [beg, end] the
+ * bases of the reads' M / = / X operations with base quality >= min_baseq
+ * give the site's log10 likelihoods (at most max_depth reads: the lowest
+ * keys quality << 5 | strand << 4 | base, as the reference's sort leaves
+ * them), quantised as the reference does: GL[j] = min(int(-10 (llk[j] - i)),
+ * 255), i the index of the largest llk.  Indel sites stay flat (the
+ * reference's default).  Outputs: gl [L][ploidy + 1], flat [L], depth [L]. */
+typedef struct {
+    int32_t min_mapq;           /* --mapq 10 */
+    int32_t min_baseq;          /* --baseq 10 */
+    int32_t max_depth;          /* --max-depth 40 */
+    int32_t fflag;              /* excluded flags: unmapped | secondary | QC fail | duplicate */
+    int32_t keep_orphan;        /* --keep-orphan-reads */
+    int32_t check_orientation;  /* 1 unless --ignore-orientation */
+    int32_t check_proper_pair;  /* --check-proper-pairing */
+} gh_bam_params;
+typedef struct {
+    const char *contig;
+    int32_t beg, end;           /* 1-based, inclusive; <= 0: unbounded */
+    int32_t n_sites;
+    const int32_t *pos;         /* 1-based, ascending */
+    const char *const *ref, *const *alt;
+} gh_bam_region;
+typedef struct {
+    int64_t n_records;          /* BAM records read */
+    int64_t n_kept;             /* kept reads aligned inside the region */
+    int64_t sites_over_max_depth;
+} gh_bam_stats;
+void gh_bam_default_params(gh_bam_params *p);
+int gh_bam_call_gls(const char *path, int32_t ploidy, const gh_bam_region *r, const gh_bam_params *p, uint8_t *gl,
+                    uint8_t *flat, int32_t *depth, gh_bam_stats *stats);
+/* The four error tables of the model (calling_utils, genotype_bam_caller.h:36-42):
+ * t[0..255] log10(1 - e), t[256..] log10(e / 3), t[512..] log10(1/2 - e / 3),
+ * t[768..] log10(e), e = 10^(-q/10). */
+int gh_bam_tables(float *t);
+
 #ifdef __cplusplus
 }
 #endif
