@@ -21,7 +21,7 @@ HIP_SOURCES = [
 ]
 
 # host-only C++ (file I/O on zlib), built with g++ into the same library
-CXX_SOURCES = ["gh_vcf.cpp"]
+CXX_SOURCES = ["gh_vcf.cpp", "gh_binpanel.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

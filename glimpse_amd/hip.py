@@ -151,7 +151,7 @@ def _desc(P, k, ypacked=None, asmall=None):
     if ypacked is not None:
         d.ypacked, d.ypacked_len = _p(ypacked).value, ypacked.size
         d.asmall_ptr, d.asmall_idx = _p(asmall[0]).value, _p(asmall[1]).value
-    d.sparse_maf = 0.001
+    d.sparse_maf = P.get("sparse_maf", 0.001)
     return d
 
 
@@ -162,7 +162,13 @@ class Panel:
         self.L, self.N = P["L"], P["N"]
         self._k = _desc_arrays(P)
         y = a = None
-        if with_pbwt:
+        if with_pbwt and "ypacked" in P:   # a split_reference panel carries its sparse-PBWT stream
+            y = np.ascontiguousarray(P["ypacked"], np.uint8)
+            ap = np.ascontiguousarray(P["asmall_ptr"], np.int32)
+            ai = np.ascontiguousarray(P["asmall_idx"], np.int32)
+            self._k.update(y=y, ap=ap, ai=ai)
+            a = (ap, ai)
+        elif with_pbwt:
             y, ap, ai = build_sparse_pbwt(P)
             self._k.update(y=y, ap=ap, ai=ai)
             a = (ap, ai)

@@ -486,6 +486,45 @@ typedef struct {
 int gh_write_genotypes(const char *path, const gh_vcf_out *d);
 
 /* ---------------------------------------------------------------------------
+ * The split_reference binary panel: the boost binary archive of
+ * ref_haplotype_set and variant_map that split_reference writes
+ * (split_reference/src/caller/caller_initialise.cpp:86-89) and the phase tool
+ * reads (read_binary_reference_panel, phase/src/caller/caller_initialise.cpp:
+ * 178-246).  gh_bin_open decodes it (a bad signature, another platform's
+ * native format or an archive whose counts disagree is GH_ERR_IO, where the
+ * reference stops with its version / signature errors); gh_bin_desc fills a
+ * gh_panel_desc for gh_panel_create (arrays owned by the handle: the panel's
+ * bit matrix as site-major words, ShapRef transposed into per-site carrier
+ * lists as transposeRareRef does, the stored sparse-PBWT stream and
+ * A_small_idx, and the sparse-MAF threshold the stored common flags imply);
+ * gh_bin_write writes a panel in the same layout. */
+typedef struct gh_bin_panel gh_bin_panel;
+typedef struct {
+    const char *chrid;
+    int32_t input_start, input_stop, output_start, output_stop;
+    const char *input_gregion, *output_gregion;
+    int32_t n_sites, n_haps, n_common, n_common_hq;
+    int32_t library_version;   /* boost archive library version */
+    float sparse_maf;
+} gh_bin_panel_info;
+typedef struct {
+    const char *chrid;
+    int32_t input_start, input_stop, output_start, output_stop;
+    const char *input_gregion, *output_gregion;
+    const int32_t *pos;              /* 1-based */
+    const char *const *id, *const *ref, *const *alt;
+    const uint8_t *type;             /* htslib variant type per site; NULL: SNP / indel from the alleles */
+    int32_t library_version;         /* 0: 18 */
+} gh_bin_meta;
+int gh_bin_open(const char *path, gh_bin_panel **out);
+int gh_bin_close(gh_bin_panel *b);
+int gh_bin_info(const gh_bin_panel *b, gh_bin_panel_info *info);
+int gh_bin_desc(const gh_bin_panel *b, gh_panel_desc *d);
+int gh_bin_sites(const gh_bin_panel *b, const int32_t **pos, const char *const **id, const char *const **ref,
+                 const char *const **alt, const uint8_t **type);
+int gh_bin_write(const char *path, const gh_panel_desc *d, const gh_bin_meta *m);
+
+/* ---------------------------------------------------------------------------
  * Genotype likelihoods from sequencing reads: genotype_bam_caller with the
  * "standard" model (phase/src/io/genotype_bam_caller.cpp:225-488), for one
  * sample's coordinate-sorted BAM file over one contig.  Reads pass read_bam's
