@@ -36,7 +36,9 @@ void run_set_selector(gh_run *G, gh_mlp *M, const int32_t *klearn, float min_mas
 void impute_batch_noalloc(gh_panel *P, int n, const int32_t *state_ptr, const int32_t *states, const float *HL,
                           const uint8_t *flat, float err_imp, int ne, float *HP);
 void output_fields(int T, int L, const float *gp0, const float *gp1, const uint8_t *hds, const uint32_t *cref,
-                   const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info, float *af, float *raf);
+                   const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info, float *af, float *raf,
+                   const int32_t *ploidy);
+void run_set_ploidy(gh_run *G, const int32_t *ploidy);
 void concordance(const gh_conc_params *prm, const gh_conc_input *in, gh_conc_output *out);
 double conc_r2(uint64_t n, double sx, double sxx, double sxy, double sy, double syy, bool xc, bool yc);
 void ligate_scan(int S, int N, const int32_t *gt_a, const int32_t *gt_b, const uint8_t *swap_a, int32_t *nmatch,
@@ -499,7 +501,15 @@ int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_
 int gh_output_fields(int32_t T, int32_t L, const float *gp0, const float *gp1, const uint8_t *hds,
                      const uint32_t *cref, const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info,
                      float *af, float *raf) {
-    return guard([&] { output_fields(T, L, gp0, gp1, hds, cref, calt, ds, gp, gt, info, af, raf); });
+    return guard([&] { output_fields(T, L, gp0, gp1, hds, cref, calt, ds, gp, gt, info, af, raf, nullptr); });
+}
+int gh_output_fields_ploidy(int32_t T, int32_t L, const int32_t *ploidy, const float *gp0, const float *gp1,
+                            const uint8_t *hds, const uint32_t *cref, const uint32_t *calt, float *ds, float *gp,
+                            uint8_t *gt, float *info, float *af, float *raf) {
+    return guard([&] { output_fields(T, L, gp0, gp1, hds, cref, calt, ds, gp, gt, info, af, raf, ploidy); });
+}
+int gh_run_set_ploidy(gh_run *r, const int32_t *ploidy) {
+    return guard([&] { run_set_ploidy(r, ploidy); });
 }
 int gh_select_features(gh_panel *p, int32_t T, const uint8_t *H, const uint8_t *gl, const uint8_t *flat,
                        const int32_t *pool, const int32_t *pool_site, const int32_t *K, int32_t kstride, int32_t depth,

@@ -44,17 +44,27 @@ static void unphred_table(float *t) {
     }
 }
 
-// genotype::initHaplotypeLikelihoods / makeHaplotypeLikelihoods (genotype.cpp:56-116)
+// genotype::initHaplotypeLikelihoods / makeHaplotypeLikelihoods (genotype.cpp:56-116).
+// A haploid target (ploidy[t] == 1) always takes initHaplotypeLikelihoods with
+// its two GL values (caller_algorithm.cpp:58-62): HL = the normalised pair.
 __global__ void hl_kernel(const uint8_t *__restrict__ gl, const uint8_t *__restrict__ flat,
                           const uint8_t *__restrict__ H, int L, int T, int mode, int which, float min_gl,
-                          float *__restrict__ HL) {
+                          const uint8_t *__restrict__ ploidy, float *__restrict__ HL) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     if (l >= L) return;
     const size_t o = (size_t)t * L + l;
+    const bool hap = ploidy && ploidy[t] == 1;
     float h0, h1;
     if (flat[o]) {
         h0 = 0.5f; h1 = 0.5f;
+    } else if (hap) {
+        const uint8_t *g = gl + o * 3;
+        float t0 = c_unphred[g[0]], t1 = c_unphred[g[1]];
+        const float s = (t0 + t1) + 0.0f;
+        t0 /= s; t1 /= s;
+        h0 = t0;
+        h1 = t1;
     } else {
         const uint8_t *g = gl + o * 3;
         float t0 = c_unphred[g[0]], t1 = c_unphred[g[1]], t2 = c_unphred[g[2]];
@@ -90,13 +100,30 @@ __global__ void sample_kernel(const float *__restrict__ HP, uint8_t *__restrict_
 // genotype::storeGenotypePosteriorsAndHaplotypes, diploid (genotype.cpp:168-209),
 // in dense form: a site not yet stored whose P(RR) >= 0.99999 contributes the
 // implicit (1, 0); from its first storage on, the actual normalised GP.
+// A haploid target stores the normalised HP0 pair (genotype.cpp:127-166; its
+// first-storage offset is stored_cnt % 16, equal to stored_cnt for --main <= 15).
 __global__ void gp_kernel(const float *__restrict__ HP0, const float *__restrict__ HP1, int L,
-                          float *__restrict__ gp0, float *__restrict__ gp1, uint8_t *__restrict__ stored,
-                          uint8_t *__restrict__ hds) {
+                          const uint8_t *__restrict__ ploidy, float *__restrict__ gp0, float *__restrict__ gp1,
+                          uint8_t *__restrict__ stored, uint8_t *__restrict__ hds) {
+#pragma clang fp contract(off)
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     if (l >= L) return;
     const size_t o = (size_t)t * L + l;
+    if (ploidy && ploidy[t] == 1) {
+        const float p0 = HP0[2 * o], p1 = HP0[2 * o + 1];
+        const float sc = 1.0f / (p0 + p1);
+        const float n0 = p0 * sc, n1 = p1 * sc;
+        if (!stored[o] && !(n0 < 0.99999f)) {
+            gp0[o] += 1.0f;
+            return;
+        }
+        stored[o] = 1;
+        gp0[o] += n0;
+        gp1[o] += n1;
+        hds[o] = 0;
+        return;
+    }
     const float a0 = HP0[2 * o], a1 = HP0[2 * o + 1], b0 = HP1[2 * o], b1 = HP1[2 * o + 1];
     const float p0 = fminf(fmaxf(a0 * b0, 0.0f), 1.0f);
     const float p1 = fminf(fmaxf(a0 * b1 + a1 * b0, 0.0f), 1.0f);
@@ -115,7 +142,7 @@ __global__ void gp_kernel(const float *__restrict__ HP0, const float *__restrict
 // genotype::sortAndNormAndInferGenotype + inferred_genotype::infer (genotype.h:30-36, genotype.cpp:211-244)
 __global__ void finalize_kernel(float *__restrict__ gp0, float *__restrict__ gp1, uint8_t *__restrict__ stored,
                                 uint8_t *__restrict__ hds, uint8_t *__restrict__ h0, uint8_t *__restrict__ h1,
-                                int L, int cnt) {
+                                int L, int cnt, const uint8_t *__restrict__ ploidy) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     if (l >= L) return;
@@ -126,6 +153,12 @@ __global__ void finalize_kernel(float *__restrict__ gp0, float *__restrict__ gp1
     }
     const float a = gp0[o] / cnt, b = gp1[o] / cnt;
     gp0[o] = a; gp1[o] = b;
+    if (ploidy && ploidy[t] == 1) {   // inferred_genotype::infer_haploid (genotype.h:66-68)
+        h0[o] = b > a;
+        h1[o] = 0;
+        hds[o] = 0;
+        return;
+    }
     const float c = 1.0f - b - a;
     int g = 0;
     if (a > b && a > c) g = 0;
@@ -159,9 +192,13 @@ static void select_sample(const int32_t *pop, int n, int k, std::vector<int32_t>
     }
 }
 
-// haplotype_set::initRareTar + performSelection_RARE_INIT_GL (haplotype_set.cpp:108-169, 775-833)
+// haplotype_set::initRareTar + performSelection_RARE_INIT_GL (haplotype_set.cpp:108-169, 775-833).
+// gl: the target's [L][3] values.  For a haploid target the reference reads its
+// GL array ([L][2]) at 2v + 0..2 (its offsets assume three values), so the
+// third value it compares is the next site's first one (0 past the last
+// site), and the third probability is 0.
 static void init_states(gh_panel *P, int t, const uint8_t *gl, const uint8_t *flat, int kinit, uint32_t seed,
-                        std::vector<int32_t> &states, const float *unph) {
+                        std::vector<int32_t> &states, const float *unph, int ploidy) {
     const int L = P->L, N = P->N;
     std::vector<int> sind;
     for (int v = 0; v < L; v++) {
@@ -169,9 +206,10 @@ static void init_states(gh_panel *P, int t, const uint8_t *gl, const uint8_t *fl
         const uint8_t f = P->flags[v];
         const int mg = 2 * ((f & SF_MAJOR) ? 1 : 0);
         if (!(f & SF_COMMON) && !(f & SF_LQ)) {
-            const uint8_t *g = gl + 3 * v;
+            uint8_t g[3] = {gl[3 * v], gl[3 * v + 1], gl[3 * v + 2]};
+            if (ploidy == 1) g[2] = v + 1 < L ? gl[3 * (v + 1)] : 0;
             if (g[1] <= g[mg] || g[2 - mg] <= g[mg]) {
-                float x[3] = {unph[g[0]], unph[g[1]], unph[g[2]]};
+                float x[3] = {unph[g[0]], unph[g[1]], ploidy == 1 ? 0.0f : unph[g[2]]};
                 float s = x[0] + x[1] + x[2];
                 x[0] /= s; x[1] /= s; x[2] /= s;
                 const float af = P->calt[v] * 1.0 / N;
@@ -310,6 +348,8 @@ struct gh_run {
     float unph[256];
     std::vector<uint8_t> gl_h, flat_h;   // host copies for the INIT selection
     std::vector<int> kp, kmax, hap_cls, pbwt_targets;
+    std::vector<uint8_t> ploidy;         // [T] 1 or 2 (genotype::ploidy)
+    bool any_haploid = false;
     std::vector<std::vector<int>> cls_grp;
     std::vector<float> cls_mod;
     std::vector<gh::CondItem> items;
@@ -319,7 +359,8 @@ struct gh_run {
     std::vector<int32_t> init_buf;   // host state sets of the INIT iteration, [T][kstride]
     std::vector<int32_t> start_h, nstates;   // nstates [T][n_iter]
     gh::DBuf d_gl, d_flat, d_K, d_items, d_fb, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_status, d_pst,
-        d_pstart, d_pcls, d_pmod, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow;
+        d_pstart, d_pcls, d_pmod, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow, d_ploidy, d_pone, d_dord;
+    std::vector<int32_t> dord;   // diploid targets in launch order (H1 forward-backward, phasing)
     gh_workspace own;
     gh_workspace *W = &own;
     hipStream_t stream = nullptr;   // the run's own stream (runs on one panel may overlap)
@@ -355,6 +396,40 @@ struct gh_run {
 
 namespace gh {
 
+// Walked haplotype rows of the PBWT targets: a diploid target walks H0 and H1;
+// a haploid one walks H0 only (its second walker repeats H0 and its lists are
+// not used), haplotype_set.cpp:263-270.
+static void upload_pbwt_rows(gh_run *G, hipStream_t s) {
+    const int npt = G->npt;
+    if (npt == 0) return;
+    std::vector<int32_t> hrow;
+    std::vector<uint8_t> one(npt);
+    for (int i = 0; i < npt; i++) {
+        const int t = G->pbwt_targets[i];
+        const bool hap = G->ploidy[t] == 1;
+        hrow.push_back(2 * t);
+        hrow.push_back(hap ? 2 * t : 2 * t + 1);
+        one[i] = hap;
+    }
+    G->R.hap_row = dalloc_copy(G->d_hrow, hrow.data(), hrow.size(), s);
+    G->R.sample_one = G->any_haploid ? dalloc_copy(G->d_pone, one.data(), one.size(), s) : nullptr;
+}
+
+void run_set_ploidy(gh_run *G, const int32_t *ploidy) {
+    GH_CHECK(G && ploidy, GH_ERR_ARG, "gh_run_set_ploidy: null argument");
+    GH_CHECK(!G->done, GH_ERR_ARG, "gh_run_set_ploidy: set the ploidy before the first execution");
+    bool any = false;
+    for (int t = 0; t < G->T; t++) {
+        GH_CHECK(ploidy[t] == 1 || ploidy[t] == 2, GH_ERR_ARG, "gh_run_set_ploidy: ploidy must be 1 or 2");
+        G->ploidy[t] = (uint8_t)ploidy[t];
+        any = any || ploidy[t] == 1;
+    }
+    G->any_haploid = any;
+    dalloc_copy(G->d_ploidy, G->ploidy.data(), (size_t)G->T, G->stream);
+    upload_pbwt_rows(G, G->stream);
+    GH_HIP(hipStreamSynchronize(G->stream));
+}
+
 void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, const gh_params *prm,
                 const int32_t *kpt, gh_run **out) {
     GH_CHECK(P && T > 0 && gl_h && flat_h && prm && out, GH_ERR_ARG, "gh_run_create: bad arguments");
@@ -373,6 +448,7 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         G->ee_i = (float)(1.0 - (double)err_imp);
         G->n_iter = 1 + prm->burnin + prm->main_iter;
         unphred_table(G->unph);
+        G->ploidy.assign(T, 2);
         G->gl_h.assign(gl_h, gl_h + (size_t)T * L * 3);
         G->flat_h.assign(flat_h, flat_h + (size_t)T * L);
         // per-target K (caller_initialise.cpp:150) and PBWT classes by effective modulo
@@ -457,10 +533,10 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             const float minmod = *std::min_element(G->cls_mod.begin(), G->cls_mod.end());
             R.n_haps = 2 * npt; R.n_samples = npt; R.depth = prm->pbwt_depth;
             R.cap = pbwt_event_cap(P, minmod, ngrp);
-            std::vector<int32_t> hrow, srow, pc(2 * npt), kpv(npt), sh(npt);
+            std::vector<int32_t> srow, pc(2 * npt), kpv(npt), sh(npt);
             for (int i = 0; i < npt; i++) {
                 const int t = G->pbwt_targets[i];
-                hrow.push_back(2 * t); hrow.push_back(2 * t + 1); srow.push_back(t);
+                srow.push_back(t);
                 pc[2 * i] = pc[2 * i + 1] = G->hap_cls[2 * t];
                 kpv[i] = G->kp[t];
                 sh[i] = 2 * i;
@@ -469,8 +545,8 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
             R.modulo = dalloc_copy(G->d_pmod, G->cls_mod.data(), G->cls_mod.size(), s);
             R.kpbwt = dalloc_copy(G->d_pkp, kpv.data(), kpv.size(), s);
             R.sample_hap = dalloc_copy(G->d_psh, sh.data(), sh.size(), s);
-            R.hap_row = dalloc_copy(G->d_hrow, hrow.data(), hrow.size(), s);
             R.sample_row = dalloc_copy(G->d_prow, srow.data(), srow.size(), s);
+            upload_pbwt_rows(G, s);
             R.ev_cnt = (int32_t *)G->d_evcnt.reserve(sizeof(int32_t) * 2 * npt);
         }
         dalloc_copy(G->d_ptg, G->pbwt_targets.data(), G->pbwt_targets.size(), s);
@@ -499,6 +575,8 @@ void run_execute(gh_run *G) {
     const int kst = G->kstride, prs = G->pstride;
     const bool learned = G->mlp != nullptr, collecting = G->collect_from >= 1;
     GH_CHECK(!(learned && collecting), GH_ERR_ARG, "a run either records training rows or uses the learned selector");
+    GH_CHECK(!(learned || collecting) || !G->any_haploid, GH_ERR_ARG,
+             "the learned selector and its training rows need diploid targets");
     // workspace of this execution (grows to the largest execution it serves)
     Wk->H.reserve((size_t)2 * T * L);
     Wk->HL.reserve(sizeof(float) * (size_t)T * 2 * L);
@@ -527,6 +605,12 @@ void run_execute(gh_run *G) {
     int32_t *status = G->d_status.as<int32_t>();
     const uint8_t *gl_d = G->d_gl.as<uint8_t>(), *flat_d = G->d_flat.as<uint8_t>();
     int32_t *ptg = G->d_ptg.as<int32_t>();
+    const uint8_t *ploidy_d = G->any_haploid ? G->d_ploidy.as<uint8_t>() : nullptr;
+    int n_dip = T;
+    if (G->any_haploid) {
+        n_dip = 0;
+        for (int t = 0; t < T; t++) n_dip += G->ploidy[t] == 2;
+    }
     uint8_t *pst = G->d_pst.as<uint8_t>();
     int32_t *pstart = G->d_pstart.as<int32_t>();
     if ((learned || collecting) && npt > 0) {
@@ -620,7 +704,7 @@ void run_execute(gh_run *G) {
                 std::vector<int32_t> S;
                 if (stage == 0 && G->kinit > 0)
                     init_states(P, t, G->gl_h.data() + (size_t)t * L * 3, G->flat_h.data() + (size_t)t * L,
-                                G->kinit, seed, S, G->unph);
+                                G->kinit, seed, S, G->unph, G->ploidy[t]);
                 else if (G->kp[t] >= N) { S.resize(N); std::iota(S.begin(), S.end(), 0); }
                 else return;
                 if (S.empty() || (int)S.size() > G->kstride) {
@@ -817,13 +901,42 @@ void run_execute(gh_run *G) {
             GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
         };
         // H0 (caller_algorithm.cpp:58-65)
-        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl, HL);
+        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl,
+                           ploidy_d, HL);
         fb_all(HP0, mass0, e_k0, e_k0b);
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
-        // H1 (caller_algorithm.cpp:66-71)
-        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, HL);
-        fb_all(HP1, mass1, e_k1, e_k1b);
+        // H1 (caller_algorithm.cpp:66-71), diploid targets only
+        hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, ploidy_d, HL);
+        if (!G->any_haploid) {
+            fb_all(HP1, mass1, e_k1, e_k1b);
+        } else if (n_dip == 0) {
+            GH_HIP(hipEventRecord(e_k1.a, s));   // an empty interval keeps the timing code uniform
+            GH_HIP(hipEventRecord(e_k1.b, s));
+        } else {
+            // the diploid targets in pass-0 launch order, split where pass 0 split its classes
+            std::vector<int32_t> &d1 = G->dord;
+            d1.clear();
+            int a1 = 0;
+            for (int i = 0; i < T; i++) {
+                const int t = G->tune ? i : G->ford[i];
+                if (G->ploidy[t] != 2) continue;
+                d1.push_back(t);
+                a1 += i < fbA;
+            }
+            const int32_t *ord = dalloc_copy(G->d_dord, d1.data(), d1.size(), s);
+            GH_HIP(hipEventRecord(e_k1.a, s));
+            if (a1 > 0)
+                fb_launch(P, a1, fb_d, items_d, maxW_it, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HP1, G->ee_i,
+                          G->ed_i, s, mass1, wlo, whi, fbp, ord);
+            if (n_dip - a1 > 0)
+                fb_launch(P, n_dip - a1, fb_d, items_d, fbB > 0 ? fbWB : maxW_it, G->maxW, poly, tr, hvar, HL, flat_d,
+                          alpha, HP1, G->ee_i, G->ed_i, s, mass1, wlo, whi,
+                          (G->latency || n_dip - a1 <= FEW_ITEMS) ? fb_default_pick(fbB > 0 ? fbWB : maxW_it, true)
+                                                                  : FbPick{0, 0},
+                          ord + a1);
+            GH_HIP(hipEventRecord(e_k1.b, s));
+        }
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
         hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
         GH_HIP(hipGetLastError());
@@ -845,21 +958,29 @@ void run_execute(gh_run *G) {
         GH_HIP(hipMemcpyAsync(ph.data(), ph_d, sizeof(PhItem) * T, hipMemcpyDeviceToHost, s));
         GH_HIP(hipMemcpyAsync(items.data(), items_d, sizeof(CondItem) * T, hipMemcpyDeviceToHost, s));
         GH_HIP(hipStreamSynchronize(s));
+        if (G->any_haploid)
+            for (int t = 0; t < T; t++)
+                if (G->ploidy[t] == 1) { ph[t].n_segs = 0; ph[t].n_miss = 0; ph[t].n_loci = 0; }   // no phasing
         int maxK = 1;
-        for (int t = 0; t < T; t++) maxK = std::max(maxK, items[t].K);
-        // K-class batching of the phasing launch, as for forward-backward
-        int phA = T, phB = 0, phKB = 1;
-        if (!G->tune) {
+        for (int t = 0; t < T; t++)
+            if (G->ploidy[t] == 2) maxK = std::max(maxK, items[t].K);
+        // K-class batching of the phasing launch, as for forward-backward; haploid targets are left out
+        int phA = n_dip, phB = 0, phKB = 1;
+        const bool ph_order = G->any_haploid || !G->tune;
+        if (ph_order) {
             auto capk = [](int k) { return k <= 256 ? 256 : k <= 512 ? 512 : k <= 1024 ? 1024 : k <= 2048 ? 2048 : 4096; };
             const int top = capk(maxK);
             G->pord.clear();
             for (int t = 0; t < T; t++)
-                if (capk(items[t].K) == top) G->pord.push_back(t);
+                if (G->ploidy[t] == 2 && (G->tune || capk(items[t].K) == top)) G->pord.push_back(t);
             phA = (int)G->pord.size();
             for (int t = 0; t < T; t++)
-                if (capk(items[t].K) < top) { G->pord.push_back(t); phKB = std::max(phKB, items[t].K); }
-            phB = T - phA;
-            if (phB > 0) dalloc_copy(G->d_pord, G->pord.data(), (size_t)T, s);
+                if (G->ploidy[t] == 2 && !G->tune && capk(items[t].K) < top) {
+                    G->pord.push_back(t);
+                    phKB = std::max(phKB, items[t].K);
+                }
+            phB = n_dip - phA;
+            if (!G->pord.empty()) dalloc_copy(G->d_pord, G->pord.data(), G->pord.size(), s);
         }
         const int kcls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : maxK <= 2048 ? 3 : 4;
         const PhaseShape phs = (G->tune && stage > 0)
@@ -873,9 +994,10 @@ void run_execute(gh_run *G) {
             ph[t].imp_off = io; io += (int64_t)ph[t].n_miss * ph[t].msg;
             ph[t].impsum_off = is; is += ph[t].n_miss * 8;
             ph[t].dip_off = dd; dd += ph[t].n_segs;
-            st.fb_state_sites += 2.0 * items[t].K * items[t].Lp;
-            st.fb_bytes += 2.0 * ((double)items[t].Lp * (2.0 * 8 * items[t].W + 8.0 * items[t].K + 16) + 8.0 * L);
-            st.fb_poly_sites += 2.0 * items[t].Lp;
+            const double np = G->ploidy[t];   // forward-backward passes of the target
+            st.fb_state_sites += np * items[t].K * items[t].Lp;
+            st.fb_bytes += np * ((double)items[t].Lp * (2.0 * 8 * items[t].W + 8.0 * items[t].K + 16) + 8.0 * L);
+            st.fb_poly_sites += np * items[t].Lp;
             st.ph_state_sites += (double)items[t].K * ph[t].n_loci;
             st.ph_segments += ph[t].n_segs + ph[t].n_miss;
             G->nstates[(size_t)t * n_iter + it] = items[t].K;
@@ -895,16 +1017,21 @@ void run_execute(gh_run *G) {
         PB.of1 = Wk->of1.as<float>();
         PB.dip = Wk->dip.as<int32_t>();
         GH_HIP(hipEventRecord(e_pk.a, s));
-        if (phB == 0) {
+        if (!ph_order) {
             phase_launch(P, T, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
                          (uint32_t)it, status, s, phs);
+        } else if (phB == 0) {
+            if (phA > 0)
+                phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
+                             (uint32_t)it, status, s, phs, G->d_pord.as<int32_t>());
         } else {
             ensure_s2();
             const int32_t *ord = G->d_pord.as<int32_t>();
             GH_HIP(hipEventRecord(G->ev_fork, s));
             GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
-            phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
-                         (uint32_t)it, status, s, phs, ord);
+            if (phA > 0)
+                phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
+                             (uint32_t)it, status, s, phs, ord);
             phase_launch(P, phB, phKB, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
                          (uint32_t)it, status, G->s2, phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
             GH_HIP(hipEventRecord(G->ev_join, G->s2));
@@ -913,7 +1040,7 @@ void run_execute(gh_run *G) {
         GH_HIP(hipEventRecord(e_pk.b, s));
         GH_HIP(hipEventRecord(e_ph.b, s));
         if (stage == 2) {
-            hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, gp0, gp1, sto, hds);
+            hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, ploidy_d, gp0, gp1, sto, hds);
             GH_HIP(hipGetLastError());
             main_done++;
         }
@@ -926,8 +1053,9 @@ void run_execute(gh_run *G) {
         st.ms_condset += e_cond.ms();
         st.ms_fb += e_fb.ms();
         st.ms_phase += e_ph.ms();
-        st.fb_kernel_ms += e_k0.ms() + e_k1.ms() + (fbB > 0 ? e_k0b.ms() + e_k1b.ms() : 0.0f);
-        st.fb_launches += fbB > 0 ? 4 : 2;   // kernel dispatches
+        const bool k1b = fbB > 0 && !G->any_haploid;   // the H1 pass used the second stream
+        st.fb_kernel_ms += e_k0.ms() + e_k1.ms() + (fbB > 0 ? e_k0b.ms() : 0.0f) + (k1b ? e_k1b.ms() : 0.0f);
+        st.fb_launches += (fbB > 0 ? 2 : 1) + (G->any_haploid ? (n_dip > 0 ? 1 : 0) : (fbB > 0 ? 2 : 1));
         if (G->tune && stage > 0) {
             G->fb_tuner.record(e_k0.ms() + e_k1.ms());
             G->ph_tuner.record(e_pk.ms());
@@ -947,7 +1075,7 @@ void run_execute(gh_run *G) {
     }
     st.collected_rows = (double)G->col_rows;
     hipLaunchKernelGGL(finalize_kernel, gsite, bsite, 0, s, gp0, gp1, sto, hds, G->d_h0.as<uint8_t>(),
-                       G->d_h1.as<uint8_t>(), L, main_done);
+                       G->d_h1.as<uint8_t>(), L, main_done, ploidy_d);
     GH_HIP(hipGetLastError());
     GH_HIP(hipStreamSynchronize(s));
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();

@@ -881,6 +881,7 @@ struct CompactArgs {
     int cap, depth, N;
     const int32_t *kpbwt;       // [n_samples]
     const int32_t *sample_hap;  // [n_samples]: first haplotype index (second is +1)
+    const uint8_t *sample_one;  // [n_samples]: haploid sample (only the first haplotype's lists), or NULL
     const int32_t *sample_row;  // [n_samples]: output row
     int32_t *out_states;        // [n_samples][out_stride]
     int out_stride;
@@ -898,7 +899,8 @@ __global__ __launch_bounds__(64) void pbwt_compact_kernel(CompactArgs c) {
     const int rec = 2 + 2 * c.depth;
     const int h0 = c.sample_hap[t], h1 = h0 + 1;
     const int32_t *eA = c.ev + (size_t)h0 * c.cap * rec, *eB = c.ev + (size_t)h1 * c.cap * rec;
-    const int nA = c.ev_cnt[h0], nB = c.ev_cnt[h1];
+    // a haploid target has one haplotype (haplotype_set.cpp:263-270): its second walker is not read
+    const int nA = c.ev_cnt[h0], nB = (c.sample_one && c.sample_one[t]) ? 0 : c.ev_cnt[h1];
     const int kp = c.kpbwt[t];
     const int orow = c.sample_row ? c.sample_row[t] : t;
     int32_t *ins = c.ins ? c.ins + (size_t)orow * c.out_stride : nullptr;
@@ -1026,7 +1028,7 @@ void pbwt_select_launch(gh_panel *P, const PbwtRun &R, hipStream_t s) {
     if (R.out_states) {
         CompactArgs c;
         c.ev = R.ev; c.ev_cnt = R.ev_cnt; c.cap = R.cap; c.depth = R.depth; c.N = P->N;
-        c.kpbwt = R.kpbwt; c.sample_hap = R.sample_hap; c.sample_row = R.sample_row; c.out_states = R.out_states; c.out_stride = R.out_stride;
+        c.kpbwt = R.kpbwt; c.sample_hap = R.sample_hap; c.sample_one = R.sample_one; c.sample_row = R.sample_row; c.out_states = R.out_states; c.out_stride = R.out_stride;
         c.out_K = R.out_K;
         c.ins = R.ins;
         c.ins_site = R.ins_site;

@@ -67,6 +67,7 @@ struct PbwtRun {
     // optional compaction
     const int32_t *kpbwt;       // [n_samples]
     const int32_t *sample_hap;  // [n_samples] first walked haplotype of the sample
+    const uint8_t *sample_one;  // [n_samples] 1: haploid sample, its second walker's lists are not used (NULL: none)
     const int32_t *sample_row;  // [n_samples] output row (NULL: identity)
     int32_t *out_states;        // [n_samples][out_stride]
     int out_stride;

@@ -759,6 +759,7 @@ void write_genotypes(const char *path, const gh_vcf_out *d) {
     od.fploidy = d->fploidy;
     for (int i = 0; i < d->n_samples; i++) od.samples.push_back(d->sample_names ? d->sample_names[i] : ("S" + std::to_string(i)));
     const int T = d->n_samples, L = d->n_sites;
+    auto hap = [&](int i) { return d->ploidy != nullptr && d->ploidy[i] == 1; };
     try {
         io::OutStream out(p, gz);
         const std::string htext = io::header_text(od);
@@ -806,9 +807,10 @@ void write_genotypes(const char *path, const gh_vcf_out *d) {
                 ind.desc(1, 2);
                 for (int i = 0; i < T; i++) {
                     const uint8_t *g = d->gt + ((size_t)i * L + l) * 2;
-                    // bcf_gt_phased on both alleles, as the writer fills them
+                    // bcf_gt_phased on both alleles, as the writer fills them; a
+                    // haploid sample's second value is the vector end
                     ind.u8((uint8_t)(((g[0] + 1) << 1) | 1));
-                    ind.u8((uint8_t)(((g[1] + 1) << 1) | 1));
+                    ind.u8(hap(i) ? (uint8_t)0x81 : (uint8_t)(((g[1] + 1) << 1) | 1));
                 }
                 ind.typed_int(kDS);
                 ind.desc(5, 1);
@@ -816,7 +818,8 @@ void write_genotypes(const char *path, const gh_vcf_out *d) {
                 ind.typed_int(kGP);
                 ind.desc(5, 3);
                 for (int i = 0; i < T; i++)
-                    for (int k = 0; k < 3; k++) ind.f32(d->gp[((size_t)i * L + l) * 3 + k]);
+                    for (int k = 0; k < 3; k++)
+                        ind.f32(hap(i) && k == 2 ? io::f32_bits(io::F32_END) : d->gp[((size_t)i * L + l) * 3 + k]);
                 const uint32_t ls = (uint32_t)sh.s.size(), li = (uint32_t)ind.s.size();
                 out.write(&ls, 4);
                 out.write(&li, 4);
@@ -832,9 +835,13 @@ void write_genotypes(const char *path, const gh_vcf_out *d) {
                     const size_t o = (size_t)i * L + l;
                     const uint8_t *g = d->gt + o * 2;
                     line += "\t";
-                    line += std::to_string(g[0]) + "|" + std::to_string(g[1]) + ":" + io::fmt_float(d->ds[o]) + ":" +
-                            io::fmt_float(d->gp[o * 3]) + "," + io::fmt_float(d->gp[o * 3 + 1]) + "," +
-                            io::fmt_float(d->gp[o * 3 + 2]);
+                    if (hap(i))
+                        line += std::to_string(g[0]) + ":" + io::fmt_float(d->ds[o]) + ":" + io::fmt_float(d->gp[o * 3]) +
+                                "," + io::fmt_float(d->gp[o * 3 + 1]);
+                    else
+                        line += std::to_string(g[0]) + "|" + std::to_string(g[1]) + ":" + io::fmt_float(d->ds[o]) + ":" +
+                                io::fmt_float(d->gp[o * 3]) + "," + io::fmt_float(d->gp[o * 3 + 1]) + "," +
+                                io::fmt_float(d->gp[o * 3 + 2]);
                 }
                 line += "\n";
                 out.write(line);

@@ -89,6 +89,8 @@ def lib():
         L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
         L.gh_select_topk.argtypes = [i32, vp, vp, vp, vp, i32, f32, vp, vp, vp]
         L.gh_output_fields.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.gh_output_fields_ploidy.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.gh_run_set_ploidy.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -287,8 +289,9 @@ class Panel:
         return out
 
 
-def output_fields(gp0, gp1, hds, cref, calt):
-    """gh_output_fields: the writer's DS, GP, GT and per-site INFO / AF / RAF."""
+def output_fields(gp0, gp1, hds, cref, calt, ploidy=None):
+    """gh_output_fields (gh_output_fields_ploidy with ploidy [T]): the writer's
+    DS, GP, GT and per-site INFO / AF / RAF."""
     gp0 = np.ascontiguousarray(gp0, np.float32)
     gp1 = np.ascontiguousarray(gp1, np.float32)
     hds = np.ascontiguousarray(hds, np.uint8)
@@ -297,8 +300,14 @@ def output_fields(gp0, gp1, hds, cref, calt):
     T, L = gp0.shape
     out = dict(ds=np.zeros((T, L), np.float32), gp=np.zeros((T, L, 3), np.float32), gt=np.zeros((T, L, 2), np.uint8),
                info=np.zeros(L, np.float32), af=np.zeros(L, np.float32), raf=np.zeros(L, np.float32))
-    _check(lib().gh_output_fields(T, L, _p(gp0), _p(gp1), _p(hds), _p(cref), _p(calt), _p(out["ds"]), _p(out["gp"]),
-                                  _p(out["gt"]), _p(out["info"]), _p(out["af"]), _p(out["raf"])))
+    if ploidy is None:
+        _check(lib().gh_output_fields(T, L, _p(gp0), _p(gp1), _p(hds), _p(cref), _p(calt), _p(out["ds"]),
+                                      _p(out["gp"]), _p(out["gt"]), _p(out["info"]), _p(out["af"]), _p(out["raf"])))
+    else:
+        pl = np.ascontiguousarray(np.broadcast_to(np.asarray(ploidy, np.int32), (T,)), np.int32)
+        _check(lib().gh_output_fields_ploidy(T, L, _p(pl), _p(gp0), _p(gp1), _p(hds), _p(cref), _p(calt),
+                                             _p(out["ds"]), _p(out["gp"]), _p(out["gt"]), _p(out["info"]),
+                                             _p(out["af"]), _p(out["raf"])))
     return out
 
 
@@ -349,7 +358,7 @@ class Run:
     releases the GIL while it runs, so runs on different panels (streams) can
     be driven concurrently from Python threads."""
 
-    def __init__(self, panel, gl, flat, params, kpbwt_per_target=None):
+    def __init__(self, panel, gl, flat, params, kpbwt_per_target=None, ploidy=None):
         self.panel = panel
         self.gl = np.ascontiguousarray(gl, np.uint8)
         self.flat = np.ascontiguousarray(flat, np.uint8)
@@ -362,6 +371,9 @@ class Run:
         _check(lib().gh_run_create(panel.h, self.T, _p(self.gl), _p(self.flat), ctypes.byref(params), _p(kpt),
                                    ctypes.byref(h)))
         self.h = h
+        if ploidy is not None:
+            pl = np.ascontiguousarray(np.broadcast_to(np.asarray(ploidy, np.int32), (self.T,)), np.int32)
+            _check(lib().gh_run_set_ploidy(self.h, _p(pl)))
 
     def execute(self):
         _check(lib().gh_run_execute(self.h))

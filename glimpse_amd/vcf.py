@@ -30,7 +30,7 @@ class VcfOut(ctypes.Structure):
                 ("n_sites", ctypes.c_int32), ("pos", vp), ("id", ctypes.POINTER(cp)), ("ref", ctypes.POINTER(cp)),
                 ("alt", ctypes.POINTER(cp)), ("gt", vp), ("ds", vp), ("gp", vp), ("info", vp), ("af", vp),
                 ("raf", vp), ("n_main", ctypes.c_int32), ("fploidy", ctypes.c_int32), ("source", cp),
-                ("file_date", cp)]
+                ("file_date", cp), ("ploidy", vp)]
 
 
 _bound = False
@@ -151,10 +151,14 @@ def read_target_gls(path, pos, ref, alt, use_gl=False, use_indels=False):
         r.close()
 
 
-def write_genotypes(path, contig, samples, pos, ref, alt, fields, ids=None, n_main=15, fploidy=2,
-                    source="GLIMPSE_phase (MI355X)", file_date=""):
+def write_genotypes(path, contig, samples, pos, ref, alt, fields, ids=None, n_main=15, fploidy=None,
+                    source="GLIMPSE_phase (MI355X)", file_date="", ploidy=None):
     """gh_write_genotypes from the dict gh_output_fields returns
-    (gt [T][L][2], ds [T][L], gp [T][L][3], info / af / raf [L])."""
+    (gt [T][L][2], ds [T][L], gp [T][L][3], info / af / raf [L]); ploidy [T]
+    (None: all diploid) also sets the format ploidy (2, 1 or -2 when mixed)."""
+    pl = None if ploidy is None else np.ascontiguousarray(ploidy, np.int32)
+    if fploidy is None:
+        fploidy = 2 if pl is None or (pl == 2).all() else (1 if (pl == 1).all() else -2)
     pos = np.ascontiguousarray(pos, np.int32)
     T, L = len(samples), pos.shape[0]
     gt = np.ascontiguousarray(fields["gt"], np.uint8)
@@ -169,5 +173,6 @@ def write_genotypes(path, contig, samples, pos, ref, alt, fields, ids=None, n_ma
     aa, k3 = _strs(alt)
     ia, k4 = _strs(ids) if ids is not None else (None, None)
     d = VcfOut(contig.encode(), T, sa, L, hip._p(pos), ia, ra, aa, hip._p(gt), hip._p(ds), hip._p(gp),
-               hip._p(info), hip._p(af), hip._p(raf), n_main, fploidy, source.encode(), file_date.encode())
+               hip._p(info), hip._p(af), hip._p(raf), n_main, fploidy, source.encode(), file_date.encode(),
+               hip._p(pl))
     hip._check(_lib().gh_write_genotypes(str(path).encode(), ctypes.byref(d)))

@@ -211,6 +211,15 @@ int gh_run_set_workspace(gh_run *r, gh_workspace *w);
  * GH_PHASE_SHAPE=<NW>x<KPT> pinning the shapes in mode 0). */
 int gh_run_set_tuning(gh_run *r, int32_t mode);
 
+/* Per-target ploidy (1 or 2; default 2), before the first execution: a haploid
+ * target (genotype with ploidy 1, caller_algorithm.cpp:53-84) takes one
+ * forward-backward pass per iteration with the marginal haplotype
+ * likelihoods of its two GL values (gl[t][l][0..1]), walks one haplotype in
+ * the PBWT selection, is not phased, and stores GP from its one posterior
+ * (genotype.cpp:127-166); its gp0 / gp1 are the two genotype probabilities,
+ * h0 the allele of gp1 > gp0.  The learned selector needs diploid targets. */
+int gh_run_set_ploidy(gh_run *r, const int32_t *ploidy);
+
 /* Output fields as genotype_writer::writeGenotypes computes them
  * (phase/src/io/genotype_writer.cpp:109-190), bit for bit, for T diploid
  * targets at L sites from the finalized run outputs gp0, gp1, hds [T][L] and
@@ -224,6 +233,14 @@ int gh_run_set_tuning(gh_run *r, int32_t mode);
 int gh_output_fields(int32_t T, int32_t L, const float *gp0, const float *gp1, const uint8_t *hds,
                      const uint32_t *cref, const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info,
                      float *af, float *raf);
+/* The same for targets of ploidy[t] 1 or 2 (genotype_writer.cpp:138-190): a
+ * haploid target gets DS = gp1, GP (gp0, gp1) with the third value NaN (the
+ * writer's vector end), gt (gp0 < gp1, 255 = vector end); AF divides by the
+ * number of target haplotypes; INFO is computed only when every target is
+ * diploid, else 1. */
+int gh_output_fields_ploidy(int32_t T, int32_t L, const int32_t *ploidy, const float *gp0, const float *gp1,
+                            const uint8_t *hds, const uint32_t *cref, const uint32_t *calt, float *ds, float *gp,
+                            uint8_t *gt, float *info, float *af, float *raf);
 
 /* ---------------------------------------------------------------------------
  * Learned state selector.  No counterpart in the reference: it is an addition
@@ -482,6 +499,7 @@ typedef struct {
     const float *ds, *gp, *info, *af, *raf;
     int32_t n_main, fploidy;
     const char *source, *file_date;
+    const int32_t *ploidy;         /* [T] 1 or 2; NULL: every sample diploid.  Haploid: one allele, two GP values */
 } gh_vcf_out;
 int gh_write_genotypes(const char *path, const gh_vcf_out *d);
 

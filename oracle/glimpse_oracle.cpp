@@ -1398,6 +1398,23 @@ static void hl_init(int L, const uint8_t *gl, const std::vector<char> &flat, flo
         if (HL[2 * l + 1] < min_gl) { HL[2 * l + 0] = 1.0f - min_gl; HL[2 * l + 1] = min_gl; }
     }
 }
+/* initHaplotypeLikelihoods for ploidy 1 (genotype.cpp:56-90): the pair of GL values, normalised */
+static void hl_init_hap(int L, const uint8_t *gl, const std::vector<char> &flat, float min_gl, std::vector<float> &HL) {
+    for (int l = 0; l < L; l++) {
+        if (!flat[l]) {
+            float t0 = UNPHRED[gl[3 * l + 0]], t1 = UNPHRED[gl[3 * l + 1]], t2 = 0.0f;
+            float s = t0 + t1 + t2;
+            t0 /= s; t1 /= s;
+            HL[2 * l + 0] = t0;
+            HL[2 * l + 1] = t1;
+        } else {
+            HL[2 * l + 0] = 0.5f;
+            HL[2 * l + 1] = 0.5f;
+        }
+        if (HL[2 * l + 0] < min_gl) { HL[2 * l + 0] = min_gl; HL[2 * l + 1] = 1.0f - min_gl; }
+        if (HL[2 * l + 1] < min_gl) { HL[2 * l + 0] = 1.0f - min_gl; HL[2 * l + 1] = min_gl; }
+    }
+}
 static void hl_make(int L, const uint8_t *gl, const std::vector<char> &flat, const std::vector<uint8_t> &other,
                     float min_gl, std::vector<float> &HL) {
     for (int l = 0; l < L; l++) {
@@ -1417,6 +1434,7 @@ struct Stored { float gp0, gp1; bool hds; int32_t idx; };
 
 struct Target {
     int L;
+    int ploidy = 2;
     const uint8_t *gl;
     std::vector<char> flat;
     std::vector<uint8_t> H0, H1;
@@ -1448,6 +1466,27 @@ struct Target {
         }
         stored_cnt++;
     }
+
+    /* genotype.cpp:127-166, haploid */
+    void store_hap(const std::vector<float> &HP0) {
+        std::vector<bool> flag(L, false);
+        for (auto &e : stored) {
+            const int v = e.idx;
+            const float p0 = HP0[2 * v], p1 = HP0[2 * v + 1];
+            const float sc = 1.0f / (p0 + p1);
+            e.gp0 += p0 * sc;
+            e.gp1 += p1 * sc;
+            e.hds = false;
+            flag[v] = true;
+        }
+        for (int l = 0; l < L; l++) {
+            if (flag[l]) continue;
+            const float p0 = HP0[2 * l], p1 = HP0[2 * l + 1];
+            const float sc = 1.0f / (p0 + p1);
+            if (p0 * sc < 0.99999f) stored.push_back({p0 * sc + (stored_cnt % 16) * 1.0f, p1 * sc, false, l});
+        }
+        stored_cnt++;
+    }
 };
 
 /* haplotype_set.cpp:108-169 */
@@ -1457,9 +1496,13 @@ static void init_rare_tar(const Panel &P, Target &G) {
         if (G.flat[v]) continue;
         const int mg = 2 * P.major[v];
         if (!P.common[v] && !P.lq[v]) {
-            const uint8_t *gl = G.gl + 3 * v;
+            /* a haploid target's GL array holds two values per site: the
+             * reference's offsets v*2 + {0, 1, 2} reach the next site's first
+             * value (0 past the last site); its third probability is 0 */
+            uint8_t gl[3] = {G.gl[3 * v], G.gl[3 * v + 1], G.gl[3 * v + 2]};
+            if (G.ploidy == 1) gl[2] = v + 1 < P.L ? G.gl[3 * (v + 1)] : 0;
             if (gl[1] <= gl[mg] || gl[2 - mg] <= gl[mg]) {
-                float t[3] = {UNPHRED[gl[0]], UNPHRED[gl[1]], UNPHRED[gl[2]]};
+                float t[3] = {UNPHRED[gl[0]], UNPHRED[gl[1]], G.ploidy == 1 ? 0.0f : UNPHRED[gl[2]]};
                 float s = t[0] + t[1] + t[2];
                 t[0] /= s; t[1] /= s; t[2] /= s;
                 const float af = P.calt[v] * 1.0 / P.N;
@@ -1539,6 +1582,7 @@ static void finalize_target(Target &g, int L, float *gp0, float *gp1, uint8_t *h
     for (auto &e : g.stored) {
         const size_t o = e.idx;
         gp0[o] = e.gp0; gp1[o] = e.gp1; hds[o] = e.hds;
+        if (g.ploidy == 1) { h0[o] = e.gp1 > e.gp0; h1[o] = 0; continue; }   /* infer_haploid */
         const float g2 = 1.0f - e.gp1 - e.gp0;
         int gt = 0;
         if (e.gp0 > e.gp1 && e.gp0 > g2) gt = 0;
@@ -1567,6 +1611,8 @@ static void phase_run(const Panel &P, int T, const uint8_t *gl, const uint8_t *f
         G[t].L = L; G[t].gl = gl + (size_t)t * L * 3;
         G[t].flat.assign(flat + (size_t)t * L, flat + (size_t)(t + 1) * L);
         G[t].H0.assign(L, 0); G[t].H1.assign(L, 0);
+        G[t].ploidy = prm.ploidy ? prm.ploidy[t] : 2;
+        if (G[t].ploidy != 1 && G[t].ploidy != 2) throw OrcError("ploidy must be 1 or 2");
     }
     /* PBWT classes: targets sharing an effective modulo share groups and stored draws */
     std::map<float, std::vector<int>> classes;
@@ -1616,16 +1662,19 @@ static void phase_run(const Panel &P, int T, const uint8_t *gl, const uint8_t *f
                     stored[s] = true;
                     lo = grp[gi];
                 }
+                /* target haplotypes: two per diploid target, one per haploid one
+                 * (tar_hapid2ind, haplotype_set.cpp:263-270) */
                 std::vector<std::vector<uint8_t>> haps;
-                std::vector<int> start;
-                for (int t : tg) {
-                    haps.push_back(G[t].H0); haps.push_back(G[t].H1);
+                std::vector<int> start, h2s, hid;
+                for (size_t i = 0; i < tg.size(); i++) {
+                    const int t = tg[i];
+                    haps.push_back(G[t].H0); h2s.push_back((int)i); hid.push_back(2 * t);
+                    if (G[t].ploidy > 1) { haps.push_back(G[t].H1); h2s.push_back((int)i); hid.push_back(2 * t + 1); }
                 }
-                for (size_t e = 0; e < haps.size(); e++) {
-                    int t = tg[e / 2];
-                    start.push_back((int)rng.getInt(0, N - 1, P_PBWT_START, (uint32_t)(2 * t + (e & 1)), 0u));
-                }
+                for (size_t e = 0; e < haps.size(); e++)
+                    start.push_back((int)rng.getInt(0, N - 1, P_PBWT_START, (uint32_t)hid[e], 0u));
                 Sweep S(P, haps, (int)tg.size(), prm.pbwt_depth, em);
+                S.hap2sample = h2s;
                 S.run(stored, start);
                 for (size_t i = 0; i < tg.size(); i++) pbwt_states[tg[i]] = S.states[i];
             }
@@ -1639,6 +1688,13 @@ static void phase_run(const Panel &P, int T, const uint8_t *gl, const uint8_t *f
             else if (kp[t] >= N) { states.resize(N); std::iota(states.begin(), states.end(), 0); }
             w.C.build(states);
             if (nstates_out) nstates_out[t * n_iter + it] = (int32_t)w.C.n_states;
+            if (g.ploidy == 1) {   /* caller_algorithm.cpp:58-77 with ploidy 1 */
+                hl_init_hap(L, g.gl, g.flat, prm.min_gl, w.HL);
+                w.HMM.run(w.HL, g.flat, w.HP0);
+                for (int l = 0; l < L; l++) g.H0[l] = r.getFloat(P_H0, (uint32_t)t, (uint32_t)l) > w.HP0[2 * l];
+                if (stage == 2) g.store_hap(w.HP0);
+                return;
+            }
             if (stage == 0) hl_init(L, g.gl, g.flat, prm.min_gl, w.HL);
             else hl_make(L, g.gl, g.flat, g.H1, prm.min_gl, w.HL);
             w.HMM.run(w.HL, g.flat, w.HP0);
@@ -1958,23 +2014,35 @@ int orc_draws(uint32_t seed, int32_t it, int32_t purpose, int32_t a, int32_t n, 
     });
 }
 
-/* genotype_writer::writeGenotypes per-site fields for diploid targets
+/* genotype_writer::writeGenotypes per-site fields for diploid and haploid targets
  * (phase/src/io/genotype_writer.cpp:109-190): DS, GP with the writer's
  * floor-then-raise rounding, phased GT, INFO / AF / RAF.  A target whose
  * genotype is not stored at a site enters as gp = (1, 0). */
-int orc_output_fields(int32_t T, int32_t L, const float *gp0s, const float *gp1s, const uint8_t *hdss,
-                      const uint32_t *cref, const uint32_t *calt, float *ds_out, float *gp_out, uint8_t *gt_out,
-                      float *info_out, float *af_out, float *raf_out) {
+int orc_output_fields_ploidy(int32_t T, int32_t L, const int32_t *ploidy, const float *gp0s, const float *gp1s,
+                             const uint8_t *hdss, const uint32_t *cref, const uint32_t *calt, float *ds_out,
+                             float *gp_out, uint8_t *gt_out, float *info_out, float *af_out, float *raf_out) {
     ORC_TRY(
+        int n_tar_haps = 0, n_hap = 0;
+        for (int i = 0; i < T; i++) {
+            const int pl = ploidy ? ploidy[i] : 2;
+            n_tar_haps += pl;
+            n_hap += pl == 1;
+        }
+        /* format ploidy (genotype_reader.cpp:553-554): 2, 1, or -2 when mixed */
+        const int fploidy = n_hap == 0 ? 2 : (n_hap == T ? 1 : -2);
         for (int l = 0; l < L; l++) {
             float ds_sum = 0.0, ds2_sum = 0.0, ds4_sum = 0.0;
             for (int i = 0; i < T; i++) {
                 const size_t o = (size_t)i * L + l;
                 const float gp0 = gp0s[o], gp1 = gp1s[o];
-                const float gp2 = std::clamp(1.0f - gp1 - gp0, 0.0f, 1.0f);
-                const float ds = gp1 + 2.0f * gp2;
+                const bool hap = ploidy && ploidy[i] == 1;
+                const float gp2 = hap ? 0.0f : std::clamp(1.0f - gp1 - gp0, 0.0f, 1.0f);
+                const float ds = hap ? gp1 : gp1 + 2.0f * gp2;
                 uint8_t g[2];
-                if (gp1 > gp0 && gp1 > gp2) {
+                if (hap) {
+                    g[0] = gp0 < gp1;
+                    g[1] = 255;   /* bcf_int32_vector_end */
+                } else if (gp1 > gp0 && gp1 > gp2) {
                     const bool hds = hdss[o] != 0;
                     g[hds] = 1;
                     g[1 - hds] = 0;
@@ -1988,28 +2056,42 @@ int orc_output_fields(int32_t T, int32_t L, const float *gp0s, const float *gp1s
                 std::map<float, float *> mp;
                 mp.insert(std::make_pair(1.0f - (gp0 - P[0]), &P[0]));
                 mp.insert(std::make_pair(1.0f - (gp1 - P[1]), &P[1]));
-                P[2] = floorf(std::max(1.0f - (P[0] + P[1]), 0.0f) * 1000.0) / 1000.0;
-                mp.insert(std::make_pair(1.0f - (gp2 - P[2]), &P[2]));
-                for (auto it = mp.begin(); it != mp.end() && std::accumulate(P, P + 3, 0.0f) < 0.9999f; ++it)
+                const int n_gp = hap ? 2 : 3;
+                if (!hap) {
+                    P[2] = floorf(std::max(1.0f - (P[0] + P[1]), 0.0f) * 1000.0) / 1000.0;
+                    mp.insert(std::make_pair(1.0f - (gp2 - P[2]), &P[2]));
+                } else {
+                    P[2] = std::numeric_limits<float>::quiet_NaN();   /* bcf_float_vector_end */
+                }
+                for (auto it = mp.begin(); it != mp.end() && std::accumulate(P, P + n_gp, 0.0f) < 0.9999f; ++it)
                     *it->second += 0.001f;
                 std::memcpy(gp_out + 3 * o, P, sizeof(P));
                 gt_out[2 * o] = g[0];
                 gt_out[2 * o + 1] = g[1];
                 ds_sum += ds;
-                ds2_sum += ds * ds;
-                ds4_sum += gp1 + 4.0 * gp2;
+                if (fploidy == 2) {
+                    ds2_sum += ds * ds;
+                    ds4_sum += gp1 + 4.0 * gp2;
+                }
             }
-            const int n_tar_haps = 2 * T;
             const float raf = calt[l] * 1.0f / (calt[l] + cref[l]);
             const float af = ds_sum / n_tar_haps;
-            float info = (af > 0.0 && af < 1.0) ? (float)(1.0 - (ds4_sum - ds2_sum) / (n_tar_haps * af * (1.0 - af)))
-                                                : 1.0f;
+            float info = (fploidy == 2 && af > 0.0 && af < 1.0)
+                             ? (float)(1.0 - (ds4_sum - ds2_sum) / (n_tar_haps * af * (1.0 - af)))
+                             : 1.0f;
             info = (info < 0.0f) ? 0.0f : info;
             info = roundf(info * 1000.0) / 1000.0;
             info_out[l] = info;
             af_out[l] = af;
             raf_out[l] = raf;
         })
+}
+
+int orc_output_fields(int32_t T, int32_t L, const float *gp0s, const float *gp1s, const uint8_t *hdss,
+                      const uint32_t *cref, const uint32_t *calt, float *ds_out, float *gp_out, uint8_t *gt_out,
+                      float *info_out, float *af_out, float *raf_out) {
+    return orc_output_fields_ploidy(T, L, nullptr, gp0s, gp1s, hdss, cref, calt, ds_out, gp_out, gt_out, info_out,
+                                    af_out, raf_out);
 }
 
 } // extern "C"

@@ -52,6 +52,7 @@ typedef struct {
     int32_t n_threads;          /* --threads: targets of an iteration spread over threads after the
                                    serial PBWT sweep (caller_algorithm.cpp:94-116); counter mode only,
                                    the reference's shared mt19937 makes threaded runs irreproducible */
+    const int32_t *ploidy;      /* [T] 1 or 2 per target (the --samples-file ploidy); NULL: all diploid */
 } orc_params;
 
 void orc_default_params(orc_params *p);
@@ -150,6 +151,9 @@ void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4
 /* The n-th output (1-based) of a default-seeded std::mt19937. */
 uint32_t orc_mt19937_nth(uint32_t seed, int64_t n);
 float orc_unphred(int32_t i);
+int orc_output_fields_ploidy(int32_t T, int32_t L, const int32_t *ploidy, const float *gp0, const float *gp1,
+                             const uint8_t *hds, const uint32_t *cref, const uint32_t *calt, float *ds, float *gp,
+                             uint8_t *gt, float *info, float *af, float *raf);
 int orc_output_fields(int32_t T, int32_t L, const float *gp0, const float *gp1, const uint8_t *hds,
                       const uint32_t *cref, const uint32_t *calt, float *ds, float *gp, uint8_t *gt, float *info,
                       float *af, float *raf);

@@ -24,7 +24,8 @@ class Params(ctypes.Structure):
                 ("min_gl", ctypes.c_float), ("err_imp", ctypes.c_float), ("err_phase", ctypes.c_float),
                 ("pbwt_depth", ctypes.c_int32), ("pbwt_modulo_cm", ctypes.c_float),
                 ("kinit", ctypes.c_int32), ("kpbwt", ctypes.c_int32), ("sparse_maf", ctypes.c_float),
-                ("seed", ctypes.c_uint32), ("rng_mode", ctypes.c_int32), ("n_threads", ctypes.c_int32)]
+                ("seed", ctypes.c_uint32), ("rng_mode", ctypes.c_int32), ("n_threads", ctypes.c_int32),
+                ("ploidy", ctypes.c_void_p)]
 
 
 def build():
@@ -77,6 +78,7 @@ def lib():
         L.orc_unphred.restype = ctypes.c_float
         L.orc_unphred.argtypes = [ctypes.c_int32]
         L.orc_output_fields.argtypes = [ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 11
+        L.orc_output_fields_ploidy.argtypes = [ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 12
         i32, u32 = ctypes.c_int32, ctypes.c_uint32
         L.orc_sample.argtypes = [i32, i32, i32, u32, i32, vp, vp]
         L.orc_init_states.argtypes = [vp, vp, vp, i32, i32, u32, i32, vp, i32, vp]
@@ -269,7 +271,7 @@ class Panel:
                                      self.N, ctypes.byref(n)))
         return out[:n.value]
 
-    def phase_run(self, gl, flat, params, kpbwt_per_target=None):
+    def phase_run(self, gl, flat, params, kpbwt_per_target=None, ploidy=None):
         gl = np.ascontiguousarray(gl, np.uint8)
         flat = np.ascontiguousarray(flat, np.uint8)
         T, L = flat.shape
@@ -281,6 +283,12 @@ class Panel:
         h1 = np.zeros((T, L), np.uint8)
         ns = np.zeros((T, n_iter), np.int32)
         kpt = None if kpbwt_per_target is None else np.ascontiguousarray(kpbwt_per_target, np.int32)
+        pl = None
+        if ploidy is not None:
+            pl = np.ascontiguousarray(np.broadcast_to(np.asarray(ploidy, np.int32), (T,)), np.int32)
+            params.ploidy = pl.ctypes.data
+        else:
+            params.ploidy = None
         _check(lib().orc_phase_run(self.h, T, _p(gl), _p(flat), ctypes.byref(params), _p(kpt),
                                    _p(gp0), _p(gp1), _p(hds), _p(h0), _p(h1), _p(ns)))
         return dict(gp0=gp0, gp1=gp1, hds=hds, h0=h0, h1=h1, nstates=ns)
@@ -313,8 +321,8 @@ def unphred(i):
     return lib().orc_unphred(i)
 
 
-def output_fields(gp0, gp1, hds, cref, calt):
-    """genotype_writer per-site fields (DS, GP, GT, INFO, AF, RAF) for diploid targets."""
+def output_fields(gp0, gp1, hds, cref, calt, ploidy=None):
+    """genotype_writer per-site fields (DS, GP, GT, INFO, AF, RAF); ploidy [T] (None: diploid)."""
     gp0 = np.ascontiguousarray(gp0, np.float32)
     gp1 = np.ascontiguousarray(gp1, np.float32)
     hds = np.ascontiguousarray(hds, np.uint8)
@@ -324,8 +332,10 @@ def output_fields(gp0, gp1, hds, cref, calt):
     out = dict(ds=np.zeros((T, L), np.float32), gp=np.zeros((T, L, 3), np.float32), gt=np.zeros((T, L, 2), np.uint8),
                info=np.zeros(L, np.float32), af=np.zeros(L, np.float32), raf=np.zeros(L, np.float32))
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    rc = lib().orc_output_fields(T, L, p(gp0), p(gp1), p(hds), p(cref), p(calt), p(out["ds"]), p(out["gp"]),
-                                 p(out["gt"]), p(out["info"]), p(out["af"]), p(out["raf"]))
+    pl = None if ploidy is None else np.ascontiguousarray(np.broadcast_to(np.asarray(ploidy, np.int32), (T,)), np.int32)
+    rc = lib().orc_output_fields_ploidy(T, L, None if pl is None else p(pl), p(gp0), p(gp1), p(hds), p(cref), p(calt),
+                                        p(out["ds"]), p(out["gp"]), p(out["gt"]), p(out["info"]), p(out["af"]),
+                                        p(out["raf"]))
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())
     return out

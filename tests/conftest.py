@@ -64,3 +64,20 @@ def nrc(g, truth):
     mism = int(np.sum(g != t))
     match_nonref = int(np.sum((g == t) & (t > 0)))
     return 1.0 - mism / max(1, mism + match_nonref)
+
+
+def haploid_gls(P, targets, cov=2.0, seed=3):
+    """Diploid-shaped GL arrays where the listed targets are haploid: their
+    GLs are (PL of the hom-ref, PL of the hom-alt) of a homozygous read set,
+    rescaled to a minimum of 0."""
+    from glimpse_amd import synth
+    haps = P["tgt_haps"].copy()
+    for t in targets:
+        haps[2 * t + 1] = haps[2 * t]
+    gl, flat = synth.make_reads(haps, cov, seed=seed)
+    for t in targets:
+        g = np.stack([gl[t, :, 0], gl[t, :, 2]], -1).astype(np.int32)
+        g -= g.min(axis=1, keepdims=True)
+        gl[t, :, 0], gl[t, :, 1], gl[t, :, 2] = g[:, 0], g[:, 1], 0
+        flat[t] |= (g[:, 0] == g[:, 1]).astype(np.uint8)
+    return gl, flat
