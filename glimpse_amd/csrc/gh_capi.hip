@@ -26,6 +26,7 @@ void run_fetch(gh_run *G, float *gp0_h, float *gp1_h, uint8_t *hds_h, uint8_t *h
 void run_destroy(gh_run *G);
 void run_collect(gh_run *G, int from_iter);
 void run_set_tuning(gh_run *G, int mode);
+void run_set_graph(gh_run *G, int mode);
 void workspace_create(gh_workspace **out);
 void workspace_destroy(gh_workspace *w);
 int64_t workspace_bytes(const gh_workspace *w);
@@ -488,6 +489,10 @@ int gh_run_set_workspace(gh_run *r, gh_workspace *w) {
 
 int gh_run_set_tuning(gh_run *r, int32_t mode) {
     return guard([&] { run_set_tuning(r, mode); });
+}
+
+int gh_run_set_graph(gh_run *r, int32_t mode) {
+    return guard([&] { run_set_graph(r, mode); });
 }
 int gh_run_collect(gh_run *r, int32_t from_iter) {
     return guard([&] { run_collect(r, from_iter); });

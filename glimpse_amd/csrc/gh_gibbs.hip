@@ -26,6 +26,7 @@
 #include <thread>
 
 #include "gh_device.hpp"
+#include "gh_graph.hpp"
 #include "gh_internal.hpp"
 #include "gh_pbwt.hpp"
 #include "gh_phase.hpp"
@@ -88,11 +89,13 @@ __global__ void hl_kernel(const uint8_t *__restrict__ gl, const uint8_t *__restr
 }
 
 // genotype::sampleHaplotypeH0/H1 (genotype.cpp:118-125) with counter draws
+// (iter_dev, when set: the iteration is read from the device, for replayed iterations)
 __global__ void sample_kernel(const float *__restrict__ HP, uint8_t *__restrict__ H, int L, int which, uint32_t seed,
-                              uint32_t it) {
+                              uint32_t it, const uint32_t *__restrict__ iter_dev) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     if (l >= L) return;
+    if (iter_dev) it = *iter_dev;
     const float u = u01(philox_draw(seed, it, which ? P_H1 : P_H0, (uint32_t)t, (uint32_t)l));
     H[(size_t)(2 * t + which) * L + l] = u > HP[((size_t)t * L + l) * 2];
 }
@@ -102,13 +105,16 @@ __global__ void sample_kernel(const float *__restrict__ HP, uint8_t *__restrict_
 // implicit (1, 0); from its first storage on, the actual normalised GP.
 // A haploid target stores the normalised HP0 pair (genotype.cpp:127-166; its
 // first-storage offset is stored_cnt % 16, equal to stored_cnt for --main <= 15).
+// (iter_dev, when set: a replayed iteration, which stores only from iteration first_main on)
 __global__ void gp_kernel(const float *__restrict__ HP0, const float *__restrict__ HP1, int L,
                           const uint8_t *__restrict__ ploidy, float *__restrict__ gp0, float *__restrict__ gp1,
-                          uint8_t *__restrict__ stored, uint8_t *__restrict__ hds) {
+                          uint8_t *__restrict__ stored, uint8_t *__restrict__ hds,
+                          const uint32_t *__restrict__ iter_dev, uint32_t first_main) {
 #pragma clang fp contract(off)
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     if (l >= L) return;
+    if (iter_dev && *iter_dev < first_main) return;
     const size_t o = (size_t)t * L + l;
     if (ploidy && ploidy[t] == 1) {
         const float p0 = HP0[2 * o], p1 = HP0[2 * o + 1];
@@ -386,6 +392,10 @@ struct gh_run {
     int64_t col_rows = 0;
     bool masks_ready = false;
     gh::DBuf d_klearn, d_cov, d_cnt, d_tmask, d_cx, d_cy, d_cw, d_sgrp, d_tcls;
+    // replayed iterations (gh_graph.hpp)
+    int graph_mode = 1;        // 1: iterations from 2 on replay a captured graph; 0: all launched from the host
+    gh::GraphCaps need{};      // largest placement needs of the iterations launched from the host
+    gh::DBuf d_ctl, d_nst, d_gptr, d_gend;
     ~gh_run() {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
@@ -563,8 +573,12 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
     }
 }
 
-void run_execute(gh_run *G) {
-    GH_CHECK(G, GH_ERR_ARG, "gh_run_execute: null run");
+// One execution of the schedule.  With use_graph, iterations 0 and 1 are
+// launched from the host and iteration 2 is captured as a HIP graph, which is
+// replayed for every later iteration (gh_graph.hpp).  Returns false when a
+// replayed iteration needed more placed memory than the captured buffers hold
+// (its results are then invalid and the caller executes again without the graph).
+static bool run_execute_once(gh_run *G, bool use_graph) {
     const auto wall0 = std::chrono::steady_clock::now();
     gh_panel *P = G->P;
     hipStream_t s = G->stream;
@@ -687,7 +701,59 @@ void run_execute(gh_run *G) {
     std::vector<CondItem> &items = G->items;
     std::vector<PhItem> &ph = G->ph;
 
-    for (int it = 0; it < n_iter; it++) {
+    // learned selection (features, network, top-k) and training-row windows
+    // of one PBWT iteration, on s
+    auto learned_select = [&](int it, bool col_it) {
+        FeatArgs fa;
+        fa.hm = P->d_hm.as<uint64_t>(); fa.Wc = P->Wc; fa.Lc = P->Lc; fa.N = N; fa.L = L;
+        fa.depth = prm->pbwt_depth; fa.kstride = kst; fa.rstride = prs;
+        fa.c2t = P->d_c2t.as<int32_t>(); fa.crow = P->d_crow.as<int32_t>(); fa.flags = P->d_flags.as<uint8_t>();
+        fa.ins_site = Wk->isite.as<int32_t>();
+        fa.rare_ptr = P->d_rare_ptr.as<int32_t>(); fa.rare_idx = P->d_rare_idx.as<int32_t>();
+        fa.H = H; fa.tmask = G->d_tmask.as<uint64_t>(); fa.targets = ptg; fa.Kpool = Kdev;
+        fa.ins = Wk->ins.as<int32_t>();
+        if (learned) {
+            fa.X = Wk->fx.as<float>();
+            fa.row0 = 0;
+        } else {
+            const int64_t need = G->col_rows + (int64_t)npt * prs;
+            if ((int64_t)(G->d_cx.bytes / (sizeof(float) * MLP_FEAT)) < need) {
+                // grow the row store (rare: sized for all collected iterations up front)
+                const int64_t cap = (int64_t)npt * prs * std::max(1, n_iter - G->collect_from);
+                GH_CHECK(G->col_rows == 0, GH_ERR_CAPACITY, "training-row store overflow");
+                G->d_cx.reserve(sizeof(float) * (size_t)cap * MLP_FEAT);
+                G->d_cy.reserve(sizeof(float) * (size_t)cap);
+                G->d_cw.reserve(sizeof(float) * (size_t)cap);
+            }
+            fa.X = G->d_cx.as<float>();
+            fa.row0 = G->col_rows;
+        }
+        feature_launch(P, npt, fa, s);
+        if (col_it) {
+            WinArgs wa;
+            wa.targets = ptg; wa.Kpool = Kdev; wa.states = states; wa.ins = Wk->ins.as<int32_t>();
+            wa.ins_site = Wk->isite.as<int32_t>(); wa.c2t = P->d_c2t.as<int32_t>();
+            wa.crow = P->d_crow.as<int32_t>(); wa.Lc = P->Lc; wa.Wc = P->Wc; wa.kstride = kst;
+            wa.win_lo = Wk->wlo.as<int32_t>(); wa.win_hi = Wk->whi.as<int32_t>();
+            win_launch(npt, wa, s);
+        }
+        if (learned) {
+            mlp_forward_launch(G->mlp->w.as<float>(), G->mlp->wb.as<uint16_t>(), Wk->fx.as<float>(),
+                               (int64_t)npt * prs, Wk->fy.as<float>(), s);
+            TopkArgs ta;
+            ta.targets = ptg; ta.Kpool = Kdev; ta.klearn = G->d_klearn.as<int32_t>();
+            ta.y = Wk->fy.as<float>(); ta.row0 = 0; ta.kstride = kst; ta.rstride = prs;
+            ta.min_mass = G->min_mass;
+            ta.ins = Wk->ins.as<int32_t>(); ta.states = states; ta.Kdev = Kdev; ta.items = items_d;
+            ta.coverage = G->d_cov.as<float>(); ta.counters = G->d_cnt.as<int32_t>();
+            ta.ins_site = G->sel_mode == 1 ? Wk->isite.as<int32_t>() : nullptr;
+            ta.site_grp = G->d_sgrp.as<int32_t>(); ta.tcls = G->d_tcls.as<int32_t>(); ta.L = L;
+            topk_launch(npt, ta, s);
+        }
+    };
+
+    const int n_host = use_graph ? std::min(n_iter, 2) : n_iter;   // iterations launched from the host
+    for (int it = 0; it < n_host; it++) {
         const int stage = it == 0 ? 0 : (it <= prm->burnin ? 1 : 2);
         bool learn_ev = false;
         GH_HIP(hipEventRecord(e_sel.a, s));
@@ -758,52 +824,7 @@ void run_execute(gh_run *G) {
             const bool col_it = collecting && it >= G->collect_from;
             if (learned || col_it) {
                 GH_HIP(hipEventRecord(e_learn.a, s));
-                FeatArgs fa;
-                fa.hm = P->d_hm.as<uint64_t>(); fa.Wc = P->Wc; fa.Lc = P->Lc; fa.N = N; fa.L = L;
-                fa.depth = prm->pbwt_depth; fa.kstride = kst; fa.rstride = prs;
-                fa.c2t = P->d_c2t.as<int32_t>(); fa.crow = P->d_crow.as<int32_t>(); fa.flags = P->d_flags.as<uint8_t>();
-                fa.ins_site = Wk->isite.as<int32_t>();
-                fa.rare_ptr = P->d_rare_ptr.as<int32_t>(); fa.rare_idx = P->d_rare_idx.as<int32_t>();
-                fa.H = H; fa.tmask = G->d_tmask.as<uint64_t>(); fa.targets = ptg; fa.Kpool = Kdev;
-                fa.ins = Wk->ins.as<int32_t>();
-                if (learned) {
-                    fa.X = Wk->fx.as<float>();
-                    fa.row0 = 0;
-                } else {
-                    const int64_t need = G->col_rows + (int64_t)npt * prs;
-                    if ((int64_t)(G->d_cx.bytes / (sizeof(float) * MLP_FEAT)) < need) {
-                        // grow the row store (rare: sized for all collected iterations up front)
-                        const int64_t cap = (int64_t)npt * prs * std::max(1, n_iter - G->collect_from);
-                        GH_CHECK(G->col_rows == 0, GH_ERR_CAPACITY, "training-row store overflow");
-                        G->d_cx.reserve(sizeof(float) * (size_t)cap * MLP_FEAT);
-                        G->d_cy.reserve(sizeof(float) * (size_t)cap);
-                        G->d_cw.reserve(sizeof(float) * (size_t)cap);
-                    }
-                    fa.X = G->d_cx.as<float>();
-                    fa.row0 = G->col_rows;
-                }
-                feature_launch(P, npt, fa, s);
-                if (col_it) {
-                    WinArgs wa;
-                    wa.targets = ptg; wa.Kpool = Kdev; wa.states = states; wa.ins = Wk->ins.as<int32_t>();
-                    wa.ins_site = Wk->isite.as<int32_t>(); wa.c2t = P->d_c2t.as<int32_t>();
-                    wa.crow = P->d_crow.as<int32_t>(); wa.Lc = P->Lc; wa.Wc = P->Wc; wa.kstride = kst;
-                    wa.win_lo = Wk->wlo.as<int32_t>(); wa.win_hi = Wk->whi.as<int32_t>();
-                    win_launch(npt, wa, s);
-                }
-                if (learned) {
-                    mlp_forward_launch(G->mlp->w.as<float>(), G->mlp->wb.as<uint16_t>(), Wk->fx.as<float>(),
-                                       (int64_t)npt * prs, Wk->fy.as<float>(), s);
-                    TopkArgs ta;
-                    ta.targets = ptg; ta.Kpool = Kdev; ta.klearn = G->d_klearn.as<int32_t>();
-                    ta.y = Wk->fy.as<float>(); ta.row0 = 0; ta.kstride = kst; ta.rstride = prs;
-                    ta.min_mass = G->min_mass;
-                    ta.ins = Wk->ins.as<int32_t>(); ta.states = states; ta.Kdev = Kdev; ta.items = items_d;
-                    ta.coverage = G->d_cov.as<float>(); ta.counters = G->d_cnt.as<int32_t>();
-                    ta.ins_site = G->sel_mode == 1 ? Wk->isite.as<int32_t>() : nullptr;
-                    ta.site_grp = G->d_sgrp.as<int32_t>(); ta.tcls = G->d_tcls.as<int32_t>(); ta.L = L;
-                    topk_launch(npt, ta, s);
-                }
+                learned_select(it, col_it);
                 GH_HIP(hipEventRecord(e_learn.b, s));
                 learn_ev = true;
             }
@@ -831,6 +852,9 @@ void run_execute(gh_run *G) {
             grow(Wk->hvar, sizeof(uint64_t) * (size_t)std::max<int64_t>(words, 1));
             grow(Wk->alpha, sizeof(float) * (size_t)std::max<int64_t>(aoff, 1));
             grow(Wk->loci, sizeof(Locus) * (size_t)std::max<int64_t>(loff, 1));
+            G->need.hvar_words = std::max(G->need.hvar_words, words);
+            G->need.alpha_floats = std::max(G->need.alpha_floats, aoff);
+            G->need.loci = std::max(G->need.loci, loff);
         }
         uint64_t *hvar = Wk->hvar.as<uint64_t>();
         float *alpha = Wk->alpha.as<float>();
@@ -905,7 +929,7 @@ void run_execute(gh_run *G) {
                            ploidy_d, HL);
         fb_all(HP0, mass0, e_k0, e_k0b);
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
-        hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it);
+        hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, (uint32_t)it, nullptr);
         // H1 (caller_algorithm.cpp:66-71), diploid targets only
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, ploidy_d, HL);
         if (!G->any_haploid) {
@@ -938,7 +962,7 @@ void run_execute(gh_run *G) {
             GH_HIP(hipEventRecord(e_k1.b, s));
         }
         fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
-        hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it);
+        hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, (uint32_t)it, nullptr);
         GH_HIP(hipGetLastError());
         GH_HIP(hipEventRecord(e_fb.b, s));
         if (label_it) {
@@ -1003,6 +1027,11 @@ void run_execute(gh_run *G) {
             G->nstates[(size_t)t * n_iter + it] = items[t].K;
         }
         GH_HIP(hipMemcpyAsync(ph_d, ph.data(), sizeof(PhItem) * T, hipMemcpyHostToDevice, s));
+        G->need.seg = std::max(G->need.seg, so);
+        G->need.segsum = std::max(G->need.segsum, ss);
+        G->need.imp = std::max(G->need.imp, io);
+        G->need.impsum = std::max(G->need.impsum, is);
+        G->need.dip = std::max(G->need.dip, dd);
         PhaseBufs PB;
         grow(Wk->seg, sizeof(float) * std::max<int64_t>(so, 1));
         grow(Wk->segsum, sizeof(float) * std::max<int64_t>(ss, 1));
@@ -1040,7 +1069,7 @@ void run_execute(gh_run *G) {
         GH_HIP(hipEventRecord(e_pk.b, s));
         GH_HIP(hipEventRecord(e_ph.b, s));
         if (stage == 2) {
-            hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, ploidy_d, gp0, gp1, sto, hds);
+            hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, ploidy_d, gp0, gp1, sto, hds, nullptr, 0u);
             GH_HIP(hipGetLastError());
             main_done++;
         }
@@ -1066,6 +1095,176 @@ void run_execute(gh_run *G) {
         }
         if (learn_ev) st.ms_learn += e_learn.ms();
     }
+    if (n_host < n_iter) {
+        const auto g0 = std::chrono::steady_clock::now();
+        // buffers placed per iteration: what the host-launched iterations
+        // needed, with headroom (GH_GRAPH_HEADROOM, default 1.5; the whole
+        // buffer when it is larger).  A headroom below 1 caps the replays at
+        // that fraction of the need, which forces the repeated execution.
+        const char *eh = std::getenv("GH_GRAPH_HEADROOM");
+        const double head = eh ? std::max(0.01, std::atof(eh)) : 1.5;
+        auto cap_of = [&](DBuf &b, int64_t need, size_t elem) -> int64_t {
+            const int64_t n = (int64_t)std::max(1.0, std::ceil((double)need * head));
+            if ((size_t)n * elem > b.bytes) b.reserve((size_t)n * elem);
+            return head >= 1.0 ? (int64_t)(b.bytes / elem) : n;
+        };
+        GraphCaps caps;
+        caps.hvar_words = cap_of(Wk->hvar, G->need.hvar_words, sizeof(uint64_t));
+        caps.alpha_floats = cap_of(Wk->alpha, G->need.alpha_floats, sizeof(float));
+        caps.loci = cap_of(Wk->loci, G->need.loci, sizeof(Locus));
+        caps.seg = cap_of(Wk->seg, G->need.seg, sizeof(float));
+        caps.segsum = cap_of(Wk->segsum, G->need.segsum, sizeof(float));
+        caps.imp = cap_of(Wk->imp, G->need.imp, sizeof(float));
+        caps.impsum = std::min(cap_of(Wk->impsum, G->need.impsum, sizeof(float)),
+                               cap_of(Wk->of1, G->need.impsum, sizeof(float)));
+        caps.dip = cap_of(Wk->dip, G->need.dip, sizeof(int32_t));
+        GH_HIP(hipStreamSynchronize(s));   // the buffers reallocated above are no longer in use
+        // device tables of the replays
+        GraphCtl c0;
+        std::memset(&c0, 0, sizeof(c0));
+        c0.iter = (uint32_t)n_host;
+        GraphCtl *ctl = dalloc_copy(G->d_ctl, &c0, 1, s);
+        int32_t *nst = (int32_t *)G->d_nst.reserve(sizeof(int32_t) * (size_t)T * n_iter);
+        int32_t *ford = (int32_t *)G->d_ford.reserve(sizeof(int32_t) * 2 * (size_t)T);
+        int32_t *pord = (int32_t *)G->d_pord.reserve(sizeof(int32_t) * 2 * (size_t)T);
+        std::vector<int32_t> gptr(1, 0), gend;
+        for (auto &g : G->cls_grp) {
+            gend.insert(gend.end(), g.begin(), g.end());
+            gptr.push_back((int32_t)gend.size());
+        }
+        const int32_t *d_gptr = dalloc_copy(G->d_gptr, gptr.data(), gptr.size(), s);
+        const int32_t *d_gend = dalloc_copy(G->d_gend, gend.data(), gend.size(), s);
+        PhItem *ph_d = G->d_ph.as<PhItem>();
+        uint64_t *hvar = Wk->hvar.as<uint64_t>();
+        float *alpha = Wk->alpha.as<float>();
+        Locus *loci = Wk->loci.as<Locus>();
+        PhaseBufs PB;
+        PB.seg = Wk->seg.as<float>(); PB.segsum = Wk->segsum.as<float>(); PB.imp = Wk->imp.as<float>();
+        PB.impsum = Wk->impsum.as<float>(); PB.of1 = Wk->of1.as<float>(); PB.dip = Wk->dip.as<int32_t>();
+        if (!G->s2) {
+            GH_HIP(hipStreamCreateWithFlags(&G->s2, hipStreamNonBlocking));
+            GH_HIP(hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming));
+            GH_HIP(hipEventCreateWithFlags(&G->ev_join, hipEventDisableTiming));
+        }
+        GH_HIP(hipStreamSynchronize(s));   // host tables above are copied
+        const uint32_t *itp = &ctl->iter;
+        const int fb_capmax = fb_cap_words(G->maxW), ph_capmax = ph_cap_states(G->kstride);
+        // every launch variant a group can take (capacity class x latency
+        // shape), each gated by the group's key of the iteration
+        auto fb_groups = [&](float *HPx) {
+            GH_HIP(hipEventRecord(G->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
+            for (int cap = 4; cap <= fb_capmax; cap *= 2)
+                for (int lat = G->latency ? 1 : 0; lat <= 1; lat++) {
+                    const int mw = std::min(cap, G->maxW);
+                    const FbPick pick = lat ? fb_default_pick(mw, true) : FbPick{0, 0};
+                    GraphGate ga{&ctl->fb_key[0], graph_key(cap, lat), &ctl->fb_n[0], itp};
+                    fb_launch(P, T, fb_d, items_d, mw, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i,
+                              G->ed_i, s, nullptr, nullptr, nullptr, pick, ford, ga);
+                    if (cap < fb_capmax) {
+                        GraphGate gb{&ctl->fb_key[1], graph_key(cap, lat), &ctl->fb_n[1], itp};
+                        fb_launch(P, T, fb_d, items_d, mw, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i,
+                                  G->ed_i, G->s2, nullptr, nullptr, nullptr, pick, ford + T, gb);
+                    }
+                }
+            GH_HIP(hipEventRecord(G->ev_join, G->s2));
+            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+        };
+        auto ph_groups = [&]() {
+            GH_HIP(hipEventRecord(G->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
+            for (int kc = 256; kc <= ph_capmax; kc *= 2)
+                for (int lat = G->latency ? 1 : 0; lat <= 1; lat++) {
+                    const PhaseShape sh = phase_shape(kc, lat);
+                    GraphGate ga{&ctl->ph_key[0], graph_key(kc, lat), &ctl->ph_n[0], itp};
+                    phase_launch(P, T, kc, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
+                                 0u, status, s, sh, pord, ga);
+                    if (kc < ph_capmax) {
+                        GraphGate gb{&ctl->ph_key[1], graph_key(kc, lat), &ctl->ph_n[1], itp};
+                        phase_launch(P, T, kc, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase,
+                                     seed, 0u, status, G->s2, sh, pord + T, gb);
+                    }
+                }
+            GH_HIP(hipEventRecord(G->ev_join, G->s2));
+            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+        };
+        // one PBWT iteration, every decision on the device
+        auto iteration = [&]() {
+            if (npt > 0) {
+                pbwt_draw_launch(ctl, seed, d_gptr, d_gend, (int)G->cls_grp.size(), P->Lhq, pst, ptg, npt, N, pstart,
+                                 s);
+                pbwt_select_launch(P, R, s);
+                hipLaunchKernelGGL(set_k_kernel, dim3((npt + 127) / 128), dim3(128), 0, s, items_d, Kdev, ptg, npt);
+                GH_HIP(hipGetLastError());
+                if (learned) learned_select(-1, false);
+            }
+            condset_index(P, T, items_d, states, poly, pos, tr, mark, prm->ne, s);
+            fb_place_launch(ctl, items_d, fb_d, ph_d, T, L, fb_block_sites(G->maxW), FEW_ITEMS, G->latency, caps,
+                            ford, nst, n_iter, s);
+            condset_fill(P, T, items_d, states, poly, pos, hvar, G->maxW, s);
+            hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 0, prm->min_gl, ploidy_d, HL);
+            fb_groups(HP0);
+            fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP0, G->ee_i, G->ed_i, s);
+            hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP0, H, L, 0, seed, 0u, itp);
+            hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 1, prm->min_gl, ploidy_d, HL);
+            fb_groups(HP1);
+            fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
+            hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, 0u, itp);
+            phase_prep_launch(T, ph_d, items_d, poly, H, H, flat_d, P->d_flags.as<uint8_t>(), loci, s);
+            ph_place_launch(ctl, ph_d, items_d, T, FEW_ITEMS, G->latency, caps, pord, s);
+            ph_groups();
+            hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, ploidy_d, gp0, gp1, sto, hds, itp,
+                               (uint32_t)(1 + prm->burnin));
+            graph_advance_launch(ctl, s);
+            GH_HIP(hipGetLastError());
+        };
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        GH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            iteration();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        GH_HIP(hipStreamEndCapture(s, &graph));
+        struct GraphHold {
+            hipGraph_t &g;
+            hipGraphExec_t &e;
+            ~GraphHold() {
+                if (e) (void)hipGraphExecDestroy(e);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        } hold{graph, exec};
+        GH_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (int it = n_host; it < n_iter; it++) {
+            GH_HIP(hipGraphLaunch(exec, s));
+            if (it > prm->burnin) main_done++;
+        }
+        GraphCtl cend;
+        std::vector<int32_t> nst_h((size_t)T * n_iter);
+        GH_HIP(hipMemcpyAsync(&cend, ctl, sizeof(cend), hipMemcpyDeviceToHost, s));
+        GH_HIP(hipMemcpyAsync(nst_h.data(), nst, sizeof(int32_t) * nst_h.size(), hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
+        if (cend.overflow) return false;
+        int32_t sflags[2] = {0, 0};
+        GH_HIP(hipMemcpy(sflags, status, sizeof(sflags), hipMemcpyDeviceToHost));
+        GH_CHECK(sflags[0] == 0, GH_ERR_NUMERIC, "phasing HMM underflow (TRANS_HAP / SAMPLE_DIP)");
+        GH_CHECK(sflags[1] == 0, GH_ERR_CAPACITY, "PBWT selection record capacity exceeded");
+        for (int t = 0; t < T; t++)
+            for (int it = n_host; it < n_iter; it++)
+                G->nstates[(size_t)t * n_iter + it] = nst_h[(size_t)t * n_iter + it];
+        st.fb_state_sites += (double)cend.stats[GS_FB_STATE_SITES];
+        st.fb_bytes += (double)cend.stats[GS_FB_BYTES];
+        st.fb_poly_sites += (double)cend.stats[GS_FB_POLY_SITES];
+        st.ph_state_sites += (double)cend.stats[GS_PH_STATE_SITES];
+        st.ph_segments += (double)cend.stats[GS_PH_SEGMENTS];
+        st.pbwt_hap_sites += (double)(n_iter - n_host) * 2 * npt * L;
+        st.graph_iters = n_iter - n_host;
+        st.ms_graph = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
+    }
     if ((learned || collecting) && npt > 0) {
         int32_t cnt[2] = {0, 0};
         GH_HIP(hipMemcpyAsync(cnt, G->d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
@@ -1079,9 +1278,26 @@ void run_execute(gh_run *G) {
     GH_HIP(hipGetLastError());
     GH_HIP(hipStreamSynchronize(s));
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
-    st.ms_other = st.ms_total - st.ms_select - st.ms_condset - st.ms_fb - st.ms_phase;
+    st.ms_other = st.ms_total - st.ms_select - st.ms_condset - st.ms_fb - st.ms_phase - st.ms_graph;
     G->st = st;
     G->done = true;
+    return true;
+}
+
+void run_execute(gh_run *G) {
+    GH_CHECK(G, GH_ERR_ARG, "gh_run_execute: null run");
+    const bool graph = G->graph_mode == 1 && !G->tune && G->collect_from < 1 && !G->any_haploid && G->n_iter > 2;
+    if (run_execute_once(G, graph)) return;
+    // a replayed iteration outgrew its captured buffers: execute again with
+    // every iteration launched from the host (the needs it records size the
+    // next capture)
+    run_execute_once(G, false);
+    G->st.graph_reruns = 1;
+}
+
+void run_set_graph(gh_run *G, int mode) {
+    GH_CHECK(G && (mode == 0 || mode == 1), GH_ERR_ARG, "gh_run_set_graph: mode must be 0 or 1");
+    G->graph_mode = mode;
 }
 
 void run_collect(gh_run *G, int from_iter) {

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <type_traits>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 namespace gh {
@@ -49,6 +50,7 @@ struct FbArgs {
     float *mass_all;   // optional: per-state posterior mass (summed over sites)
     const int32_t *win_lo, *win_hi;   // optional with mass_all: per-state site window [lo, hi) of the sum
     const int32_t *order;             // optional: workgroup b runs item order[b]
+    GraphGate gate;                   // optional: launch gated by the device (replayed iterations)
     float ee, ed;
     int C;             // sites per checkpoint block (divides FB_TILE)
     int maxW;          // LDS row stride bound
@@ -60,6 +62,7 @@ template <int NW, int MWL>
 // possible; the 16-word shapes (tuner candidates only) keep their registers
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 8 ? 3 : 1))) void fb_kernel(FbArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
+    if (A.gate.key && (*A.gate.key != A.gate.want || (int)blockIdx.x >= *A.gate.n)) return;
     const FbItem f = wave_uniform(A.fb[A.order ? A.order[blockIdx.x] : (int)blockIdx.x]);
     const CondItem c = wave_uniform(A.cond[f.cond]);
     const int K = c.K, W = c.W, Lp = c.Lp, C = A.C;
@@ -480,21 +483,31 @@ __global__ void fb_mono_kernel(const FbItem *__restrict__ fb, const CondItem *__
 
 template <int NW, int MWL>
 static void fb_go(const FbArgs &A, int n, size_t smem, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
-        GH_HIP(hipFuncSetAttribute((const void *)fb_kernel<NW, MWL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   FB_LDS_BYTES));
-        attr = true;
-    }
     hipLaunchKernelGGL((fb_kernel<NW, MWL>), dim3(n), dim3(64 * NW), smem, s, A);
+}
+
+template <int NW, int MWL>
+static void fb_attr() {
+    GH_HIP(hipFuncSetAttribute((const void *)fb_kernel<NW, MWL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               FB_LDS_BYTES));
+}
+
+// The LDS limit of every shape is raised once, before any launch (a launch
+// captured into a graph must not be the first use of its kernel).
+static void fb_set_attributes() {
+    fb_attr<1, 4>(); fb_attr<2, 2>(); fb_attr<4, 1>(); fb_attr<1, 8>(); fb_attr<2, 4>(); fb_attr<4, 2>();
+    fb_attr<8, 1>(); fb_attr<1, 16>(); fb_attr<2, 8>(); fb_attr<4, 4>(); fb_attr<8, 2>(); fb_attr<16, 1>();
+    fb_attr<2, 16>(); fb_attr<4, 8>(); fb_attr<8, 4>(); fb_attr<16, 2>(); fb_attr<8, 8>(); fb_attr<16, 4>();
 }
 
 void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, int maxW, int ckpt_maxW,
                const int32_t *d_poly,
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass, const int32_t *d_win_lo,
-               const int32_t *d_win_hi, FbPick pick, const int32_t *d_order) {
+               const int32_t *d_win_hi, FbPick pick, const int32_t *d_order, GraphGate gate) {
     if (n <= 0) return;
+    static std::once_flag attr_once;
+    std::call_once(attr_once, fb_set_attributes);
     GH_CHECK(maxW <= FB_MAX_WORDS, GH_ERR_ARG, "fb: more than 4096 conditioning states per haplotype is not supported");
     const FbShape sh = fb_shape(maxW, pick);
     // the launch writes a checkpoint every sh.C sites into buffers sized for
@@ -505,7 +518,7 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
     FbArgs A;
     A.fb = d_fb; A.cond = d_cond; A.poly_all = d_poly; A.t_all = d_t; A.hvar_all = d_hvar; A.HL_all = d_HL;
     A.flat_all = d_flat; A.flags = P->d_flags.as<uint8_t>(); A.ckpt_all = d_ckpt; A.HP_all = d_HP;
-    A.mass_all = d_mass; A.win_lo = d_win_lo; A.win_hi = d_win_hi; A.order = d_order;
+    A.mass_all = d_mass; A.win_lo = d_win_lo; A.win_hi = d_win_hi; A.order = d_order; A.gate = gate;
     A.ee = ee; A.ed = ed; A.C = sh.C; A.maxW = maxW;
 #define GH_FB(nw, mwl)                    \
     if (sh.NW == nw && sh.MWL == mwl) {   \

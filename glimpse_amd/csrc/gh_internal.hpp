@@ -81,6 +81,17 @@ struct gh_panel {
 
 namespace gh {
 
+// A launch replayed from a captured graph (gh_graph.hpp) is gated by the
+// device: it returns at once unless *key == want, workgroups at or beyond *n
+// return, and the iteration that keys its draws is read from *iter.  A
+// default gate (key == nullptr) gates nothing.
+struct GraphGate {
+    const int32_t *key = nullptr;
+    int32_t want = 0;
+    const int32_t *n = nullptr;
+    const uint32_t *iter = nullptr;
+};
+
 // Per-item description of a conditioning set living on the device.
 struct CondItem {
     int32_t K;          // states
@@ -124,7 +135,7 @@ void fb_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, i
                const float *d_t, const uint64_t *d_hvar, const float *d_HL, const uint8_t *d_flat, float *d_ckpt,
                float *d_HP, float ee, float ed, hipStream_t s, float *d_mass = nullptr,
                const int32_t *d_win_lo = nullptr, const int32_t *d_win_hi = nullptr, FbPick pick = FbPick{0, 0},
-               const int32_t *d_order = nullptr);
+               const int32_t *d_order = nullptr, GraphGate gate = GraphGate{});
 // HP at monomorphic sites (closed form) for the same items.
 void fb_mono_launch(gh_panel *P, int n, const FbItem *d_fb, const CondItem *d_cond, const int32_t *d_pos,
                     const float *d_HL, const uint8_t *d_flat, float *d_HP, float ee, float ed, hipStream_t s);

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(KPT == 
                                                        uint8_t *__restrict__ H0_all, uint8_t *__restrict__ H1_all,
                                                        int L, float ee, float ed, uint32_t seed, uint32_t iter,
                                                        int32_t *__restrict__ status,
-                                                       const int32_t *__restrict__ order) {
+                                                       const int32_t *__restrict__ order, GraphGate gate) {
     constexpr int PH_NT = 64 * NW;
     extern __shared__ __align__(16) uint64_t s_rows[];   // [PH_TILE][W] Hvar rows of the current tile
     __shared__ float lds8[2][NW * 8];
@@ -174,6 +174,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(KPT == 
     __shared__ signed char s_type[PH_TILE];
     __shared__ unsigned char s_alt[PH_TILE], s_first[PH_TILE], s_last[PH_TILE];
     __shared__ float s_yin[PH_TILE], s_yout[PH_TILE];
+    if (gate.key) {
+        if (*gate.key != gate.want || (int)blockIdx.x >= *gate.n) return;
+        iter = *gate.iter;
+    }
     const PhItem it = wave_uniform(items[order ? order[blockIdx.x] : (int)blockIdx.x]);   // optional item order
     const CondItem c = wave_uniform(cond[it.cond]);
     const int K = c.K, W = c.W, tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -605,11 +609,11 @@ template <int NW, int KPT>
 static void phase_go(gh_panel *P, int n, const PhItem *d_items, const CondItem *d_cond, const uint64_t *d_hvar,
                      const Locus *d_loci, const int32_t *d_pos, double nrho, PhaseBufs &B, uint8_t *d_H0,
                      uint8_t *d_H1, float ee, float ed, uint32_t seed, uint32_t iter, int32_t *d_status,
-                     hipStream_t s, const int32_t *d_order) {
+                     hipStream_t s, const int32_t *d_order, GraphGate gate) {
     const size_t smem = (size_t)PH_TILE * ((KPT * 64 * NW + 63) / 64) * 8;
     hipLaunchKernelGGL((phase_kernel<NW, KPT>), dim3(n), dim3(64 * NW), smem, s, d_items, d_cond, d_hvar, d_loci,
                        d_pos, P->d_cm.as<double>(), nrho, B.seg, B.segsum, B.imp, B.impsum, B.of1, B.dip, d_H0, d_H1,
-                       P->L, ee, ed, seed, iter, d_status, d_order);
+                       P->L, ee, ed, seed, iter, d_status, d_order, gate);
 }
 
 // Launch shapes (waves per target, states per thread) the kernel is built for,
@@ -654,7 +658,7 @@ PhaseShape phase_shape(int maxK, bool latency) {
 void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
                   const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
                   uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
-                  hipStream_t s, PhaseShape sh, const int32_t *d_order) {
+                  hipStream_t s, PhaseShape sh, const int32_t *d_order, GraphGate gate) {
     if (n <= 0) return;
     const double nrho = (-0.04 * ne) / P->N;
     const float ed = err_phase, ee = (float)(1.0 - (double)err_phase);
@@ -663,7 +667,7 @@ void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const Con
 #define GH_PH(nw, kpt)                                                                                          \
     if (sh.NW == nw && sh.KPT == kpt) {                                                                         \
         phase_go<nw, kpt>(P, n, d_items, d_cond, d_hvar, d_loci, d_pos, nrho, B, d_H0, d_H1, ee, ed, seed, iter, \
-                          d_status, s, d_order);                                                                \
+                          d_status, s, d_order, gate);                                                          \
         GH_HIP(hipGetLastError());                                                                              \
         return;                                                                                                 \
     }

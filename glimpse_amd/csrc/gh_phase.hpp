@@ -58,7 +58,8 @@ PhaseShape phase_shape(int maxK, bool latency = false);
 void phase_launch(gh_panel *P, int n, int maxK, const PhItem *d_items, const CondItem *d_cond,
                   const uint64_t *d_hvar, const Locus *d_loci, const int32_t *d_pos, int ne, PhaseBufs &B,
                   uint8_t *d_H0, uint8_t *d_H1, float err_phase, uint32_t seed, uint32_t iter, int32_t *d_status,
-                  hipStream_t s, PhaseShape sh = PhaseShape{0, 0}, const int32_t *d_order = nullptr);
+                  hipStream_t s, PhaseShape sh = PhaseShape{0, 0}, const int32_t *d_order = nullptr,
+                  GraphGate gate = GraphGate{});
 // Padded state count of a shape.
 int phase_kpt_states(PhaseShape sh);
 // Floats per stored message of a K-state target (8 per state, K rounded up to 64).

@@ -41,7 +41,7 @@ class RunStats(ctypes.Structure):
         "fb_state_sites", "fb_poly_sites", "ph_state_sites", "ph_segments", "pbwt_hap_sites",
         "ms_select", "ms_condset", "ms_fb", "ms_phase", "ms_other", "ms_total",
         "fb_kernel_ms", "fb_launches", "fb_bytes", "ms_learn", "learned_sel", "pool_kept", "collected_rows",
-        "fb_nw", "fb_mwl", "ph_nw", "ph_kpt")]
+        "fb_nw", "fb_mwl", "ph_nw", "ph_kpt", "graph_iters", "graph_reruns", "ms_graph")]
 
 MLP_IN, MLP_HIDDEN, MLP_NPARAM = 32, 128, 20865
 
@@ -84,6 +84,7 @@ def lib():
         L.gh_mlp_train_run.argtypes = [vp, vp, i32, i32, f32, u32, vp]
         L.gh_run_collect.argtypes = [vp, i32]
         L.gh_run_set_tuning.argtypes = [vp, i32]
+        L.gh_run_set_graph.argtypes = [vp, i32]
         L.gh_run_collected.argtypes = [vp, vp, vp, vp, vp]
         L.gh_run_set_selector.argtypes = [vp, vp, vp, f32, i32]
         L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
@@ -384,6 +385,11 @@ class Run:
         defaults for the run on a batch's critical path."""
         m = 2 if mode == "latency" else int(mode)
         _check(lib().gh_run_set_tuning(self.h, m))
+
+    def set_graph(self, mode=1):
+        """gh_run_set_graph: 1 (default) = iterations from 2 on replay a
+        captured HIP graph; 0 = every iteration launched from the host."""
+        _check(lib().gh_run_set_graph(self.h, int(mode)))
 
     def set_workspace(self, ws):
         """Execute in a shared Workspace (None: the run's own); runs sharing one

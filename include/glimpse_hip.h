@@ -89,6 +89,9 @@ typedef struct {
     double collected_rows;     /* training rows recorded by gh_run_collect */
     double fb_nw, fb_mwl;      /* forward-backward launch shape of the last iteration (waves, words per lane) */
     double ph_nw, ph_kpt;      /* phasing launch shape of the last iteration (waves, states per thread) */
+    double graph_iters;        /* iterations replayed from the captured graph (gh_run_set_graph) */
+    double graph_reruns;       /* 1 when the execution was repeated without the graph (captured buffers outgrown) */
+    double ms_graph;           /* wall time of the capture and the replays (not in ms_select .. ms_phase) */
 } gh_run_stats;
 
 const char *gh_last_error(void);
@@ -210,6 +213,22 @@ int gh_run_set_workspace(gh_run *r, gh_workspace *w);
  * is bit-reproducible with modes 0 and 2 (or with GH_FB_SHAPE=<NW>x<MWL> and
  * GH_PHASE_SHAPE=<NW>x<KPT> pinning the shapes in mode 0). */
 int gh_run_set_tuning(gh_run *r, int32_t mode);
+
+/* How the iterations are launched (no reference counterpart: the schedule of
+ * caller::phase_loop, caller_algorithm.cpp:223-243, on the device).  mode 1
+ * (the default): iterations 0 (INIT) and 1 are launched from the host; the
+ * next one is captured once as a HIP graph in which every per-iteration
+ * decision (stored PBWT sites and start indices, the placement of Hvar rows,
+ * checkpoints, loci and phasing messages by each target's counts, the launch
+ * groups and shapes) is made by device kernels, and the graph is replayed for
+ * every later iteration with no host round trip.  The placed buffers are sized
+ * before the capture from the host-launched iterations with headroom; if a
+ * replay needs more, the execution is repeated with mode 0 (same results;
+ * gh_run_stats.graph_reruns).  mode 0: every iteration launched from the host
+ * (per-stage timings in gh_run_stats).  Runs that tune shapes
+ * (gh_run_set_tuning 1), record training rows or have haploid targets launch
+ * every iteration from the host.  Results are identical in both modes. */
+int gh_run_set_graph(gh_run *r, int32_t mode);
 
 /* Per-target ploidy (1 or 2; default 2), before the first execution: a haploid
  * target (genotype with ploidy 1, caller_algorithm.cpp:53-84) takes one
