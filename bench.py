@@ -18,6 +18,9 @@ its units one after another in its own device workspace, so 6 runs (1,500
 Gibbs chains) are in flight at a time and only their workspaces are held
 (4 in flight: 13.4M/s before the last forward-backward change; 8 units of
 125 targets: slower, smaller launches).
+Within a unit, iterations 0 (INIT) and 1 are launched from the host and
+iteration 2 is captured as a HIP graph that makes every per-iteration decision
+on the device; the graph is replayed for iterations 3-20 (gh_run_set_graph).
 Before the timed step each worker executes its largest unit once on both
 paths (outputs discarded) so the workspaces are allocated; --warmup adds full
 untimed steps.
@@ -42,8 +45,9 @@ plain run are in the same line.
 roofline: the dominant kernel (haploid forward-backward) against HBM,
 algorithmic bytes per dispatch / average dispatch duration (HIP events on the
 launch stream), measured on one unit running alone (the largest chunk's first
-block, plain path: 500 haplotypes per dispatch fill the CUs), so no other
-dispatch shares its time; `step_level_frac` is the whole step's algorithmic
+block, plain path: 500 haplotypes per dispatch fill the CUs; every iteration
+launched from the host so each dispatch is timed), so no other dispatch shares
+its time; `step_level_frac` is the whole step's algorithmic
 forward-backward bytes over the step time; `traffic` is the measured HBM bytes
 per dispatch from the committed rocprofv3 counter summary of the same probe.
 cpu_baseline: the CPU oracle (a C++ restatement of the reference path) timed
@@ -359,6 +363,9 @@ def main():
                     help="only the roofline probe: the largest unit alone on the plain path (for rocprofv3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB three times (HBM counter calibration)")
+    ap.add_argument("--graph", type=int, choices=(0, 1, 2), default=1,
+                    help="1: iterations from 2 on replay a captured HIP graph; 0: every iteration launched from the "
+                         "host (gh_run_set_graph)")
     ap.add_argument("--seed", type=int, default=15052011)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -389,6 +396,8 @@ def main():
     meta["block"] = block
     units = [Unit(c, ci, lo, min(lo + block, c["gl"].shape[0]), panels[ci])
              for ci, c in enumerate(chunks) for lo in range(0, c["gl"].shape[0], block)]
+    for u in units:
+        u.run.set_graph(a.graph)
     n_workers = a.workers or (6 if a.workload == "c5" else len(chunks))
     workspaces = [hip.Workspace() for _ in range(n_workers)]
     tsites = sum(c["gl"].shape[0] * c["L"] for c in chunks)
@@ -398,6 +407,7 @@ def main():
     if a.probe:
         u = max(units, key=lambda u: u.work)
         set_path([u], "plain", None, 0, 0, chunks)
+        u.run.set_graph(0)
         u.run.set_workspace(workspaces[0])
         u.run.execute()
         st = u.run.stats()
@@ -437,7 +447,10 @@ def main():
                       mean_states=float(np.mean(np.concatenate([o["nstates"][:, 1:].ravel() for o in outs]))),
                       stage_ms_summed=({k: round(sum(s[k] for s in stats), 1)
                                         for k in ("ms_select", "ms_learn", "ms_condset", "ms_fb", "ms_phase",
-                                                  "ms_other")}),
+                                                  "ms_graph", "ms_other")}),
+                      graph=dict(replayed_iterations=int(sum(s["graph_iters"] for s in stats)),
+                                 iterations=int(sum(o["nstates"].shape[1] for o in outs for _ in o["stats"])),
+                                 reruns=int(sum(s["graph_reruns"] for s in stats))),
                       ds=np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in outs]))
         if p == "learned":
             res[p]["target_iterations_learned"] = sum(s["learned_sel"] for s in stats)
@@ -450,6 +463,7 @@ def main():
     # roofline probe: the largest unit alone on the plain path
     u = max(units, key=lambda u: u.work)
     set_path([u], "plain", None, 0, 0, chunks)
+    u.run.set_graph(0)
     u.run.set_workspace(workspaces[0])
     u.run.execute()
     probe = fb_roofline([u.run.stats()])
@@ -490,7 +504,7 @@ def main():
     for p in paths:
         R = res[p]
         d = {k: R[k] for k in ("value", "ms_per_step", "accuracy", "mean_states", "step_level_frac",
-                               "stage_ms_summed")}
+                               "stage_ms_summed", "graph")}
         if p == "learned":
             d.update(keep=a.keep, min_mass=a.min_mass, training=train,
                      target_iterations_learned=R["target_iterations_learned"],

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -341,6 +342,26 @@ struct EvPair {
 struct gh_workspace {
     gh::DBuf H, HL, HP0, HP1, states, poly, pos, t, mark, hvar, alpha, loci, seg, segsum, imp, impsum, of1, dip, ev,
         evci, ins, isite, fx, fy, mass0, mass1, wlo, whi;
+    // The executions' streams (the second for K-class batching) belong to the
+    // workspace, not the run: a batch of many resident runs has as many
+    // streams in use as workspaces, so runs executing together get distinct
+    // hardware queues (streams beyond GPU_MAX_HW_QUEUES share queues, and a
+    // stream holding a whole run's replayed iterations would block the other).
+    hipStream_t s = nullptr, s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void streams() {
+        if (s) return;
+        GH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        GH_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        GH_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        GH_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
+    ~gh_workspace() {
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (s2) (void)hipStreamDestroy(s2);
+        if (s) (void)hipStreamDestroy(s);
+    }
 };
 
 // One prepared Gibbs run: inputs, outputs and tables resident on the device;
@@ -369,7 +390,7 @@ struct gh_run {
     std::vector<int32_t> dord;   // diploid targets in launch order (H1 forward-backward, phasing)
     gh_workspace own;
     gh_workspace *W = &own;
-    hipStream_t stream = nullptr;   // the run's own stream (runs on one panel may overlap)
+    hipStream_t stream = nullptr;   // the run's own stream: uploads, fetches (executions use the workspace's)
     gh::PbwtRun R;
     gh_run_stats st;
     bool done = false;
@@ -377,10 +398,8 @@ struct gh_run {
     bool tune = false;
     bool latency = false;   // latency default shapes (a run on the critical path)
     // K-class batching: the items of the largest set-size class run in one
-    // launch on the run's stream, the others in a launch sized for their own
-    // class on a second stream beside it
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // launch on the execution stream, the others in a launch sized for their
+    // own class on the workspace's second stream beside it
     gh::DBuf d_ford, d_pord;
     std::vector<int32_t> ford, pord;
     gh::ShapeTuner<gh::FbPick> fb_tuner;
@@ -395,11 +414,10 @@ struct gh_run {
     // replayed iterations (gh_graph.hpp)
     int graph_mode = 1;        // 1: iterations from 2 on replay a captured graph; 0: all launched from the host
     gh::GraphCaps need{};      // largest placement needs of the iterations launched from the host
+    int gk_fb[2] = {-1, -1}, gk_ph[2] = {-1, -1};   // launch keys of the last host-launched iteration
+    int graph_fail = 0;        // GO_* bits of the last execution whose replays placed nothing
     gh::DBuf d_ctl, d_nst, d_gptr, d_gend;
     ~gh_run() {
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (s2) (void)hipStreamDestroy(s2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -565,6 +583,23 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         G->d_pst.reserve(G->stored_h.size());
         G->d_pstart.reserve(sizeof(int32_t) * G->start_h.size());
         G->nstates.assign((size_t)T * G->n_iter, 0);
+        // per-iteration tables, allocated once here (an execution frees nothing:
+        // a free waits for the whole device)
+        G->d_ph.reserve(sizeof(PhItem) * T);
+        G->d_ford.reserve(sizeof(int32_t) * 2 * (size_t)T);
+        G->d_pord.reserve(sizeof(int32_t) * 2 * (size_t)T);
+        G->d_dord.reserve(sizeof(int32_t) * (size_t)T);
+        G->d_ctl.reserve(sizeof(GraphCtl));
+        G->d_nst.reserve(sizeof(int32_t) * (size_t)T * G->n_iter);
+        {
+            std::vector<int32_t> gptr(1, 0), gend;
+            for (auto &g : G->cls_grp) {
+                gend.insert(gend.end(), g.begin(), g.end());
+                gptr.push_back((int32_t)gend.size());
+            }
+            dalloc_copy(G->d_gptr, gptr.data(), gptr.size(), s);
+            dalloc_copy(G->d_gend, gend.data(), gend.size(), s);
+        }
         GH_HIP(hipStreamSynchronize(s));
         *out = G;
     } catch (...) {
@@ -581,11 +616,12 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
 static bool run_execute_once(gh_run *G, bool use_graph) {
     const auto wall0 = std::chrono::steady_clock::now();
     gh_panel *P = G->P;
-    hipStream_t s = G->stream;
     const int T = G->T, L = G->L, N = G->N, n_iter = G->n_iter, npt = G->npt;
     const gh_params *prm = &G->prm;
     const uint32_t seed = prm->seed;
     gh_workspace *Wk = G->W;
+    Wk->streams();
+    hipStream_t s = Wk->s;
     const int kst = G->kstride, prs = G->pstride;
     const bool learned = G->mlp != nullptr, collecting = G->collect_from >= 1;
     GH_CHECK(!(learned && collecting), GH_ERR_ARG, "a run either records training rows or uses the learned selector");
@@ -884,6 +920,9 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         // a launch of a few items cannot fill the GPU: its chains are the
         // critical path, so it takes the latency shapes
         const bool fewA = fbA <= FEW_ITEMS, fewB = fbB <= FEW_ITEMS;
+        // the launch keys of this iteration's groups: the only ones a graph captured after it holds
+        G->gk_fb[0] = graph_key(fb_cap_words(maxW_it), G->latency || fewA);
+        G->gk_fb[1] = fbB > 0 ? graph_key(fb_cap_words(fbWB), G->latency || fewB) : -1;
         const FbPick fbp = (G->tune && stage > 0) ? G->fb_tuner.pick(wcls, fb_shapes(maxW_it), FbPick{0, 0})
                            : (G->latency || fewA) ? fb_default_pick(maxW_it, true)
                                                   : FbPick{0, 0};
@@ -894,12 +933,6 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         GH_HIP(hipEventRecord(e_fb.a, s));
         // one forward-backward pass over every item: the top class on s, the rest
         // beside it on s2 (sized for its own class), joined before s goes on
-        auto ensure_s2 = [&]() {
-            if (G->s2) return;
-            GH_HIP(hipStreamCreateWithFlags(&G->s2, hipStreamNonBlocking));
-            GH_HIP(hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming));
-            GH_HIP(hipEventCreateWithFlags(&G->ev_join, hipEventDisableTiming));
-        };
         // (each kernel dispatch is timed by its own event pair)
         auto fb_all = [&](float *HPx, float *massx, EvPair &ea, EvPair &eb) {
             if (fbB == 0) {
@@ -909,20 +942,19 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                 GH_HIP(hipEventRecord(ea.b, s));
                 return;
             }
-            ensure_s2();
             const int32_t *ord = G->d_ford.as<int32_t>();
-            GH_HIP(hipEventRecord(G->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
+            GH_HIP(hipEventRecord(Wk->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
             GH_HIP(hipEventRecord(ea.a, s));
             fb_launch(P, fbA, fb_d, items_d, fbWA, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
                       massx, wlo, whi, fbp, ord);
             GH_HIP(hipEventRecord(ea.b, s));
-            GH_HIP(hipEventRecord(eb.a, G->s2));
-            fb_launch(P, fbB, fb_d, items_d, fbWB, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, G->s2,
+            GH_HIP(hipEventRecord(eb.a, Wk->s2));
+            fb_launch(P, fbB, fb_d, items_d, fbWB, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, Wk->s2,
                       massx, wlo, whi, (G->latency || fewB) ? fb_default_pick(fbWB, true) : FbPick{0, 0}, ord + fbA);
-            GH_HIP(hipEventRecord(eb.b, G->s2));
-            GH_HIP(hipEventRecord(G->ev_join, G->s2));
-            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+            GH_HIP(hipEventRecord(eb.b, Wk->s2));
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         };
         // H0 (caller_algorithm.cpp:58-65)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl,
@@ -1006,6 +1038,8 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
             phB = n_dip - phA;
             if (!G->pord.empty()) dalloc_copy(G->d_pord, G->pord.data(), G->pord.size(), s);
         }
+        G->gk_ph[0] = graph_key(ph_cap_states(maxK), G->latency || phA <= FEW_ITEMS);
+        G->gk_ph[1] = phB > 0 ? graph_key(ph_cap_states(phKB), G->latency || phB <= FEW_ITEMS) : -1;
         const int kcls = maxK <= 256 ? 0 : maxK <= 512 ? 1 : maxK <= 1024 ? 2 : maxK <= 2048 ? 3 : 4;
         const PhaseShape phs = (G->tune && stage > 0)
                                    ? G->ph_tuner.pick(kcls, phase_shapes(maxK), phase_shape(maxK, G->latency))
@@ -1054,17 +1088,16 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                 phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
                              (uint32_t)it, status, s, phs, G->d_pord.as<int32_t>());
         } else {
-            ensure_s2();
             const int32_t *ord = G->d_pord.as<int32_t>();
-            GH_HIP(hipEventRecord(G->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
+            GH_HIP(hipEventRecord(Wk->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
             if (phA > 0)
                 phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
                              (uint32_t)it, status, s, phs, ord);
             phase_launch(P, phB, phKB, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
-                         (uint32_t)it, status, G->s2, phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
-            GH_HIP(hipEventRecord(G->ev_join, G->s2));
-            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+                         (uint32_t)it, status, Wk->s2, phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         }
         GH_HIP(hipEventRecord(e_pk.b, s));
         GH_HIP(hipEventRecord(e_ph.b, s));
@@ -1097,16 +1130,17 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
     }
     if (n_host < n_iter) {
         const auto g0 = std::chrono::steady_clock::now();
-        // buffers placed per iteration: what the host-launched iterations
-        // needed, with headroom (GH_GRAPH_HEADROOM, default 1.5; the whole
-        // buffer when it is larger).  A headroom below 1 caps the replays at
-        // that fraction of the need, which forces the repeated execution.
+        // Buffers placed per iteration: the replays use what the host-launched
+        // iterations reserved (at least a quarter above their largest need,
+        // often more in a shared workspace).  Nothing is reallocated here: a
+        // free waits for the whole device, every run in flight included.
+        // GH_GRAPH_HEADROOM=f < 1 (testing) caps the replays at f x the need,
+        // which forces the repeated execution.
         const char *eh = std::getenv("GH_GRAPH_HEADROOM");
-        const double head = eh ? std::max(0.01, std::atof(eh)) : 1.5;
+        const double head = eh ? std::max(0.01, std::atof(eh)) : 1.0;
         auto cap_of = [&](DBuf &b, int64_t need, size_t elem) -> int64_t {
-            const int64_t n = (int64_t)std::max(1.0, std::ceil((double)need * head));
-            if ((size_t)n * elem > b.bytes) b.reserve((size_t)n * elem);
-            return head >= 1.0 ? (int64_t)(b.bytes / elem) : n;
+            const int64_t have = (int64_t)(b.bytes / elem);
+            return head >= 1.0 ? have : std::min(have, (int64_t)std::max(1.0, std::ceil((double)need * head)));
         };
         GraphCaps caps;
         caps.hvar_words = cap_of(Wk->hvar, G->need.hvar_words, sizeof(uint64_t));
@@ -1118,22 +1152,17 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         caps.impsum = std::min(cap_of(Wk->impsum, G->need.impsum, sizeof(float)),
                                cap_of(Wk->of1, G->need.impsum, sizeof(float)));
         caps.dip = cap_of(Wk->dip, G->need.dip, sizeof(int32_t));
-        GH_HIP(hipStreamSynchronize(s));   // the buffers reallocated above are no longer in use
         // device tables of the replays
         GraphCtl c0;
         std::memset(&c0, 0, sizeof(c0));
         c0.iter = (uint32_t)n_host;
-        GraphCtl *ctl = dalloc_copy(G->d_ctl, &c0, 1, s);
-        int32_t *nst = (int32_t *)G->d_nst.reserve(sizeof(int32_t) * (size_t)T * n_iter);
-        int32_t *ford = (int32_t *)G->d_ford.reserve(sizeof(int32_t) * 2 * (size_t)T);
-        int32_t *pord = (int32_t *)G->d_pord.reserve(sizeof(int32_t) * 2 * (size_t)T);
-        std::vector<int32_t> gptr(1, 0), gend;
-        for (auto &g : G->cls_grp) {
-            gend.insert(gend.end(), g.begin(), g.end());
-            gptr.push_back((int32_t)gend.size());
-        }
-        const int32_t *d_gptr = dalloc_copy(G->d_gptr, gptr.data(), gptr.size(), s);
-        const int32_t *d_gend = dalloc_copy(G->d_gend, gend.data(), gend.size(), s);
+        // (per-run tables, allocated by run_create)
+        GraphCtl *ctl = G->d_ctl.as<GraphCtl>();
+        GH_HIP(hipMemcpyAsync(ctl, &c0, sizeof(c0), hipMemcpyHostToDevice, s));
+        int32_t *nst = G->d_nst.as<int32_t>();
+        int32_t *ford = G->d_ford.as<int32_t>();
+        int32_t *pord = G->d_pord.as<int32_t>();
+        const int32_t *d_gptr = G->d_gptr.as<int32_t>(), *d_gend = G->d_gend.as<int32_t>();
         PhItem *ph_d = G->d_ph.as<PhItem>();
         uint64_t *hvar = Wk->hvar.as<uint64_t>();
         float *alpha = Wk->alpha.as<float>();
@@ -1141,52 +1170,53 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         PhaseBufs PB;
         PB.seg = Wk->seg.as<float>(); PB.segsum = Wk->segsum.as<float>(); PB.imp = Wk->imp.as<float>();
         PB.impsum = Wk->impsum.as<float>(); PB.of1 = Wk->of1.as<float>(); PB.dip = Wk->dip.as<int32_t>();
-        if (!G->s2) {
-            GH_HIP(hipStreamCreateWithFlags(&G->s2, hipStreamNonBlocking));
-            GH_HIP(hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming));
-            GH_HIP(hipEventCreateWithFlags(&G->ev_join, hipEventDisableTiming));
-        }
         GH_HIP(hipStreamSynchronize(s));   // host tables above are copied
         const uint32_t *itp = &ctl->iter;
-        const int fb_capmax = fb_cap_words(G->maxW), ph_capmax = ph_cap_states(G->kstride);
-        // every launch variant a group can take (capacity class x latency
-        // shape), each gated by the group's key of the iteration
+        // the launch groups of the last host-launched iteration (keys checked by the placement kernels)
+        // GH_GRAPH_FLIP_LATENCY=1 (testing): capture group A of both passes with
+        // the other latency shape, so every replay's keys differ (the repeat path)
+        if (const char *fl = std::getenv("GH_GRAPH_FLIP_LATENCY"))
+            if (std::atoi(fl) == 1) {
+                G->gk_fb[0] ^= 1;
+                G->gk_ph[0] ^= 1;
+            }
+        auto fb_key_launch = [&](int grp, hipStream_t st, float *HPx) {
+            const int key = G->gk_fb[grp], cap = key / 2;
+            const int mw = std::min(cap, G->maxW);
+            const FbPick pick = (key & 1) ? fb_default_pick(mw, true) : FbPick{0, 0};
+            GraphGate g{&ctl->fb_key[grp], key, &ctl->fb_n[grp], itp};
+            fb_launch(P, T, fb_d, items_d, mw, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, st,
+                      nullptr, nullptr, nullptr, pick, ford + (size_t)grp * T, g);
+        };
+        auto ph_key_launch = [&](int grp, hipStream_t st) {
+            const int key = G->gk_ph[grp], cap = key / 2;
+            GraphGate g{&ctl->ph_key[grp], key, &ctl->ph_n[grp], itp};
+            phase_launch(P, T, cap, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed, 0u,
+                         status, st, phase_shape(cap, key & 1), pord + (size_t)grp * T, g);
+        };
         auto fb_groups = [&](float *HPx) {
-            GH_HIP(hipEventRecord(G->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
-            for (int cap = 4; cap <= fb_capmax; cap *= 2)
-                for (int lat = G->latency ? 1 : 0; lat <= 1; lat++) {
-                    const int mw = std::min(cap, G->maxW);
-                    const FbPick pick = lat ? fb_default_pick(mw, true) : FbPick{0, 0};
-                    GraphGate ga{&ctl->fb_key[0], graph_key(cap, lat), &ctl->fb_n[0], itp};
-                    fb_launch(P, T, fb_d, items_d, mw, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i,
-                              G->ed_i, s, nullptr, nullptr, nullptr, pick, ford, ga);
-                    if (cap < fb_capmax) {
-                        GraphGate gb{&ctl->fb_key[1], graph_key(cap, lat), &ctl->fb_n[1], itp};
-                        fb_launch(P, T, fb_d, items_d, mw, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i,
-                                  G->ed_i, G->s2, nullptr, nullptr, nullptr, pick, ford + T, gb);
-                    }
-                }
-            GH_HIP(hipEventRecord(G->ev_join, G->s2));
-            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+            if (G->gk_fb[1] < 0) {
+                fb_key_launch(0, s, HPx);
+                return;
+            }
+            GH_HIP(hipEventRecord(Wk->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
+            fb_key_launch(0, s, HPx);
+            fb_key_launch(1, Wk->s2, HPx);
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         };
         auto ph_groups = [&]() {
-            GH_HIP(hipEventRecord(G->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(G->s2, G->ev_fork, 0));
-            for (int kc = 256; kc <= ph_capmax; kc *= 2)
-                for (int lat = G->latency ? 1 : 0; lat <= 1; lat++) {
-                    const PhaseShape sh = phase_shape(kc, lat);
-                    GraphGate ga{&ctl->ph_key[0], graph_key(kc, lat), &ctl->ph_n[0], itp};
-                    phase_launch(P, T, kc, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
-                                 0u, status, s, sh, pord, ga);
-                    if (kc < ph_capmax) {
-                        GraphGate gb{&ctl->ph_key[1], graph_key(kc, lat), &ctl->ph_n[1], itp};
-                        phase_launch(P, T, kc, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase,
-                                     seed, 0u, status, G->s2, sh, pord + T, gb);
-                    }
-                }
-            GH_HIP(hipEventRecord(G->ev_join, G->s2));
-            GH_HIP(hipStreamWaitEvent(s, G->ev_join, 0));
+            if (G->gk_ph[1] < 0) {
+                ph_key_launch(0, s);
+                return;
+            }
+            GH_HIP(hipEventRecord(Wk->ev_fork, s));
+            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
+            ph_key_launch(0, s);
+            ph_key_launch(1, Wk->s2);
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         };
         // one PBWT iteration, every decision on the device
         auto iteration = [&]() {
@@ -1200,7 +1230,7 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
             }
             condset_index(P, T, items_d, states, poly, pos, tr, mark, prm->ne, s);
             fb_place_launch(ctl, items_d, fb_d, ph_d, T, L, fb_block_sites(G->maxW), FEW_ITEMS, G->latency, caps,
-                            ford, nst, n_iter, s);
+                            G->gk_fb, ford, nst, n_iter, s);
             condset_fill(P, T, items_d, states, poly, pos, hvar, G->maxW, s);
             hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, 1, 0, prm->min_gl, ploidy_d, HL);
             fb_groups(HP0);
@@ -1211,7 +1241,7 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
             fb_mono_launch(P, T, fb_d, items_d, pos, HL, flat_d, HP1, G->ee_i, G->ed_i, s);
             hipLaunchKernelGGL(sample_kernel, gsite, bsite, 0, s, HP1, H, L, 1, seed, 0u, itp);
             phase_prep_launch(T, ph_d, items_d, poly, H, H, flat_d, P->d_flags.as<uint8_t>(), loci, s);
-            ph_place_launch(ctl, ph_d, items_d, T, FEW_ITEMS, G->latency, caps, pord, s);
+            ph_place_launch(ctl, ph_d, items_d, T, FEW_ITEMS, G->latency, caps, G->gk_ph, pord, s);
             ph_groups();
             hipLaunchKernelGGL(gp_kernel, gsite, bsite, 0, s, HP0, HP1, L, ploidy_d, gp0, gp1, sto, hds, itp,
                                (uint32_t)(1 + prm->burnin));
@@ -1220,17 +1250,7 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         };
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        GH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-            iteration();
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(s, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
-        }
-        GH_HIP(hipStreamEndCapture(s, &graph));
-        struct GraphHold {
+        struct GraphHold {   // released after the replays have finished (the sync below)
             hipGraph_t &g;
             hipGraphExec_t &e;
             ~GraphHold() {
@@ -1238,9 +1258,43 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                 if (g) (void)hipGraphDestroy(g);
             }
         } hold{graph, exec};
-        GH_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        // Capture and instantiation are serialised over the process (runs
+        // executing on several host threads capture one at a time).  A capture
+        // the runtime refuses leaves the stream usable and the iterations are
+        // enqueued without it (GO_UNCAPTURED; the same launches and results).
+        bool captured = false;
+        if (G->graph_mode == 1) {
+            static std::mutex capture_mu;
+            std::lock_guard<std::mutex> lk(capture_mu);
+            (void)hipGetLastError();   // an earlier unchecked status must not be taken for a capture error
+            GH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            std::string why;
+            try {
+                iteration();
+            } catch (const std::exception &e) {
+                why = e.what();
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+            }
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (why.empty() && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                hipError_t e = hipStreamEndCapture(s, &graph);
+                if (e == hipSuccess && graph) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (e == hipSuccess && graph) captured = true;
+                else why = std::string("end/instantiate: ") + hipGetErrorString(e);
+            }
+            (void)hipGetLastError();
+            if (!captured) {
+                G->graph_fail |= GO_UNCAPTURED;
+                if (std::getenv("GH_GRAPH_VERBOSE")) std::fprintf(stderr, "[gh] graph capture refused: %s\n", why.c_str());
+            }
+        }
         for (int it = n_host; it < n_iter; it++) {
-            GH_HIP(hipGraphLaunch(exec, s));
+            if (captured)
+                GH_HIP(hipGraphLaunch(exec, s));
+            else
+                iteration();
             if (it > prm->burnin) main_done++;
         }
         GraphCtl cend;
@@ -1248,9 +1302,13 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         GH_HIP(hipMemcpyAsync(&cend, ctl, sizeof(cend), hipMemcpyDeviceToHost, s));
         GH_HIP(hipMemcpyAsync(nst_h.data(), nst, sizeof(int32_t) * nst_h.size(), hipMemcpyDeviceToHost, s));
         GH_HIP(hipStreamSynchronize(s));
-        if (cend.overflow) return false;
+        if (cend.overflow) {
+            G->graph_fail |= cend.overflow;
+            return false;
+        }
         int32_t sflags[2] = {0, 0};
-        GH_HIP(hipMemcpy(sflags, status, sizeof(sflags), hipMemcpyDeviceToHost));
+        GH_HIP(hipMemcpyAsync(sflags, status, sizeof(sflags), hipMemcpyDeviceToHost, s));
+        GH_HIP(hipStreamSynchronize(s));
         GH_CHECK(sflags[0] == 0, GH_ERR_NUMERIC, "phasing HMM underflow (TRANS_HAP / SAMPLE_DIP)");
         GH_CHECK(sflags[1] == 0, GH_ERR_CAPACITY, "PBWT selection record capacity exceeded");
         for (int t = 0; t < T; t++)
@@ -1263,6 +1321,7 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         st.ph_segments += (double)cend.stats[GS_PH_SEGMENTS];
         st.pbwt_hap_sites += (double)(n_iter - n_host) * 2 * npt * L;
         st.graph_iters = n_iter - n_host;
+        st.graph_reruns = G->graph_fail;   // GO_UNCAPTURED only: the replays ran, enqueued without the graph
         st.ms_graph = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
     }
     if ((learned || collecting) && npt > 0) {
@@ -1286,17 +1345,18 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
 
 void run_execute(gh_run *G) {
     GH_CHECK(G, GH_ERR_ARG, "gh_run_execute: null run");
-    const bool graph = G->graph_mode == 1 && !G->tune && G->collect_from < 1 && !G->any_haploid && G->n_iter > 2;
+    G->graph_fail = 0;
+    const bool graph = G->graph_mode >= 1 && !G->tune && G->collect_from < 1 && !G->any_haploid && G->n_iter > 2;
     if (run_execute_once(G, graph)) return;
     // a replayed iteration outgrew its captured buffers: execute again with
     // every iteration launched from the host (the needs it records size the
     // next capture)
     run_execute_once(G, false);
-    G->st.graph_reruns = 1;
+    G->st.graph_reruns = G->graph_fail;   // GO_* bits
 }
 
 void run_set_graph(gh_run *G, int mode) {
-    GH_CHECK(G && (mode == 0 || mode == 1), GH_ERR_ARG, "gh_run_set_graph: mode must be 0 or 1");
+    GH_CHECK(G && mode >= 0 && mode <= 2, GH_ERR_ARG, "gh_run_set_graph: mode must be 0, 1 or 2");
     G->graph_mode = mode;
 }
 

@@ -19,8 +19,12 @@
 // is then executed again with every iteration launched from the host; the
 // counter-keyed draws make that run identical).
 //
-// Every launch variant a group can take is captured; a variant whose key is
-// not the group's key this iteration returns at once (GraphGate).
+// A launch group's shape key (capacity class, latency shape) is data: the
+// graph holds only the launches of the keys the last host-launched iteration
+// used (HIP graphs have no device-side conditional nodes, and a launch held
+// only to return at once still waits for a CU with room for its workgroups,
+// behind the long-lived workgroups of other runs).  A replay whose keys differ
+// is treated like an overflow.
 #pragma once
 #include <stdint.h>
 
@@ -29,12 +33,16 @@
 
 namespace gh {
 
+// why an execution left the graph: placement exceeded the captured buffers,
+// launch keys changed (both: executed again host-launched), or the runtime
+// refused the capture (the iterations were enqueued without it)
+enum { GO_CAPACITY = 1, GO_SHAPE = 2, GO_UNCAPTURED = 4 };
 enum { GS_FB_STATE_SITES, GS_FB_BYTES, GS_FB_POLY_SITES, GS_PH_STATE_SITES, GS_PH_SEGMENTS, GS_FB_LAUNCHES, GS_N };
 
 // Device control block of a replayed schedule.
 struct GraphCtl {
     uint32_t iter;         // iteration the next replay runs (advanced by the last kernel of the graph)
-    int32_t overflow;      // sticky: a placement needed more than the captured buffers hold
+    int32_t overflow;      // sticky GO_* bits: why a replay placed nothing
     int32_t fb_key[2];     // launch group A (largest capacity class) and B (the rest): shape key
     int32_t fb_n[2];       //   and item count
     int32_t ph_key[2];
@@ -61,10 +69,12 @@ void pbwt_draw_launch(const GraphCtl *ctl, uint32_t seed, const int32_t *grp_ptr
                       int Lhq, uint8_t *stored, const int32_t *targets, int npt, int N, int32_t *start, hipStream_t s);
 // C: the checkpoint block length of the run's layout; few: a group of at most
 // this many items takes the latency shape; ford: [2][T] item orders of A and B.
+// want: the captured launch keys of groups A and B (-1: no group B).
 void fb_place_launch(GraphCtl *ctl, CondItem *items, FbItem *fb, PhItem *ph, int T, int L, int C, int few,
-                     bool latency, const GraphCaps &caps, int32_t *ford, int32_t *nstates, int n_iter, hipStream_t s);
+                     bool latency, const GraphCaps &caps, const int want[2], int32_t *ford, int32_t *nstates,
+                     int n_iter, hipStream_t s);
 void ph_place_launch(GraphCtl *ctl, PhItem *ph, const CondItem *items, int T, int few, bool latency,
-                     const GraphCaps &caps, int32_t *pord, hipStream_t s);
+                     const GraphCaps &caps, const int want[2], int32_t *pord, hipStream_t s);
 void graph_advance_launch(GraphCtl *ctl, hipStream_t s);
 
 }  // namespace gh

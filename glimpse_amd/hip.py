@@ -388,7 +388,8 @@ class Run:
 
     def set_graph(self, mode=1):
         """gh_run_set_graph: 1 (default) = iterations from 2 on replay a
-        captured HIP graph; 0 = every iteration launched from the host."""
+        captured HIP graph; 2 = the same device-decided iterations enqueued
+        without a capture; 0 = every iteration launched from the host."""
         _check(lib().gh_run_set_graph(self.h, int(mode)))
 
     def set_workspace(self, ws):

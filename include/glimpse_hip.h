@@ -90,7 +90,9 @@ typedef struct {
     double fb_nw, fb_mwl;      /* forward-backward launch shape of the last iteration (waves, words per lane) */
     double ph_nw, ph_kpt;      /* phasing launch shape of the last iteration (waves, states per thread) */
     double graph_iters;        /* iterations replayed from the captured graph (gh_run_set_graph) */
-    double graph_reruns;       /* 1 when the execution was repeated without the graph (captured buffers outgrown) */
+    double graph_reruns;       /* why the execution left the graph (bits): 1 captured buffers outgrown, 2 launch
+                                  groups changed (1, 2: repeated host-launched), 4 capture refused by the
+                                  runtime (the same iterations enqueued without a graph) */
     double ms_graph;           /* wall time of the capture and the replays (not in ms_select .. ms_phase) */
 } gh_run_stats;
 
@@ -220,14 +222,18 @@ int gh_run_set_tuning(gh_run *r, int32_t mode);
  * next one is captured once as a HIP graph in which every per-iteration
  * decision (stored PBWT sites and start indices, the placement of Hvar rows,
  * checkpoints, loci and phasing messages by each target's counts, the launch
- * groups and shapes) is made by device kernels, and the graph is replayed for
- * every later iteration with no host round trip.  The placed buffers are sized
- * before the capture from the host-launched iterations with headroom; if a
- * replay needs more, the execution is repeated with mode 0 (same results;
- * gh_run_stats.graph_reruns).  mode 0: every iteration launched from the host
- * (per-stage timings in gh_run_stats).  Runs that tune shapes
- * (gh_run_set_tuning 1), record training rows or have haploid targets launch
- * every iteration from the host.  Results are identical in both modes. */
+ * groups' membership and counts) is made by device kernels, and the graph is
+ * replayed for every later iteration with no host round trip.  The graph holds
+ * the launch shapes of iteration 1's groups; the placed buffers are sized
+ * before the capture from the host-launched iterations with headroom.  If a
+ * replay needs more memory or other shapes, it places nothing and the
+ * execution is repeated with mode 0 (same results; gh_run_stats.graph_reruns).
+ * mode 2: the same device-decided iterations enqueued on the run's stream
+ * without a capture.  mode 0: every iteration launched from the host, with
+ * three host round trips each (per-stage timings in gh_run_stats).  Runs that
+ * tune shapes (gh_run_set_tuning 1), record training rows or have haploid
+ * targets launch every iteration from the host.  Results are identical in all
+ * modes. */
 int gh_run_set_graph(gh_run *r, int32_t mode);
 
 /* Per-target ploidy (1 or 2; default 2), before the first execution: a haploid

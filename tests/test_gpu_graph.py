@@ -105,6 +105,32 @@ def test_graph_second_execution_is_identical(case):
     assert b["stats"]["graph_reruns"] == 0
 
 
+def test_graph_changed_launch_groups_repeat_without_graph(case, monkeypatch):
+    """A graph captured with other launch shapes than the replays' groups need
+    (forced: group A captured with the other latency shape): the first replay
+    places nothing and the execution is repeated host-launched, with the
+    host-launched run's results."""
+    P, gl, flat, panel = case
+    params = _params(seed=10)
+    monkeypatch.setenv("GH_GRAPH_FLIP_LATENCY", "1")
+    g = _exec(panel, gl, flat, params, 1)
+    monkeypatch.delenv("GH_GRAPH_FLIP_LATENCY")
+    h = _exec(panel, gl, flat, params, 0)
+    assert g["stats"]["graph_reruns"] == 2 and g["stats"]["graph_iters"] == 0
+    _same(g, h)
+
+
+def test_device_driven_stream_matches_host_launched(case):
+    """Mode 2: the device-decided iterations enqueued without a capture."""
+    P, gl, flat, panel = case
+    params = _params(seed=11)
+    kp = np.resize(np.array([100, 600, 1500], np.int32), T)
+    g = _exec(panel, gl, flat, params, 2, kp=kp)
+    h = _exec(panel, gl, flat, params, 0, kp=kp)
+    assert g["stats"]["graph_iters"] == 1 + params.burnin + params.main_iter - 2
+    _same(g, h)
+
+
 def test_graph_outgrown_buffers_repeat_without_graph(case, monkeypatch):
     """Captured buffers capped at 5% of what the host-launched iterations
     needed: the first replay's placement overflows, nothing is placed past
