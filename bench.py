@@ -13,11 +13,11 @@ set of held-out target haplotypes each).
 
 One step = the whole job: every (chunk, block of 250 targets) unit runs one
 complete phase run (all iterations) with its inputs already resident in HBM.
-The 40 units are served longest chunk first by 6 host threads, each executing
-its units one after another in its own device workspace, so 6 runs (1,500
-Gibbs chains) are in flight at a time and only their workspaces are held
-(4 in flight: 13.4M/s before the last forward-backward change; 8 units of
-125 targets: slower, smaller launches).
+The 40 units are served longest chunk first by 8 host threads, each executing
+its units one after another in its own device workspace (which owns the
+execution streams), so 8 runs (2,000 Gibbs chains) are in flight at a time
+and only their workspaces are held (about 240 GB of HBM in all; 6 in flight:
+8% slower).
 Within a unit, iterations 0 (INIT) and 1 are launched from the host and
 iteration 2 is captured as a HIP graph that makes every per-iteration decision
 on the device; the graph is replayed for iterations 3-20 (gh_run_set_graph).
@@ -352,7 +352,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="small workload (smoke / profiling of the plumbing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", choices=("both", "plain", "learned"), default="both")
-    ap.add_argument("--workers", type=int, default=0, help="units in flight (default: 6 for C5, all chunks for C2)")
+    ap.add_argument("--workers", type=int, default=0, help="units in flight (default: 8 for C5, all chunks for C2)")
     ap.add_argument("--block", type=int, default=0, help="targets per unit (default: 250 for C5, all for C2)")
     ap.add_argument("--keep", type=float, default=0.6,
                     help="learned path: kept fraction of each target's PBWT pool")
@@ -398,7 +398,7 @@ def main():
              for ci, c in enumerate(chunks) for lo in range(0, c["gl"].shape[0], block)]
     for u in units:
         u.run.set_graph(a.graph)
-    n_workers = a.workers or (6 if a.workload == "c5" else len(chunks))
+    n_workers = a.workers or (8 if a.workload == "c5" else len(chunks))
     workspaces = [hip.Workspace() for _ in range(n_workers)]
     tsites = sum(c["gl"].shape[0] * c["L"] for c in chunks)
     setup_s = time.perf_counter() - t_setup
