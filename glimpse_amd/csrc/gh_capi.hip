@@ -27,6 +27,7 @@ void run_destroy(gh_run *G);
 void run_collect(gh_run *G, int from_iter);
 void run_set_tuning(gh_run *G, int mode);
 void run_set_graph(gh_run *G, int mode);
+void run_set_selector_prior(gh_run *G, float w);
 void workspace_create(gh_workspace **out);
 void workspace_destroy(gh_workspace *w);
 int64_t workspace_bytes(const gh_workspace *w);
@@ -493,6 +494,10 @@ int gh_run_set_tuning(gh_run *r, int32_t mode) {
 
 int gh_run_set_graph(gh_run *r, int32_t mode) {
     return guard([&] { run_set_graph(r, mode); });
+}
+
+int gh_run_set_selector_prior(gh_run *r, float weight) {
+    return guard([&] { run_set_selector_prior(r, weight); });
 }
 int gh_run_collect(gh_run *r, int32_t from_iter) {
     return guard([&] { run_collect(r, from_iter); });

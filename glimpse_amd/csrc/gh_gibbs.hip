@@ -406,6 +406,7 @@ struct gh_run {
     gh::ShapeTuner<gh::PhaseShape> ph_tuner;
     gh_mlp *mlp = nullptr;
     float min_mass = 0.95f;
+    float sel_prior = 0.0f;    // PBWT-order prior of the learned ranking
     int sel_mode = 1;          // 0: global top-k, 1: top-k per PBWT group, groups served round robin
     int collect_from = -1;
     int64_t col_rows = 0;
@@ -780,6 +781,7 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
             ta.targets = ptg; ta.Kpool = Kdev; ta.klearn = G->d_klearn.as<int32_t>();
             ta.y = Wk->fy.as<float>(); ta.row0 = 0; ta.kstride = kst; ta.rstride = prs;
             ta.min_mass = G->min_mass;
+            ta.prior = G->sel_prior;
             ta.ins = Wk->ins.as<int32_t>(); ta.states = states; ta.Kdev = Kdev; ta.items = items_d;
             ta.coverage = G->d_cov.as<float>(); ta.counters = G->d_cnt.as<int32_t>();
             ta.ins_site = G->sel_mode == 1 ? Wk->isite.as<int32_t>() : nullptr;
@@ -1353,6 +1355,11 @@ void run_execute(gh_run *G) {
     // next capture)
     run_execute_once(G, false);
     G->st.graph_reruns = G->graph_fail;   // GO_* bits
+}
+
+void run_set_selector_prior(gh_run *G, float w) {
+    GH_CHECK(G && w >= 0.0f && w < 1e6f, GH_ERR_ARG, "gh_run_set_selector_prior: weight must be in [0, 1e6)");
+    G->sel_prior = w;
 }
 
 void run_set_graph(gh_run *G, int mode) {

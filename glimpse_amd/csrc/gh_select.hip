@@ -332,6 +332,13 @@ __device__ __forceinline__ float block_sum(float v, float *red) {
     return s;
 }
 
+// The ranking score of pool candidate j: the predicted log mass, less the
+// PBWT-order prior (j / K is the candidate's insertion position: the PBWT's
+// own selection of k states is the first k of the pool).
+__device__ __forceinline__ float rank_score(const TopkArgs &a, const float *y, int j, int K) {
+    return y[j] - a.prior * ((float)j / (float)K);
+}
+
 __global__ __launch_bounds__(TOPK_NT) void topk_kernel(TopkArgs a) {
     __shared__ uint64_t s_key[TOPK_MAX];
     __shared__ float red[TOPK_NT / 64];
@@ -346,7 +353,8 @@ __global__ __launch_bounds__(TOPK_NT) void topk_kernel(TopkArgs a) {
     if (!quota) {
         // global ranking: (score desc, insertion order asc)
         for (int j = threadIdx.x; j < n2; j += TOPK_NT)
-            s_key[j] = j < K ? (((uint64_t)ord_key(y[j]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)j)) : 0ull;
+            s_key[j] = j < K ? (((uint64_t)ord_key(rank_score(a, y, j, K)) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)j))
+                             : 0ull;
         __syncthreads();
         bitonic<true>(s_key, n2);
         for (int p = threadIdx.x; p < n2; p += TOPK_NT)
@@ -360,7 +368,7 @@ __global__ __launch_bounds__(TOPK_NT) void topk_kernel(TopkArgs a) {
             uint64_t key = ~0ull;
             if (j < K) {
                 const uint64_t g = (uint64_t)max(sg[isite[j]], 0);
-                key = (g << 43) | ((uint64_t)(~ord_key(y[j])) << 11) | (uint64_t)j;
+                key = (g << 43) | ((uint64_t)(~ord_key(rank_score(a, y, j, K))) << 11) | (uint64_t)j;
             }
             s_key[j] = key;
         }

@@ -65,6 +65,7 @@ struct TopkArgs {
     int64_t row0;
     int kstride, rstride;
     float min_mass;
+    float prior;               // weight of the PBWT-order prior in the ranking (0: scores alone)
     const int32_t *ins;        // [T][kstride]
     int32_t *states;           // [T][kstride] sorted state lists (pool in, selection out)
     int32_t *Kdev;             // [T]

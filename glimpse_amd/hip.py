@@ -85,6 +85,7 @@ def lib():
         L.gh_run_collect.argtypes = [vp, i32]
         L.gh_run_set_tuning.argtypes = [vp, i32]
         L.gh_run_set_graph.argtypes = [vp, i32]
+        L.gh_run_set_selector_prior.argtypes = [vp, f32]
         L.gh_run_collected.argtypes = [vp, vp, vp, vp, vp]
         L.gh_run_set_selector.argtypes = [vp, vp, vp, f32, i32]
         L.gh_select_features.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
@@ -411,6 +412,11 @@ class Run:
         w = np.zeros(n.value, np.float32)
         _check(lib().gh_run_collected(self.h, ctypes.byref(n), _p(X), _p(y), _p(w)))
         return X, y, w
+
+    def set_selector_prior(self, weight):
+        """gh_run_set_selector_prior: rank score = predicted log mass - weight x
+        (PBWT insertion position / pool size)."""
+        _check(lib().gh_run_set_selector_prior(self.h, float(weight)))
 
     def set_selector(self, mlp, k_learn, min_mass=0.95, mode=1):
         """Use the learned selector: target t keeps k_learn[t] states of its PBWT

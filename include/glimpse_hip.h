@@ -321,6 +321,13 @@ int gh_run_collected(gh_run *r, int64_t *n_rows, float *X, float *y, float *w);
  * m = NULL switches it off.  The network must not be trained while the run
  * executes. */
 int gh_run_set_selector(gh_run *r, gh_mlp *m, const int32_t *k_learn, float min_mass, int32_t mode);
+/* The learned ranking with a PBWT-order prior (no reference counterpart): a
+ * candidate's rank score is its predicted log mass less weight x (its position
+ * in the pool / pool size).  The pool is in the PBWT's insertion order
+ * (conditioning_set.cpp:96-104: rank by rank, newest group first), so the plain
+ * method's selection of k states is the pool's first k: weight 0 ranks by the
+ * network alone, a large weight gives back the plain selection. */
+int gh_run_set_selector_prior(gh_run *r, float weight);
 
 /* The selector's two steps as batch calls (what a run does between the PBWT
  * compaction and the conditioning sets).  pool [T][kstride]: candidates in PBWT

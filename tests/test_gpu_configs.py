@@ -234,11 +234,12 @@ def trained_net():
     return net
 
 
-def _run(panel, gl, flat, params, kp=None, net=None, k_learn=None, min_mass=0.95):
+def _run(panel, gl, flat, params, kp=None, net=None, k_learn=None, min_mass=0.95, prior=0.0):
     from glimpse_amd import hip
     r = hip.Run(panel, gl, flat, params, kpbwt_per_target=kp)
     if net is not None:
         r.set_selector(net, k_learn, min_mass=min_mass)
+        r.set_selector_prior(prior)
     t0 = time.time()
     r.execute()
     dt = time.time() - t0
@@ -250,32 +251,37 @@ def _run(panel, gl, flat, params, kp=None, net=None, k_learn=None, min_mass=0.95
 
 
 @pytest.mark.timeout(600)
-def test_c4_learned_k200_against_pbwt(trained_net):
+@pytest.mark.parametrize("seed", [4040, 5050, 6060])
+def test_c4_learned_k200_against_pbwt(trained_net, seed):
     """C4 on held-out seeds: a 4,000-haplotype C1-shaped panel (the 1,000-haplotype
     C1 panel collapses K = 1,000 and 2,000 to the full panel), 24 targets at 1x.
-    The network keeps 200 states of the Kpbwt = 1,000 PBWT pool (no fallback)
-    and is compared with plain PBWT at K = 1,000 and at K = 200.  Measured: the
-    learned K = 200 run stays within 0.005 of the K = 1,000 run's dosage r2 but
-    not within the 0.001 of the configuration's target, and plain PBWT at
-    K = 200 does better than both on this panel; the figures are printed and
-    the test guards the 0.005 gap."""
+    The network keeps 200 states (no fallback; groups served round robin,
+    PBWT-order prior 8) of a Kpbwt = 1,000 pool walked at depth 48, which
+    keeps the 0.1 cM selection interval (at depth 12 allocatePBWT halves the
+    interval of a 1,000-state pool on a 2 cM chunk twice, haplotype_set.cpp:
+    281-286, and the pool holds shallow neighbours at 80 positions).  Against
+    plain PBWT at K = 1,000 (default depth 12): at matched dosage r2 (within
+    0.001) or better with a fifth of the states.  Plain PBWT at K = 200 and at
+    depth 48 are printed beside it."""
     from glimpse_amd import hip
-    P = synth.make_panel(4000, 10000, 24, length_mb=2.0, seed=4040)
-    gl, flat = synth.make_reads(P["tgt_haps"], 1.0, seed=4041)
+    P = synth.make_panel(4000, 10000, 24, length_mb=2.0, seed=seed)
+    gl, flat = synth.make_reads(P["tgt_haps"], 1.0, seed=seed + 1)
     truth = synth.truth_genotypes(P["tgt_haps"])
     panel = hip.Panel(P)
     T = gl.shape[0]
     big = _run(panel, gl, flat, hip.default_params(kpbwt=1000))
     small = _run(panel, gl, flat, hip.default_params(kpbwt=200))
-    lrn = _run(panel, gl, flat, hip.default_params(kpbwt=1000), net=trained_net,
-               k_learn=np.full(T, 200, np.int32), min_mass=0.0)
-    r_big, r_small, r_lrn = (dosage_r2(o["gp0"], o["gp1"], truth) for o in (big, small, lrn))
+    deep = _run(panel, gl, flat, hip.default_params(kpbwt=200, pbwt_depth=48))
+    lrn = _run(panel, gl, flat, hip.default_params(kpbwt=1000, pbwt_depth=48), net=trained_net,
+               k_learn=np.full(T, 200, np.int32), min_mass=0.0, prior=8.0)
+    r_big, r_small, r_deep, r_lrn = (dosage_r2(o["gp0"], o["gp1"], truth) for o in (big, small, deep, lrn))
     ks = lrn["nstates"][:, 1:]
-    print("[c4] r2: PBWT K=1000 %.5f (%.2f s), PBWT K=200 %.5f (%.2f s), learned K=200 of 1000 %.5f (%.2f s); "
-          "learned minus K=1000: %+.5f; learned set sizes %d..%d" % (
-              r_big, big["s"], r_small, small["s"], r_lrn, lrn["s"], r_lrn - r_big, ks.min(), ks.max()))
+    print("[c4] seed %d r2: PBWT K=1000 %.5f (%.2f s), PBWT K=200 %.5f (%.2f s), PBWT K=200 depth 48 %.5f, "
+          "learned K=200 of 1000 %.5f (%.2f s); learned minus K=1000: %+.5f; learned set sizes %d..%d" % (
+              seed, r_big, big["s"], r_small, small["s"], r_deep, r_lrn, lrn["s"], r_lrn - r_big, ks.min(),
+              ks.max()))
     assert ks.max() <= 200
-    assert abs(r_lrn - r_big) <= 0.005, "learned K=200 r2 %.5f vs PBWT K=1000 %.5f" % (r_lrn, r_big)
+    assert r_lrn >= r_big - 0.001, "learned K=200 r2 %.5f vs PBWT K=1000 %.5f" % (r_lrn, r_big)
 
 
 @pytest.mark.timeout(900)
