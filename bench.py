@@ -40,7 +40,10 @@ sites / wall time of one step, for the learned path when its dosage r2 is
 within 0.001 of the plain path's (equal r2), else for the plain path
 (config.path says which).  Both paths' figures, per-chunk throughput, dosage
 r2 / NRC against the simulated truth, and the learned path's distance to the
-plain run are in the same line.
+plain run are in the same line.  large_k_judge: the smallest chunk phased
+again (after the timed steps) with plain PBWT selection at Kpbwt 2,000, the
+large-K plain HMM run that judges the paths' dosage error (mean |DS - DS_2000|
+and r2 on that chunk).
 
 roofline: the dominant kernel (haploid forward-backward) against HBM,
 algorithmic bytes per dispatch / average dispatch duration (HIP events on the
@@ -188,6 +191,33 @@ def accuracy(outs, chunks):
     match_nonref = np.sum((g == tr) & (tr > 0))
     nrc = 1.0 - mism / max(1, mism + match_nonref)
     return dict(dosage_r2=float(r * r), nrc=float(nrc))
+
+
+def large_k_judge(chunks, panels, workspaces, res, units, k_judge=2000):
+    """The large-K plain HMM run as the judge of the learned path's dosage
+    error: the smallest chunk phased again with plain PBWT selection at
+    Kpbwt = k_judge for every target (outside the timed region); the mean
+    absolute DS difference of each timed path's output from it, and each one's
+    dosage r2 on that chunk."""
+    ci = min(range(len(chunks)), key=lambda c: chunks[c]["L"])
+    c = chunks[ci]
+    T = c["gl"].shape[0]
+    jc = dict(c, K=np.full(T, k_judge, np.int32))
+    block = max(u.hi - u.lo for u in units)
+    ju = [Unit(jc, ci, lo, min(lo + block, T), panels[ci]) for lo in range(0, T, block)]
+    run_units(ju, workspaces[:len(ju)], "large-K judge (chunk %d, K=%d)" % (ci, k_judge))
+    fs = [u.run.fetch() for u in sorted(ju, key=lambda u: u.lo)]
+    ds_j = np.concatenate([dosage(f["gp0"], f["gp1"]) for f in fs])
+    for u in ju:
+        u.run.close()
+    tr = c["truth"].astype(np.float64).ravel()
+    out = dict(chunk_sites=c["L"], kpbwt=k_judge,
+               dosage_r2=float(np.corrcoef(ds_j.ravel(), tr)[0, 1] ** 2))
+    for p, R in res.items():
+        ds_p = R["ds_small"]
+        out[p] = dict(mean_abs_ds_vs_judge=float(np.mean(np.abs(ds_p - ds_j))),
+                      dosage_r2=float(np.corrcoef(ds_p.ravel(), tr)[0, 1] ** 2))
+    return out
 
 
 # ------------------------------------------------------------------ CPU baseline
@@ -351,6 +381,7 @@ def main():
     ap.add_argument("--workload", choices=("c5", "c2"), default="c5")
     ap.add_argument("--quick", action="store_true", help="small workload (smoke / profiling of the plumbing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-judge", action="store_true", help="skip the large-K plain run that judges dosage error")
     ap.add_argument("--path", choices=("both", "plain", "learned"), default="both")
     ap.add_argument("--workers", type=int, default=0, help="units in flight (default: 8 for C5, all chunks for C2)")
     ap.add_argument("--block", type=int, default=0, help="targets per unit (default: 250 for C5, all for C2)")
@@ -428,6 +459,7 @@ def main():
         run_units(big, workspaces, "workspace sizing (%s)" % p)
 
     res = {}
+    small = min(range(len(chunks)), key=lambda c: chunks[c]["L"])   # the large-K judge's chunk
     for p in paths:
         set_path(units, p, net, a.keep, a.min_mass, chunks)
         for i in range(a.warmup):
@@ -451,7 +483,8 @@ def main():
                       graph=dict(replayed_iterations=int(sum(s["graph_iters"] for s in stats)),
                                  iterations=int(sum(o["nstates"].shape[1] for o in outs for _ in o["stats"])),
                                  reruns=int(sum(s["graph_reruns"] for s in stats))),
-                      ds=np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in outs]))
+                      ds=np.concatenate([dosage(o["gp0"], o["gp1"]).ravel() for o in outs]),
+                      ds_small=dosage(outs[small]["gp0"], outs[small]["gp1"]))
         if p == "learned":
             res[p]["target_iterations_learned"] = sum(s["learned_sel"] for s in stats)
             res[p]["target_iterations_pool_kept"] = sum(s["pool_kept"] for s in stats)
@@ -459,6 +492,12 @@ def main():
             p, res[p]["value"] / 1e6, res[p]["accuracy"]["dosage_r2"], res[p]["accuracy"]["nrc"]))
     free_b, total_b = torch.cuda.mem_get_info()
     hbm_used_gb = (total_b - free_b) / 1e9
+
+    judge = None
+    if not a.no_judge:
+        judge = large_k_judge(chunks, panels, workspaces, {("learned" if p == "learned" else "plain_pbwt"): res[p]
+                                                            for p in paths}, units)
+        log("large-K judge: %s" % json.dumps(judge))
 
     # roofline probe: the largest unit alone on the plain path
     u = max(units, key=lambda u: u.work)
@@ -498,6 +537,7 @@ def main():
                                                                      "valu_lane_insts_per_state_site")},
         "accuracy": M["accuracy"],
         "per_chunk": M["per_chunk"],
+        "large_k_judge": judge,
         "setup_s": round(setup_s, 1),
         "hbm_used_gb": round(hbm_used_gb, 1),
     }
