@@ -315,11 +315,11 @@ def run_units(units, workspaces, label):
     return dt
 
 
-def set_path(units, path, net, keep, min_mass, chunks):
+def set_path(units, path, net, keep, min_mass, chunks, mode=1):
     for u in units:
         if path == "learned":
             kl = np.maximum(1, np.round(chunks[u.ci]["K"][u.lo:u.hi] * keep)).astype(np.int32)
-            u.run.set_selector(net, kl, min_mass)
+            u.run.set_selector(net, kl, min_mass, mode=mode)
         else:
             u.run.set_selector(None, None)
 
@@ -397,6 +397,9 @@ def main():
     ap.add_argument("--graph", type=int, choices=(0, 1, 2), default=1,
                     help="1: iterations from 2 on replay a captured HIP graph; 0: every iteration launched from the "
                          "host (gh_run_set_graph)")
+    ap.add_argument("--sel-mode", type=int, default=1, choices=(0, 1),
+                    help="learned selection mode (gh_run_set_selector): 0 best scores of the pool, 1 groups served "
+                         "round robin")
     ap.add_argument("--seed", type=int, default=15052011)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -455,13 +458,13 @@ def main():
     # workspace sizing: each worker executes its largest unit once per path (outputs discarded)
     big = sorted(units, key=lambda u: -u.work)[:n_workers]
     for p in paths:
-        set_path(big, p, net, a.keep, a.min_mass, chunks)
+        set_path(big, p, net, a.keep, a.min_mass, chunks, a.sel_mode)
         run_units(big, workspaces, "workspace sizing (%s)" % p)
 
     res = {}
     small = min(range(len(chunks)), key=lambda c: chunks[c]["L"])   # the large-K judge's chunk
     for p in paths:
-        set_path(units, p, net, a.keep, a.min_mass, chunks)
+        set_path(units, p, net, a.keep, a.min_mass, chunks, a.sel_mode)
         for i in range(a.warmup):
             run_units(units, workspaces, "%s warmup step %d" % (p, i))
         torch.cuda.synchronize()

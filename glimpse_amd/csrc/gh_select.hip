@@ -614,6 +614,7 @@ void select_topk_batch(int T, const float *y_h, const int32_t *ins_h, const int3
         items[t].W = (K_h[t] + 63) / 64;
     }
     TopkArgs a;
+    a.prior = 0.0f;   // the batch call ranks by the scores alone
     a.targets = dalloc_copy(tg, tv.data(), (size_t)T, s);
     a.y = dalloc_copy(y, y_h, (size_t)T * kstride, s);
     a.ins = dalloc_copy(ins, ins_h, (size_t)T * kstride, s);
