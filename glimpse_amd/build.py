@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 HIP_SOURCES = [
     "gh_capi.hip", "gh_panel.hip", "gh_condset.hip", "gh_impute.hip", "gh_phase.hip", "gh_pbwt.hip", "gh_gibbs.hip",
     "gh_mlp.hip", "gh_select.hip", "gh_output.hip", "gh_noalloc.hip", "gh_concord.hip", "gh_ligate.hip",
-    "gh_bam.hip", "gh_graph.hip",
+    "gh_bam.hip", "gh_graph.hip", "gh_init.hip",
 ]
 
 # host-only C++ (file I/O on zlib), built with g++ into the same library

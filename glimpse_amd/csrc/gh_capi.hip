@@ -27,6 +27,8 @@ void run_destroy(gh_run *G);
 void run_collect(gh_run *G, int from_iter);
 void run_set_tuning(gh_run *G, int mode);
 void run_set_graph(gh_run *G, int mode);
+void init_states_batch(gh_panel *P, int T, const uint8_t *gl, const uint8_t *flat, int kinit, uint32_t seed,
+                       int kstride, int32_t *states, int32_t *K);
 void run_set_selector_prior(gh_run *G, float w);
 void workspace_create(gh_workspace **out);
 void workspace_destroy(gh_workspace *w);
@@ -490,6 +492,11 @@ int gh_run_set_workspace(gh_run *r, gh_workspace *w) {
 
 int gh_run_set_tuning(gh_run *r, int32_t mode) {
     return guard([&] { run_set_tuning(r, mode); });
+}
+
+int gh_init_states(gh_panel *p, int32_t T, const uint8_t *gl, const uint8_t *flat, int32_t kinit, uint32_t seed,
+                   int32_t kstride, int32_t *states, int32_t *K) {
+    return guard([&] { init_states_batch(p, T, gl, flat, kinit, seed, kstride, states, K); });
 }
 
 int gh_run_set_graph(gh_run *r, int32_t mode) {

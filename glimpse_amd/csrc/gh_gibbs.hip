@@ -20,7 +20,6 @@
 #include <cstring>
 #include <map>
 #include <numeric>
-#include <set>
 #include <atomic>
 #include <mutex>
 #include <string>
@@ -28,6 +27,7 @@
 
 #include "gh_device.hpp"
 #include "gh_graph.hpp"
+#include "gh_init.hpp"
 #include "gh_internal.hpp"
 #include "gh_pbwt.hpp"
 #include "gh_phase.hpp"
@@ -38,7 +38,7 @@ namespace gh {
 __constant__ float c_unphred[256];
 
 // common/src/utils/otools.h:97-98: 10^(-i/10) printed with 7 significant digits.
-static void unphred_table(float *t) {
+void unphred_table(float *t) {
     char buf[64];
     for (int i = 0; i < 256; i++) {
         std::snprintf(buf, sizeof(buf), "%.6e", std::pow(10.0, -i / 10.0));
@@ -185,89 +185,6 @@ __global__ void set_k_kernel(CondItem *__restrict__ items, const int32_t *__rest
     items[t].W = (K + 63) / 64;
 }
 
-/* ------------------------------------------------------------------ host: INIT selection */
-static inline float init_u(uint32_t seed, uint32_t tgt, uint32_t ctr) {
-    return u01(philox_draw(seed, 0u, P_INIT_SEL, tgt, ctr));
-}
-// Selection sampling (Knuth, TAOCP vol. 2, Algorithm S) in place of std::sample.
-static void select_sample(const int32_t *pop, int n, int k, std::vector<int32_t> &out, uint32_t seed, uint32_t tgt,
-                          uint32_t &ctr) {
-    int need = std::min(k, n);
-    for (int i = 0; i < n && need > 0; i++) {
-        const float u = init_u(seed, tgt, ctr++);
-        if (u * (float)(n - i) < (float)need) { out.push_back(pop[i]); need--; }
-    }
-}
-
-// haplotype_set::initRareTar + performSelection_RARE_INIT_GL (haplotype_set.cpp:108-169, 775-833).
-// gl: the target's [L][3] values.  For a haploid target the reference reads its
-// GL array ([L][2]) at 2v + 0..2 (its offsets assume three values), so the
-// third value it compares is the next site's first one (0 past the last
-// site), and the third probability is 0.
-static void init_states(gh_panel *P, int t, const uint8_t *gl, const uint8_t *flat, int kinit, uint32_t seed,
-                        std::vector<int32_t> &states, const float *unph, int ploidy) {
-    const int L = P->L, N = P->N;
-    std::vector<int> sind;
-    for (int v = 0; v < L; v++) {
-        if (flat[v]) continue;
-        const uint8_t f = P->flags[v];
-        const int mg = 2 * ((f & SF_MAJOR) ? 1 : 0);
-        if (!(f & SF_COMMON) && !(f & SF_LQ)) {
-            uint8_t g[3] = {gl[3 * v], gl[3 * v + 1], gl[3 * v + 2]};
-            if (ploidy == 1) g[2] = v + 1 < L ? gl[3 * (v + 1)] : 0;
-            if (g[1] <= g[mg] || g[2 - mg] <= g[mg]) {
-                float x[3] = {unph[g[0]], unph[g[1]], ploidy == 1 ? 0.0f : unph[g[2]]};
-                float s = x[0] + x[1] + x[2];
-                x[0] /= s; x[1] /= s; x[2] /= s;
-                const float af = P->calt[v] * 1.0 / N;
-                x[0] *= (1.0f - af) * (1.0f - af);
-                x[1] *= 2.0f * af * (1.0f - af);
-                x[2] *= (af * af);
-                s = x[0] + x[1] + x[2];
-                x[0] /= s; x[1] /= s; x[2] /= s;
-                if (x[1] + x[2 - mg] > x[mg]) sind.push_back(v);
-            }
-        }
-    }
-    const int k_init = kinit, k_init8 = (int)std::floor(k_init * 0.8f);
-    std::set<int32_t> S;
-    std::vector<int32_t> sampled;
-    uint32_t ctr = 0;
-    const int32_t *rp = P->rare_ptr_h.data(), *ri = P->rare_idx_h.data();
-    if (!sind.empty()) {
-        if ((int)sind.size() > k_init8) {
-            std::vector<std::pair<int, int>> mac;
-            for (int v : sind) mac.push_back({(int)std::min(P->cref[v], P->calt[v]), v});
-            std::sort(mac.begin(), mac.end());
-            std::vector<int32_t> io;
-            for (size_t r0 = 0; r0 < sind.size() && (int)S.size() < k_init8; r0++) {
-                const int v = mac[r0].second;
-                const int nc = rp[v + 1] - rp[v];
-                io.resize(nc);
-                std::iota(io.begin(), io.end(), 0);
-                sampled.clear();
-                select_sample(io.data(), nc, 5, sampled, seed, (uint32_t)t, ctr);
-                for (size_t i = 0; i < sampled.size() && (int)S.size() < k_init8; i++) S.insert(ri[rp[v] + sampled[i]]);
-            }
-        } else {
-            const int size_max = std::max((int)std::floor(k_init8 * 1.0 / sind.size()), 1);
-            for (int v : sind) {
-                sampled.clear();
-                select_sample(ri + rp[v], rp[v + 1] - rp[v], size_max, sampled, seed, (uint32_t)t, ctr);
-                for (size_t i = 0; i < sampled.size() && (int)S.size() < k_init8; i++) S.insert(sampled[i]);
-            }
-        }
-    }
-    if ((int)S.size() < k_init) {
-        std::vector<int32_t> io(N);
-        std::iota(io.begin(), io.end(), 0);
-        sampled.clear();
-        select_sample(io.data(), N, k_init, sampled, seed, (uint32_t)t, ctr);
-        for (size_t i = 0; i < sampled.size() && (int)S.size() < k_init; i++) S.insert(sampled[i]);
-    }
-    states.assign(S.begin(), S.end());
-}
-
 /* ------------------------------------------------------------------ driver */
 // Online launch-shape tuner: while a class (set-size bucket) is being tuned,
 // each iteration runs the next untimed candidate shape and records its kernel
@@ -307,18 +224,6 @@ struct ShapeTuner {
 // per-iteration counts settle after the first iterations.
 static inline void grow(DBuf &b, size_t bytes) {
     if (bytes > b.bytes) b.reserve(bytes + bytes / 4);
-}
-
-// f(t) for t in [0, n) on up to 16 host threads (f must be thread safe).
-template <typename F>
-static void parallel_targets(int n, F &&f) {
-    const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
-    if (nt == 1) { for (int t = 0; t < n; t++) f(t); return; }
-    std::atomic<int> next(0);
-    std::vector<std::thread> th;
-    for (int i = 0; i < nt; i++)
-        th.emplace_back([&]() { for (int t = next++; t < n; t = next++) f(t); });
-    for (auto &x : th) x.join();
 }
 
 // A launch of at most this many items uses the latency launch shapes.
@@ -373,7 +278,6 @@ struct gh_run {
     gh_params prm;
     float ee_i = 0, ed_i = 0;
     float unph[256];
-    std::vector<uint8_t> gl_h, flat_h;   // host copies for the INIT selection
     std::vector<int> kp, kmax, hap_cls, pbwt_targets;
     std::vector<uint8_t> ploidy;         // [T] 1 or 2 (genotype::ploidy)
     bool any_haploid = false;
@@ -383,7 +287,6 @@ struct gh_run {
     std::vector<gh::FbItem> fb;
     std::vector<gh::PhItem> ph;
     std::vector<uint8_t> stored_h;
-    std::vector<int32_t> init_buf;   // host state sets of the INIT iteration, [T][kstride]
     std::vector<int32_t> start_h, nstates;   // nstates [T][n_iter]
     gh::DBuf d_gl, d_flat, d_K, d_items, d_fb, d_gp0, d_gp1, d_stored, d_hds, d_h0, d_h1, d_ph, d_status, d_pst,
         d_pstart, d_pcls, d_pmod, d_evcnt, d_pkp, d_psh, d_prow, d_ptg, d_hrow, d_ploidy, d_pone, d_dord;
@@ -418,6 +321,8 @@ struct gh_run {
     int gk_fb[2] = {-1, -1}, gk_ph[2] = {-1, -1};   // launch keys of the last host-launched iteration
     int graph_fail = 0;        // GO_* bits of the last execution whose replays placed nothing
     gh::DBuf d_ctl, d_nst, d_gptr, d_gend;
+    gh::DBuf d_unph, d_fullp;  // the unphred table; full-panel targets (Kpbwt >= N)
+    int n_full = 0;
     ~gh_run() {
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -478,8 +383,6 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         G->n_iter = 1 + prm->burnin + prm->main_iter;
         unphred_table(G->unph);
         G->ploidy.assign(T, 2);
-        G->gl_h.assign(gl_h, gl_h + (size_t)T * L * 3);
-        G->flat_h.assign(flat_h, flat_h + (size_t)T * L);
         // per-target K (caller_initialise.cpp:150) and PBWT classes by effective modulo
         G->kp.resize(T);
         for (int t = 0; t < T; t++) G->kp[t] = std::min(kpt ? kpt[t] : prm->kpbwt, N);
@@ -587,6 +490,14 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
         // per-iteration tables, allocated once here (an execution frees nothing:
         // a free waits for the whole device)
         G->d_ph.reserve(sizeof(PhItem) * T);
+        dalloc_copy(G->d_unph, G->unph, 256, s);
+        {
+            std::vector<int32_t> fullp;
+            for (int t = 0; t < T; t++)
+                if (G->kp[t] >= N) fullp.push_back(t);
+            G->n_full = (int)fullp.size();
+            dalloc_copy(G->d_fullp, fullp.data(), fullp.size(), s);
+        }
         G->d_ford.reserve(sizeof(int32_t) * 2 * (size_t)T);
         G->d_pord.reserve(sizeof(int32_t) * 2 * (size_t)T);
         G->d_dord.reserve(sizeof(int32_t) * (size_t)T);
@@ -795,45 +706,21 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         const int stage = it == 0 ? 0 : (it <= prm->burnin ? 1 : 2);
         bool learn_ev = false;
         GH_HIP(hipEventRecord(e_sel.a, s));
-        if (stage == 0 || it == 1) {
-            // host-side state sets: INIT selection, and full-panel targets (fixed
-            // iota); targets spread over host threads, every set in one buffer
-            // laid out with the run's stride, one copy
-            std::vector<int32_t> &buf = G->init_buf;
-            buf.resize((size_t)T * G->kstride);
-            std::vector<int32_t> kset(T, 0);
-            std::vector<std::string> errs;
-            std::mutex emu;
-            parallel_targets(T, [&](int t) {
-                std::vector<int32_t> S;
-                if (stage == 0 && G->kinit > 0)
-                    init_states(P, t, G->gl_h.data() + (size_t)t * L * 3, G->flat_h.data() + (size_t)t * L,
-                                G->kinit, seed, S, G->unph, G->ploidy[t]);
-                else if (G->kp[t] >= N) { S.resize(N); std::iota(S.begin(), S.end(), 0); }
-                else return;
-                if (S.empty() || (int)S.size() > G->kstride) {
-                    std::lock_guard<std::mutex> lk(emu);
-                    errs.push_back("target " + std::to_string(t) + ": " +
-                                   (S.empty() ? "zero conditioning states" : "state set larger than the run's stride"));
-                    return;
-                }
-                std::memcpy(buf.data() + (size_t)t * G->kstride, S.data(), sizeof(int32_t) * S.size());
-                kset[t] = (int)S.size();
-            });
-            GH_CHECK(errs.empty(), GH_ERR_STATE, errs.empty() ? std::string() : errs[0]);
-            bool any = false;
-            for (int t = 0; t < T; t++)
-                if (kset[t] > 0) {
-                    items[t].K = kset[t];
-                    items[t].W = (kset[t] + 63) / 64;
-                    any = true;
-                }
-            if (any) {
-                // targets without a host set (PBWT targets at it == 1) are rewritten by the walk
-                GH_HIP(hipMemcpyAsync(states, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice, s));
-                GH_HIP(hipMemcpyAsync(items_d, items.data(), sizeof(CondItem) * T, hipMemcpyHostToDevice, s));
-                GH_HIP(hipStreamSynchronize(s));   // buf is reused by the next execution
-            }
+        if (stage == 0 && G->kinit > 0) {
+            // INIT selection on the device (haplotype_set.cpp:108-169, 775-833); the
+            // rare-site lists use the poly / pos scratch before the conditioning sets do
+            InitArgs ia;
+            ia.targets = nullptr;
+            ia.gl = gl_d; ia.flat = flat_d; ia.ploidy = ploidy_d; ia.flags = P->d_flags.as<uint8_t>();
+            ia.cref = P->d_cref.as<uint32_t>(); ia.calt = P->d_calt.as<uint32_t>();
+            ia.rare_ptr = P->d_rare_ptr.as<int32_t>(); ia.rare_idx = P->d_rare_idx.as<int32_t>();
+            ia.unph = G->d_unph.as<float>();
+            ia.L = L; ia.N = N; ia.kinit = G->kinit; ia.kstride = kst; ia.max_mac = P->max_rare_mac; ia.seed = seed;
+            ia.sind = poly; ia.sorted = pos; ia.states = states; ia.K = Kdev; ia.items = items_d;
+            init_select_launch(ia, T, s);
+        } else if ((stage == 0 || it == 1) && G->n_full > 0) {
+            // full-panel targets (conditioning_set.cpp:105-113); PBWT targets are set by the walk
+            full_panel_launch(G->d_fullp.as<int32_t>(), G->n_full, N, kst, states, Kdev, items_d, s);
         }
         if (stage > 0 && npt > 0) {
             // haplotype_set.cpp:335-348: one stored site per group, random start per haplotype

@@ -58,6 +58,7 @@ inline T *dalloc_copy(DBuf &buf, const T *h, size_t n, hipStream_t s) {
 // Resident panel (one chunk).
 struct gh_panel {
     int L = 0, N = 0, Lc = 0, Lhq = 0;
+    int max_rare_mac = 0;             // largest minor-allele count of a rare, high-quality site
     int W64 = 0;   // ceil(N/64): words per common site row
     int Wc = 0;    // ceil(Lc/64): words per haplotype row of the transposed panel
     std::vector<double> cm;
@@ -71,7 +72,7 @@ struct gh_panel {
     hipStream_t stream = nullptr;
 
     gh::DBuf d_cm, d_cmpos, d_flags, d_crow, d_c2t, d_sm, d_hm, d_shap_ptr, d_shap_idx, d_rare_ptr,
-        d_rare_idx;
+        d_rare_idx, d_cref, d_calt;
     // sparse PBWT (per-site structures for the selection kernel)
     std::vector<uint8_t> ypacked;
     std::vector<int32_t> asmall_ptr, asmall_idx;

@@ -65,6 +65,7 @@ void panel_create(const gh_panel_desc *d, gh_panel **out) {
             P->flags[l] = (c ? SF_COMMON : 0) | (m ? SF_MAJOR : 0) | (lq ? SF_LQ : 0);
             if (c) { crow[l] = (int32_t)P->common2tot.size(); P->common2tot.push_back(l); }
             P->Lhq += c && !lq;
+            if (!c && !lq) P->max_rare_mac = std::max(P->max_rare_mac, (int)std::min(d->cref[l], d->calt[l]));
             P->cm_pos[l] = std::max(0.0f, (float)P->cm[l]);
         }
         P->Lc = (int)P->common2tot.size();
@@ -102,6 +103,8 @@ void panel_create(const gh_panel_desc *d, gh_panel **out) {
         dalloc_copy(P->d_shap_idx, shap_idx.data(), (size_t)nr, s);
         dalloc_copy(P->d_rare_ptr, d->rare_ptr, L + 1, s);
         dalloc_copy(P->d_rare_idx, d->rare_idx, (size_t)nr, s);
+        dalloc_copy(P->d_cref, d->cref, L, s);
+        dalloc_copy(P->d_calt, d->calt, L, s);
         P->d_hm.reserve(sizeof(uint64_t) * std::max<size_t>((size_t)N * P->Wc, 1));
         if (P->Lc > 0) {
             dim3 grid(P->Wc, P->W64);
@@ -118,7 +121,8 @@ void panel_create(const gh_panel_desc *d, gh_panel **out) {
         if (!P->ypacked.empty()) pbwt_decode(P);
         P->device_bytes += (int64_t)(P->d_cm.bytes + P->d_cmpos.bytes + P->d_flags.bytes + P->d_crow.bytes +
                                     P->d_c2t.bytes + P->d_sm.bytes + P->d_hm.bytes + P->d_shap_ptr.bytes +
-                                    P->d_shap_idx.bytes + P->d_rare_ptr.bytes + P->d_rare_idx.bytes);
+                                    P->d_shap_idx.bytes + P->d_rare_ptr.bytes + P->d_rare_idx.bytes +
+                                    P->d_cref.bytes + P->d_calt.bytes);
         *out = P;
     } catch (...) {
         if (P->stream) (void)hipStreamDestroy(P->stream);

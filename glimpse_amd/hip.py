@@ -85,6 +85,7 @@ def lib():
         L.gh_run_collect.argtypes = [vp, i32]
         L.gh_run_set_tuning.argtypes = [vp, i32]
         L.gh_run_set_graph.argtypes = [vp, i32]
+        L.gh_init_states.argtypes = [vp, i32, vp, vp, i32, u32, i32, vp, vp]
         L.gh_run_set_selector_prior.argtypes = [vp, f32]
         L.gh_run_collected.argtypes = [vp, vp, vp, vp, vp]
         L.gh_run_set_selector.argtypes = [vp, vp, vp, f32, i32]
@@ -217,6 +218,19 @@ class Panel:
             out.append(dict(poly=poly[p0:p1].copy(), hvar=hv[hv_ptr[i]:hv_ptr[i + 1]].reshape(Lp, W).copy(),
                             t=t[p0 - i:p1 - i - 1].copy()))
         return out
+
+    def init_states(self, gl, flat, kinit, seed, kstride=None):
+        """gh_init_states: the INIT iteration's state sets of T diploid targets
+        (gl [T][L][3], flat [T][L]); a list of ascending id arrays."""
+        gl = np.ascontiguousarray(gl, np.uint8)
+        flat = np.ascontiguousarray(flat, np.uint8)
+        T = flat.shape[0]
+        ks = int(kstride or min(kinit, self.N))
+        st = np.zeros((T, ks), np.int32)
+        K = np.zeros(T, np.int32)
+        _check(lib().gh_init_states(self.h, T, _p(gl), _p(flat), int(kinit), int(seed) & 0xFFFFFFFF, ks, _p(st),
+                                    _p(K)))
+        return [st[t, :K[t]] for t in range(T)]
 
     def impute_batch(self, state_lists, HL, flat, err_imp=1e-12, ne=100000):
         ptr, ids = _csr(state_lists)

@@ -114,6 +114,15 @@ int gh_panel_create(const gh_panel_desc *d, gh_panel **out);
 int gh_panel_destroy(gh_panel *p);
 int gh_panel_info(const gh_panel *p, int32_t *n_common, int32_t *n_common_hq, int64_t *device_bytes);
 
+/* The INIT iteration's state selection: haplotype_set::initRareTar +
+ * performSelection_RARE_INIT_GL (phase/src/containers/haplotype_set.cpp:
+ * 108-169, 775-833), for T diploid targets at once, with the counter draws of
+ * gh_run_execute's INIT iteration (selection sampling in place of std::sample).
+ * gl [T][L][3], flat [T][L]; out: states [T][kstride] (ascending ids), K [T].
+ * kinit is clipped to N. */
+int gh_init_states(gh_panel *p, int32_t T, const uint8_t *gl, const uint8_t *flat, int32_t kinit, uint32_t seed,
+                   int32_t kstride, int32_t *states, int32_t *K);
+
 /* conditioning_set::select for explicit state sets, batched
  * (phase/src/containers/conditioning_set.cpp:75-196).  Item i uses
  * states[state_ptr[i] .. state_ptr[i+1]) (sorted, unique).  Outputs per item:
