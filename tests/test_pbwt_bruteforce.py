@@ -13,6 +13,11 @@ interval), after site k:
   last reset (an empty interval or a stored site) to k.
 The selection lists are then the reference's rule applied to these
 positions; the GPU reproduces them bit for bit (test_gpu_pbwt.py).
+
+At 40-64 haplotypes every polymorphic site has a minor-allele frequency of at
+least 1/64, above the default sparse_maf of 0.001, so these panels are all
+common sites (the rare_frac argument only shapes the allele spectrum); rare
+runs are pinned in test_pbwt_rare_bruteforce.py.
 """
 import numpy as np
 import pytest
