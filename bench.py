@@ -11,13 +11,13 @@ default Gibbs schedule (1 INIT + 5 burn-in + 15 main), Kpbwt 1,000.  The
 chunks are independent synthetic regions (one panel, one sparse PBWT and one
 set of held-out target haplotypes each).
 
-One step = the whole job: every (chunk, block of 250 targets) unit runs one
+One step = the whole job: every (chunk, block of 125 targets) unit runs one
 complete phase run (all iterations) with its inputs already resident in HBM.
-The 40 units are served longest chunk first by 8 host threads, each executing
+The 80 units are served longest chunk first by 16 host threads, each executing
 its units one after another in its own device workspace (which owns the
-execution streams), so 8 runs (2,000 Gibbs chains) are in flight at a time
-and only their workspaces are held (about 240 GB of HBM in all; 6 in flight:
-8% slower).
+execution streams), so 16 runs (2,000 Gibbs chains) are in flight at a time
+and only their workspaces are held (about 240 GB of HBM in all; 8 units of
+250 targets: 4% slower on the plain path, 2% on the learned one).
 Within a unit, iterations 0 (INIT) and 1 are launched from the host and
 iteration 2 is captured as a HIP graph that makes every per-iteration decision
 on the device; the graph is replayed for iterations 3-20 (gh_run_set_graph).
@@ -47,11 +47,13 @@ and r2 on that chunk).
 
 roofline: the dominant kernel (haploid forward-backward) against HBM,
 algorithmic bytes per dispatch / average dispatch duration (HIP events on the
-launch stream), measured on one unit running alone (the largest chunk's first
-block, plain path: 500 haplotypes per dispatch fill the CUs; every iteration
-launched from the host so each dispatch is timed), so no other dispatch shares
-its time; `step_level_frac` is the whole step's algorithmic
-forward-backward bytes over the step time; `traffic` is the measured HBM bytes
+launch stream), measured on one run alone at full occupancy (all 1,000 targets
+of the largest chunk in one run, plain path: 2,000 haplotype items per dispatch
+keep every CU's workgroup slots filled; every iteration launched from the host
+so each dispatch is timed; --probe-targets sets the count), after the step's
+runs and workspaces are released, so no other dispatch shares its time;
+`step_level_frac` is the whole step's algorithmic forward-backward bytes over
+the step time; `traffic` is the measured HBM bytes
 per dispatch from the committed rocprofv3 counter summary of the same probe.
 cpu_baseline: the CPU oracle (a C++ restatement of the reference path) timed
 on this host on a bounded sample of the same workload: full 1+5+15 runs of as
@@ -96,7 +98,7 @@ def build_c5(quick, seed):
     if quick:
         N, T, n_chunks, lo, hi, block = 20000, 200, 3, 10000, 30000, 100
     else:
-        N, T, n_chunks, lo, hi, block = 100000, 1000, 10, 40000, 150000, 250
+        N, T, n_chunks, lo, hi, block = 100000, 1000, 10, 40000, 150000, 125
     rng = np.random.default_rng(seed)
     lens = [int(x) for x in rng.integers(lo, hi + 1, n_chunks)]
     cov = np.exp(rng.uniform(np.log(0.1), np.log(4.0), T))
@@ -280,6 +282,15 @@ class Unit:
         return self.L * (self.hi - self.lo)
 
 
+def probe_unit(chunks, panels, n):
+    """The roofline probe's unit: the first n targets of the largest chunk,
+    whatever the step's block size (2n haplotype items per forward-backward
+    dispatch)."""
+    ci = max(range(len(chunks)), key=lambda c: chunks[c]["L"])
+    c = chunks[ci]
+    return Unit(c, ci, 0, min(n, c["gl"].shape[0]), panels[ci])
+
+
 def run_units(units, workspaces, label):
     """Serves the units (longest first) to one host thread per workspace;
     returns the wall time.  Each thread executes its units one after another."""
@@ -383,15 +394,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-judge", action="store_true", help="skip the large-K plain run that judges dosage error")
     ap.add_argument("--path", choices=("both", "plain", "learned"), default="both")
-    ap.add_argument("--workers", type=int, default=0, help="units in flight (default: 8 for C5, all chunks for C2)")
-    ap.add_argument("--block", type=int, default=0, help="targets per unit (default: 250 for C5, all for C2)")
+    ap.add_argument("--workers", type=int, default=0, help="units in flight (default: 16 for C5, all chunks for C2)")
+    ap.add_argument("--block", type=int, default=0, help="targets per unit (default: 125 for C5, all for C2)")
     ap.add_argument("--keep", type=float, default=0.6,
                     help="learned path: kept fraction of each target's PBWT pool")
     ap.add_argument("--min-mass", type=float, default=0.95,
                     help="learned path: a target whose kept states hold less than this share of the predicted mass "
                          "keeps its whole PBWT pool (the plain selection) for that iteration")
+    ap.add_argument("--probe-targets", type=int, default=1000,
+                    help="targets of the roofline probe's unit (2 haplotype items each per dispatch)")
     ap.add_argument("--probe", action="store_true",
-                    help="only the roofline probe: the largest unit alone on the plain path (for rocprofv3)")
+                    help="only the roofline probe: the largest chunk's first targets alone, plain path (for rocprofv3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB three times (HBM counter calibration)")
     ap.add_argument("--graph", type=int, choices=(0, 1, 2), default=1,
@@ -432,21 +445,22 @@ def main():
              for ci, c in enumerate(chunks) for lo in range(0, c["gl"].shape[0], block)]
     for u in units:
         u.run.set_graph(a.graph)
-    n_workers = a.workers or (8 if a.workload == "c5" else len(chunks))
+    n_workers = a.workers or (16 if a.workload == "c5" else len(chunks))
     workspaces = [hip.Workspace() for _ in range(n_workers)]
     tsites = sum(c["gl"].shape[0] * c["L"] for c in chunks)
     setup_s = time.perf_counter() - t_setup
     log("setup %.1f s: %s; %d units, %d in flight" % (setup_s, desc, len(units), n_workers))
 
     if a.probe:
-        u = max(units, key=lambda u: u.work)
+        u = probe_unit(chunks, panels, a.probe_targets)
         set_path([u], "plain", None, 0, 0, chunks)
         u.run.set_graph(0)
         u.run.set_workspace(workspaces[0])
         u.run.execute()
         st = u.run.stats()
         rf = fb_roofline([st])
-        print(json.dumps(dict(probe="largest unit alone, plain path", chunk_sites=u.L, targets=u.hi - u.lo,
+        free_b, total_b = torch.cuda.mem_get_info()
+        print(json.dumps(dict(hbm_used_gb=round((total_b - free_b) / 1e9, 1), probe="largest chunk's first %d targets alone, plain path" % (u.hi - u.lo), chunk_sites=u.L, targets=u.hi - u.lo,
                               fb_state_sites_per_launch=st["fb_state_sites"] / st["fb_launches"], **rf)), flush=True)
         return
 
@@ -502,11 +516,17 @@ def main():
                                                             for p in paths}, units)
         log("large-K judge: %s" % json.dumps(judge))
 
-    # roofline probe: the largest unit alone on the plain path
-    u = max(units, key=lambda u: u.work)
+    # roofline probe: the largest chunk's first targets alone on the plain path,
+    # after the step's runs and workspaces have released their device memory
+    for x in units:
+        x.run.close()
+    for w in workspaces:
+        w.close()
+    u = probe_unit(chunks, panels, a.probe_targets)
     set_path([u], "plain", None, 0, 0, chunks)
     u.run.set_graph(0)
-    u.run.set_workspace(workspaces[0])
+    probe_ws = hip.Workspace()
+    u.run.set_workspace(probe_ws)
     u.run.execute()
     probe = fb_roofline([u.run.stats()])
 
@@ -532,7 +552,7 @@ def main():
         "config": dict(workload=desc, path=main_path, units_in_flight=n_workers, **meta),
         "roofline": {"bound": "hbm", "achieved": probe["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": probe["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "fb_kernel (haploid forward-backward), largest unit alone",
+                     "kernel": "fb_kernel (haploid forward-backward), largest chunk's first %d targets alone" % (u.hi - u.lo),
                      "bytes_per_launch": probe["bytes_per_launch"], "avg_launch_ms": probe["avg_launch_ms"],
                      "step_level_frac": M["step_level_frac"],
                      "valu_issue_frac": profile_value("fb_kernel_%s_probe_valu.md" % wl, "valu_issue_frac"),

@@ -247,19 +247,24 @@ struct EvPair {
 struct gh_workspace {
     gh::DBuf H, HL, HP0, HP1, states, poly, pos, t, mark, hvar, alpha, loci, seg, segsum, imp, impsum, of1, dip, ev,
         evci, ins, isite, fx, fy, mass0, mass1, wlo, whi;
-    // The executions' streams (the second for K-class batching) belong to the
-    // workspace, not the run: a batch of many resident runs has as many
-    // streams in use as workspaces, so runs executing together get distinct
-    // hardware queues (streams beyond GPU_MAX_HW_QUEUES share queues, and a
-    // stream holding a whole run's replayed iterations would block the other).
+    // The executions' streams belong to the workspace, not the run: a batch of
+    // many resident runs has as many streams in use as workspaces, so runs
+    // executing together get distinct hardware queues (streams beyond
+    // GPU_MAX_HW_QUEUES share queues, and a stream holding a whole run's
+    // replayed iterations would block the other).  The second stream (K-class
+    // batching: the other classes beside the largest) is created the first
+    // time a pass has a second class, so runs with one set size take one queue.
     hipStream_t s = nullptr, s2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     void streams() {
         if (s) return;
         GH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        GH_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         GH_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         GH_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
+    hipStream_t side() {
+        if (!s2) GH_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        return s2;
     }
     ~gh_workspace() {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -833,16 +838,16 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
             }
             const int32_t *ord = G->d_ford.as<int32_t>();
             GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
+            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
             GH_HIP(hipEventRecord(ea.a, s));
             fb_launch(P, fbA, fb_d, items_d, fbWA, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
                       massx, wlo, whi, fbp, ord);
             GH_HIP(hipEventRecord(ea.b, s));
-            GH_HIP(hipEventRecord(eb.a, Wk->s2));
-            fb_launch(P, fbB, fb_d, items_d, fbWB, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, Wk->s2,
+            GH_HIP(hipEventRecord(eb.a, Wk->side()));
+            fb_launch(P, fbB, fb_d, items_d, fbWB, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, Wk->side(),
                       massx, wlo, whi, (G->latency || fewB) ? fb_default_pick(fbWB, true) : FbPick{0, 0}, ord + fbA);
-            GH_HIP(hipEventRecord(eb.b, Wk->s2));
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+            GH_HIP(hipEventRecord(eb.b, Wk->side()));
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
             GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         };
         // H0 (caller_algorithm.cpp:58-65)
@@ -979,13 +984,13 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         } else {
             const int32_t *ord = G->d_pord.as<int32_t>();
             GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
+            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
             if (phA > 0)
                 phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
                              (uint32_t)it, status, s, phs, ord);
             phase_launch(P, phB, phKB, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
-                         (uint32_t)it, status, Wk->s2, phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+                         (uint32_t)it, status, Wk->side(), phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
             GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         }
         GH_HIP(hipEventRecord(e_pk.b, s));
@@ -1089,10 +1094,10 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                 return;
             }
             GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
+            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
             fb_key_launch(0, s, HPx);
-            fb_key_launch(1, Wk->s2, HPx);
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+            fb_key_launch(1, Wk->side(), HPx);
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
             GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         };
         auto ph_groups = [&]() {
@@ -1101,10 +1106,10 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                 return;
             }
             GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->s2, Wk->ev_fork, 0));
+            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
             ph_key_launch(0, s);
-            ph_key_launch(1, Wk->s2);
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->s2));
+            ph_key_launch(1, Wk->side());
+            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
             GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
         };
         // one PBWT iteration, every decision on the device
@@ -1152,6 +1157,7 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         // the runtime refuses leaves the stream usable and the iterations are
         // enqueued without it (GO_UNCAPTURED; the same launches and results).
         bool captured = false;
+        if (G->gk_fb[1] >= 0 || G->gk_ph[1] >= 0) (void)Wk->side();   // created outside the capture
         if (G->graph_mode == 1) {
             static std::mutex capture_mu;
             std::lock_guard<std::mutex> lk(capture_mu);
