@@ -55,6 +55,36 @@ __device__ __forceinline__ void wave_sum3(float &x, float &y, float &z) {
     z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(z), 63));
 }
 
+// Sum of v over the 64 lanes, the same value in every lane, in 8 VALU
+// instructions: the four row DPP adds of wave_sum, then the xor-32 and xor-16
+// partners.  A permlane swap of v with itself returns the low half's (even
+// rows') values and the high half's (odd rows') values of each pair, so their
+// sum is the pair's sum in every lane, with the same operands in the same order.
+__device__ __forceinline__ float wave_allsum(float v) {
+    v += dppf<0xB1>(v);
+    v += dppf<0x4E>(v);
+    v += dppf<0x141>(v);
+    v += dppf<0x140>(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+    const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+// Two sums at once: returns the wave total of x in lanes 0-31 and of y in
+// lanes 32-63 (12 VALU instructions).  v_permlane32_swap of (x, y) gives
+// [x low half, y low half] and [x high half, y high half]; their sum holds each
+// value's two halves combined, and the xor-16 step then adds the row pairs.
+__device__ __forceinline__ float wave_sum_pair(float x, float y) {
+    x += dppf<0xB1>(x); y += dppf<0xB1>(y);
+    x += dppf<0x4E>(x); y += dppf<0x4E>(y);
+    x += dppf<0x141>(x); y += dppf<0x141>(y);
+    x += dppf<0x140>(x); y += dppf<0x140>(y);
+    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    const float v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+    const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
 // Reduce-scatter of 8 per-lane values over the wave: returns, in every lane l,
 // the wave total of value (l & 7).  Three DPP halving steps inside each group
 // of 8 lanes (partner 7-i, i^2, i^1; each lane keeps the half its lane bit

@@ -9,8 +9,9 @@
 // transitions, likelihoods) of a 32-site tile is gathered once, one site per
 // lane, and stored in LDS as three 16-byte records per site, which the serial
 // site loop reads with broadcast loads; the tile's Hvar rows are staged in LDS
-// too.  Per-site scaling sums use DPP row reductions, then one LDS exchange
-// (double-buffered by site parity, one barrier) across the NW waves.
+// too.  Per-site sums use DPP row reductions and permlane swaps inside a wave,
+// then one LDS exchange (double-buffered by site parity, one barrier) across
+// the NW waves.
 //
 // Memory: alpha never round-trips through HBM.  The forward pass keeps alpha
 // in registers and writes a checkpoint (the full alpha vector) every C sites;
@@ -23,8 +24,12 @@
 //
 // Arithmetic per state follows the reference: alpha_l = (alpha_{l-1}*nt/S +
 // t/K)*e (one FMA, one multiply), beta~ = beta*nt/S + t/K, the site's own
-// emission divided out of the allele posteriors; only the order of the sums
-// over states differs from the reference's 8-lane AVX2 order.
+// emission divided out of the allele posteriors.  The order of the sums over
+// states differs from the reference's 8-lane AVX2 order, and the posterior
+// sums are accumulated as the total and the allele-1 part with FMAs (the
+// allele-0 part is their difference, never negative because rounding is
+// monotone and both sums take the same order); HP stays within float rounding
+// of the reference's.
 #include "gh_device.hpp"
 #include "gh_internal.hpp"
 
@@ -125,11 +130,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     };
     auto keep = [](int m, float v) { return __int_as_float(m & __float_as_int(v)); };
 
-    // cross-wave sums (fixed order, so the forward and the recomputation agree bit for bit).
-    // The LDS slot alternates with every reduction, so one barrier per reduction suffices.
+    // cross-wave sums in a fixed order (the recomputation takes the forward's
+    // scaling sums from ssum, so only the forward pass reduces them).  The LDS
+    // slot alternates with every reduction, so one barrier per reduction suffices.
     int par = 0;
     auto xsum1 = [&](float v) -> float {
-        v = wave_sum(v);
+        v = wave_allsum(v);
         if (NW == 1) return v;
         par ^= 1;
         if (lane == 0) red[(par * NW + wv) * 4] = v;
@@ -139,22 +145,32 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
         for (int q = 0; q < NW; q++) s += red[(par * NW + q) * 4];
         return s;
     };
-    auto xsum3 = [&](float &x, float &y, float &z) {
-        wave_sum3(x, y, z);
-        if (NW == 1) return;
-        par ^= 1;
-        if (lane == 0) {
-            red[(par * NW + wv) * 4 + 0] = x;
-            red[(par * NW + wv) * 4 + 1] = y;
-            red[(par * NW + wv) * 4 + 2] = z;
+    // the backward site's three sums: the beta sum z (every wave), and the
+    // posterior total x and allele-1 part y, read back only where need_p
+    // (wave 0, which writes HP; every wave when posterior masses are summed)
+    auto xsum_bwd = [&](float &x, float &y, float &z, bool need_p) {
+        z = wave_allsum(z);
+        const float xy = wave_sum_pair(x, y);   // x in lanes 0-31, y in lanes 32-63
+        if (NW == 1) {
+            x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xy), 0));
+            y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xy), 32));
+            return;
         }
+        par ^= 1;
+        float *r = red + (par * NW + wv) * 4;
+        if (lane == 0) { r[0] = z; r[1] = xy; }
+        if (lane == 32) r[2] = xy;
         __syncthreads();
-        x = 0.0f; y = 0.0f; z = 0.0f;
+        z = 0.0f;
 #pragma unroll
-        for (int q = 0; q < NW; q++) {
-            x += red[(par * NW + q) * 4 + 0];
-            y += red[(par * NW + q) * 4 + 1];
-            z += red[(par * NW + q) * 4 + 2];
+        for (int q = 0; q < NW; q++) z += red[(par * NW + q) * 4 + 0];
+        if (need_p) {
+            x = 0.0f; y = 0.0f;
+#pragma unroll
+            for (int q = 0; q < NW; q++) {
+                x += red[(par * NW + q) * 4 + 1];
+                y += red[(par * NW + q) * 4 + 2];
+            }
         }
     };
 
@@ -162,7 +178,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
     // broadcast load each (every lane the same address):
     //   [0] emissions of allele 0 / 1 (1 at flat and LQ sites), t/K and 1-t into the site
     //   [1] t/K and 1-t out of the site, the haplotype likelihoods h0 / h1
-    //   [2] site index, flags (bit 0 flat or LQ, bit 1 flat), unused, unused
+    //   [2] site index, flags (bit 0 flat or LQ, bit 1 flat), 1 / emission 0, 1 / emission 1
     // The next tile is fetched into registers while the current one is processed:
     // stage 1 (site ids, transitions, Hvar rows) right after a tile starts,
     // stage 2 (flags and likelihoods of those sites) once the ids have arrived.
@@ -208,7 +224,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
             smeta[3 * lane + 0] = make_float4(e0, e1, pf_tin / Kf, 1.0f - pf_tin);
             smeta[3 * lane + 1] = make_float4(pf_tout / Kf, 1.0f - pf_tout, h0, h1);
             smeta[3 * lane + 2] = make_float4(__int_as_float(site), __int_as_float((fl ? 1 : 0) | (isflat ? 2 : 0)),
-                                              0.0f, 0.0f);
+                                              1.0f / e0, 1.0f / e1);
         }
 #pragma unroll
         for (int k = 0; k < MWL; k++) {
@@ -388,8 +404,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                 const int code = __builtin_amdgcn_readfirstlane(__float_as_int(mi.y));
                 const bool fl = code & 1, isflat = code & 2;
                 const int site = __float_as_int(mi.x);
-                float p0 = 0.0f, p1 = 0.0f, bs_ = 0.0f;
-                float dpv[MWL];
+                // posterior sums of this wave's states: total pt and allele-1 part p1
+                float pt = 0.0f, p1 = 0.0f, bs_ = 0.0f;
+                float btv[MWL];   // beta~ per slot (1 at the last site), for the posterior masses
                 // row words and recomputed alpha of every slot, read before the slot loop
                 uint32_t rw[MWL];
                 rwords(j, rw);
@@ -404,9 +421,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                         const float av = avs[i];
                         const float bv = (i > 0 ? sel(m, e1, e0) : sel(m, e1p, e0p)) * invK;
                         b[i] = bv;
-                        dpv[i] = av;
+                        btv[i] = 1.0f;
+                        pt += av;
                         p1 += keep(m, av);
-                        p0 += keep(~m, av);
                         bs_ += b[i];
                     }
                 } else {
@@ -419,33 +436,35 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MWL <= 
                             const int m = rmask(rw, i);
                             const float av = avs[i];
                             const float bt = fmaf(b[i], f2, f1);   // lanes past K: b = 0 and alpha = 0
-                            const float dp = av * bt;
-                            dpv[i] = dp;
-                            p1 += keep(m, dp);
-                            p0 += keep(~m, dp);
+                            btv[i] = bt;
+                            pt = fmaf(av, bt, pt);
+                            p1 = fmaf(keep(m, av), bt, p1);
                             b[i] = bt * (i > 0 ? sel(m, e1, e0) : sel(m, e1p, e0p));
                             bs_ += b[i];
                         }
                     });
                 }
-                xsum3(p0, p1, bs_);
+                xsum_bwd(pt, p1, bs_, wv == 0 || A.mass_all != nullptr);
                 if (fl && l == Lp - 1) bs_ = 1.0f;
                 if (A.mass_all) {
-                    const float inv = 1.0f / (p0 + p1);
+                    const float inv = 1.0f / pt;
 #pragma unroll
                     for (int i = 0; i < MWL; i++) {
                         if (i >= nI) continue;
                         const bool in = site >= wlo[i] && site < whi[i];
-                        ms[i] = in ? fmaf(dpv[i], inv, ms[i]) : ms[i];
+                        ms[i] = in ? fmaf(avs[i] * btv[i], inv, ms[i]) : ms[i];
                     }
                 }
-                p0 /= e0;   // exact at flat and LQ sites, where the emissions are 1
-                p1 /= e1;
-                float o0 = p0 * ee + p1 * ed, o1 = p0 * ed + p1 * ee;
-                if (!isflat) { o0 *= mb.z; o1 *= mb.w; }
                 if (tid == 0) {
-                    HP[2 * site] = o0 / (o0 + o1);
-                    HP[2 * site + 1] = o1 / (o0 + o1);
+                    // the site's own emission divided out (the record's reciprocals, 1 at
+                    // flat and LQ sites), the genotyping error applied, the likelihoods
+                    // multiplied in away from flat sites, normalised
+                    const float q0 = (pt - p1) * mi.z, q1 = p1 * mi.w;
+                    float o0 = q0 * ee + q1 * ed, o1 = q0 * ed + q1 * ee;
+                    if (!isflat) { o0 *= mb.z; o1 *= mb.w; }
+                    const float r = __builtin_amdgcn_rcpf(o0 + o1);
+                    HP[2 * site] = o0 * r;
+                    HP[2 * site + 1] = o1 * r;
                 }
                 bsum_next = bs_;
             }
