@@ -461,7 +461,9 @@ def main():
         rf = fb_roofline([st])
         free_b, total_b = torch.cuda.mem_get_info()
         print(json.dumps(dict(hbm_used_gb=round((total_b - free_b) / 1e9, 1), probe="largest chunk's first %d targets alone, plain path" % (u.hi - u.lo), chunk_sites=u.L, targets=u.hi - u.lo,
-                              fb_state_sites_per_launch=st["fb_state_sites"] / st["fb_launches"], **rf)), flush=True)
+                              fb_state_sites_per_launch=st["fb_state_sites"] / st["fb_launches"],
+                              stage_ms={k: round(st[k], 1) for k in ("ms_select", "ms_condset", "ms_fb", "ms_phase")},
+                              **rf)), flush=True)
         return
 
     paths = ["learned", "plain"] if a.path == "both" else [a.path]
