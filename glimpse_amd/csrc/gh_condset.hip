@@ -44,7 +44,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int &total) {
     return before + x - v;
 }
 
-constexpr int COND_NT = 1024;
+// 256 threads (four waves) per item: a workgroup that needs a whole CU's wave
+// slots waits behind the other runs' long-lived workgroups when several runs
+// share the GPU
+constexpr int COND_NT = 256;
 
 __global__ __launch_bounds__(COND_NT) void cond_index_kernel(CondItem *__restrict__ items,
                                                              const int32_t *__restrict__ states, int L,

@@ -35,7 +35,9 @@
 
 namespace gh {
 
-constexpr int PREP_NT = 1024;
+// four waves per target (a workgroup that needs a whole CU's wave slots waits
+// behind the other runs' long-lived workgroups when several runs share the GPU)
+constexpr int PREP_NT = 256;
 
 template <int NT>
 __device__ __forceinline__ void block_scan3(int a, int b, int c, int &ea, int &eb, int &ec, int &ta, int &tb,
