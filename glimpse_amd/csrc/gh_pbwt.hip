@@ -51,11 +51,15 @@ namespace gh {
 constexpr int WALK_WALKERS = 64;
 constexpr int WALK_LOADERS = 192;
 constexpr int WALK_NT = WALK_WALKERS + WALK_LOADERS;
-constexpr int TILE_WORDS = 4096;
+// The tile sizes keep a walk workgroup at 80 KB of LDS (two buffers), so it
+// shares a CU with forward-backward and phasing workgroups of other runs: with
+// 4096-word tiles (147 KB) it could start only on an otherwise idle CU and kept
+// that CU's compute workgroups off it for the whole walk.
+constexpr int TILE_WORDS = 2048;
 constexpr int TILE_OPS = 64;
 // carrier positions (A_small_idx) of the tile's init ops, staged so the walk's
 // lower-bound scans over them read LDS (at large N these sets hold hundreds)
-constexpr int TILE_SI = 4096;
+constexpr int TILE_SI = 2048;
 constexpr size_t WALK_BUF = (size_t)TILE_WORDS * 12 + sizeof(SiteOp) * TILE_OPS + (size_t)TILE_OPS * WALK_WALKERS +
                             (size_t)TILE_SI * 4;
 constexpr size_t WALK_LDS = 2 * WALK_BUF;
@@ -354,7 +358,7 @@ void pbwt_decode(gh_panel *P) {
     D->n_runs = n_runs;
     // LDS tiles for the walk: greedy runs of at most TILE_OPS ops whose rank
     // data fits TILE_WORDS.  A tile holds the rare ops' columns, and the common
-    // sites' columns too when one is small (N <= 32,768); larger common columns
+    // sites' columns too when one is small (N <= 16,320: eight columns to a tile); larger common columns
     // stay in global memory and the walkers look them up there (a tile of two
     // 100,000-haplotype columns would cost a full copy per two sites).
     D->total_words = (int64_t)bits.size();
