@@ -8,11 +8,13 @@
 namespace gh {
 
 constexpr int MLP_FEAT = MLP_IN;
-constexpr int FEAT_NT = 512;                  // 8 waves, one candidate per wave at a time
+// 4-wave workgroups (one candidate per wave at a time): larger ones wait for a
+// whole CU's wave slots behind other runs' workgroups when runs share the GPU
+constexpr int FEAT_NT = 256;
 constexpr int FEAT_WIN = 8;                   // windows of the per-window mismatch features
 constexpr int FEAT_CONF_PHRED = 10;           // a genotype is "confident" when both others are >= 10 phred
 constexpr size_t FEAT_LDS_MAX = 144 * 1024;
-constexpr int TOPK_NT = 1024;
+constexpr int TOPK_NT = 256;
 constexpr int TOPK_MAX = 2048;                // largest pool (conditioning sets are capped at 2048)
 constexpr float LABEL_EPS = 1e-3f;            // label = ln(K * mass + LABEL_EPS)
 
