@@ -364,8 +364,11 @@ void mlp_forward_launch(const float *w, const uint16_t *wb, const float *X, int6
     }();
     (void)attr;
     if (n <= 0) return;
+    // at most 256 workgroups, each loading the weights once and looping over
+    // tiles: a workgroup holds 69 KB of LDS, and one per tile waited for a CU
+    // with that much room behind the other runs' workgroups
     const int64_t tiles = (n + TR - 1) / TR;
-    const int grid = (int)std::min<int64_t>(tiles, 2048);
+    const int grid = (int)std::min<int64_t>(tiles, 256);
     hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid), dim3(MLP_NT), sizeof(FwdLds), s, w, wb, X, n, y);
     GH_HIP(hipGetLastError());
 }
