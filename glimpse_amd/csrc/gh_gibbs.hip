@@ -244,6 +244,9 @@ struct EvPair {
 // records, selector rows).  Buffers grow to the largest execution they serve
 // and are kept; runs executed one after another can share one workspace, so
 // only the executing runs hold this memory.
+// workspaces whose run stream exists (executions may use a second stream each)
+static std::atomic<int> g_live_workspaces{0};
+
 struct gh_workspace {
     gh::DBuf H, HL, HP0, HP1, states, poly, pos, t, mark, hvar, alpha, loci, seg, segsum, imp, impsum, of1, dip, ev,
         evci, ins, isite, fx, fy, mass0, mass1, wlo, whi;
@@ -252,13 +255,14 @@ struct gh_workspace {
     // executing together get distinct hardware queues (streams beyond
     // GPU_MAX_HW_QUEUES share queues, and a stream holding a whole run's
     // replayed iterations would block the other).  The second stream (K-class
-    // batching: the other classes beside the largest) is created the first
-    // time a pass has a second class, so runs with one set size take one queue.
+    // batching: the other classes beside the largest, while the streams fit the
+    // hardware queues) is created the first time such a pass has a second class.
     hipStream_t s = nullptr, s2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     void streams() {
         if (s) return;
         GH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        g_live_workspaces.fetch_add(1);
         GH_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         GH_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
@@ -270,7 +274,10 @@ struct gh_workspace {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (s2) (void)hipStreamDestroy(s2);
-        if (s) (void)hipStreamDestroy(s);
+        if (s) {
+            (void)hipStreamDestroy(s);
+            g_live_workspaces.fetch_sub(1);
+        }
     }
 };
 
@@ -530,6 +537,25 @@ void run_create(gh_panel *P, int T, const uint8_t *gl_h, const uint8_t *flat_h, 
 // replayed for every later iteration (gh_graph.hpp).  Returns false when a
 // replayed iteration needed more placed memory than the captured buffers hold
 // (its results are then invalid and the caller executes again without the graph).
+// Whether the second set-size class of a pass goes beside the first on the
+// workspace's second stream: while every live workspace's two streams fit the
+// process's hardware queues (GPU_MAX_HW_QUEUES, HIP's default 4); beyond that,
+// streams share queues and a queue holding one run's replayed iterations blocks
+// the other run's.  GH_SIDE_STREAM=0 / 1 forces either.
+static bool side_stream_enabled() {
+    static const int forced = [] {
+        const char *e = std::getenv("GH_SIDE_STREAM");
+        return e ? (std::atoi(e) != 0 ? 1 : 0) : -1;
+    }();
+    static const int queues = [] {
+        const char *e = std::getenv("GPU_MAX_HW_QUEUES");
+        const int q = e ? std::atoi(e) : 0;
+        return q > 0 ? q : 4;
+    }();
+    if (forced >= 0) return forced == 1;
+    return 2 * g_live_workspaces.load() <= queues;
+}
+
 static bool run_execute_once(gh_run *G, bool use_graph) {
     const auto wall0 = std::chrono::steady_clock::now();
     gh_panel *P = G->P;
@@ -539,6 +565,23 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
     gh_workspace *Wk = G->W;
     Wk->streams();
     hipStream_t s = Wk->s;
+    // The second set-size class of a pass (K-class batching) runs beside the
+    // first on the workspace's second stream when the streams fit the hardware
+    // queues, else after the first on the run's stream (runs executing together
+    // then supply the concurrency: sixteen C5 runs whose kept sets fell in a
+    // smaller class than the fallbacks ran 2.3 times faster so).
+    const bool side_on = side_stream_enabled();
+    auto fork = [&]() -> hipStream_t {
+        if (!side_on) return s;
+        GH_HIP(hipEventRecord(Wk->ev_fork, s));
+        GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
+        return Wk->side();
+    };
+    auto join = [&]() {
+        if (!side_on) return;
+        GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
+        GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
+    };
     const int kst = G->kstride, prs = G->pstride;
     const bool learned = G->mlp != nullptr, collecting = G->collect_from >= 1;
     GH_CHECK(!(learned && collecting), GH_ERR_ARG, "a run either records training rows or uses the learned selector");
@@ -825,9 +868,9 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         const int32_t *wlo = label_it ? Wk->wlo.as<int32_t>() : nullptr, *whi = label_it ? Wk->whi.as<int32_t>() : nullptr;
         GH_HIP(hipEventRecord(e_cond.b, s));
         GH_HIP(hipEventRecord(e_fb.a, s));
-        // one forward-backward pass over every item: the top class on s, the rest
-        // beside it on s2 (sized for its own class), joined before s goes on
-        // (each kernel dispatch is timed by its own event pair)
+        // one forward-backward pass over every item: the top class, then the rest
+        // (sized for its own class) beside it on the second stream or after it on
+        // the run's stream (each dispatch timed by its own event pair)
         auto fb_all = [&](float *HPx, float *massx, EvPair &ea, EvPair &eb) {
             if (fbB == 0) {
                 GH_HIP(hipEventRecord(ea.a, s));
@@ -837,18 +880,16 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                 return;
             }
             const int32_t *ord = G->d_ford.as<int32_t>();
-            GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
+            const hipStream_t sB = fork();
             GH_HIP(hipEventRecord(ea.a, s));
             fb_launch(P, fbA, fb_d, items_d, fbWA, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, s,
                       massx, wlo, whi, fbp, ord);
             GH_HIP(hipEventRecord(ea.b, s));
-            GH_HIP(hipEventRecord(eb.a, Wk->side()));
-            fb_launch(P, fbB, fb_d, items_d, fbWB, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, Wk->side(),
+            GH_HIP(hipEventRecord(eb.a, sB));
+            fb_launch(P, fbB, fb_d, items_d, fbWB, G->maxW, poly, tr, hvar, HL, flat_d, alpha, HPx, G->ee_i, G->ed_i, sB,
                       massx, wlo, whi, (G->latency || fewB) ? fb_default_pick(fbWB, true) : FbPick{0, 0}, ord + fbA);
-            GH_HIP(hipEventRecord(eb.b, Wk->side()));
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
-            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
+            GH_HIP(hipEventRecord(eb.b, sB));
+            join();
         };
         // H0 (caller_algorithm.cpp:58-65)
         hipLaunchKernelGGL(hl_kernel, gsite, bsite, 0, s, gl_d, flat_d, H, L, T, stage == 0 ? 0 : 1, 0, prm->min_gl,
@@ -983,15 +1024,13 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                              (uint32_t)it, status, s, phs, G->d_pord.as<int32_t>());
         } else {
             const int32_t *ord = G->d_pord.as<int32_t>();
-            GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
+            const hipStream_t sB = fork();
             if (phA > 0)
                 phase_launch(P, phA, maxK, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
                              (uint32_t)it, status, s, phs, ord);
             phase_launch(P, phB, phKB, ph_d, items_d, hvar, loci, pos, prm->ne, PB, H, H, prm->err_phase, seed,
-                         (uint32_t)it, status, Wk->side(), phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
-            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
+                         (uint32_t)it, status, sB, phase_shape(phKB, G->latency || phB <= FEW_ITEMS), ord + phA);
+            join();
         }
         GH_HIP(hipEventRecord(e_pk.b, s));
         GH_HIP(hipEventRecord(e_ph.b, s));
@@ -1093,24 +1132,20 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
                 fb_key_launch(0, s, HPx);
                 return;
             }
-            GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
+            const hipStream_t sB = fork();
             fb_key_launch(0, s, HPx);
-            fb_key_launch(1, Wk->side(), HPx);
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
-            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
+            fb_key_launch(1, sB, HPx);
+            join();
         };
         auto ph_groups = [&]() {
             if (G->gk_ph[1] < 0) {
                 ph_key_launch(0, s);
                 return;
             }
-            GH_HIP(hipEventRecord(Wk->ev_fork, s));
-            GH_HIP(hipStreamWaitEvent(Wk->side(), Wk->ev_fork, 0));
+            const hipStream_t sB = fork();
             ph_key_launch(0, s);
-            ph_key_launch(1, Wk->side());
-            GH_HIP(hipEventRecord(Wk->ev_join, Wk->side()));
-            GH_HIP(hipStreamWaitEvent(s, Wk->ev_join, 0));
+            ph_key_launch(1, sB);
+            join();
         };
         // one PBWT iteration, every decision on the device
         auto iteration = [&]() {
@@ -1157,7 +1192,7 @@ static bool run_execute_once(gh_run *G, bool use_graph) {
         // the runtime refuses leaves the stream usable and the iterations are
         // enqueued without it (GO_UNCAPTURED; the same launches and results).
         bool captured = false;
-        if (G->gk_fb[1] >= 0 || G->gk_ph[1] >= 0) (void)Wk->side();   // created outside the capture
+        if (side_on && (G->gk_fb[1] >= 0 || G->gk_ph[1] >= 0)) (void)Wk->side();   // created outside the capture
         if (G->graph_mode == 1) {
             static std::mutex capture_mu;
             std::lock_guard<std::mutex> lk(capture_mu);
